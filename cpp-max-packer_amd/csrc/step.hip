@@ -1,0 +1,569 @@
+// Batched placement step (gap_fill.cpp:363-509 semantics) on gfx950.
+//
+// One step = environment forward FFT, every orientation's pruned kernel
+// spectrum, the product and inverse passes, rounding + checks + feasible mask
+// + lowest (z, y, x) choice, and stamping the chosen kernel.  Every kernel of a
+// step runs on one stream; with capture_graph the sequence is recorded once as
+// a hipGraph and replayed per step.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "common.hpp"
+#include "fft_kernels.hpp"
+
+namespace sphip {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr std::size_t kLdsBudget = 64 * 1024;
+
+// Largest tile width (power of two, <= cap) whose LDS footprint fits.
+int choose_w(int n, std::size_t elem, int buffers, bool pairs, int cap, int min_w)
+{
+    int best = min_w;
+    for (int w = min_w; w <= cap; w *= 2) {
+        const int pencils = pairs ? w / 2 : w;
+        const std::size_t bytes = elem * (std::size_t(n) + std::size_t(buffers) * n * (pencils + 1));
+        if (bytes <= kLdsBudget) best = w;
+    }
+    return best;
+}
+
+template <typename T> std::size_t lds_bytes(int n, int buffers, int pencils)
+{
+    return sizeof(cx<T>) * (std::size_t(n) + std::size_t(buffers) * n * (pencils + 1));
+}
+
+__global__ void k_reset_result(StepDeviceResult* r)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        r->key = ~0ULL;
+        r->flags = 0;
+        r->max_residual_bits = 0;
+    }
+}
+
+// ORs the chosen kernel into the occupancy (gap_fill.cpp:98-121 for the chosen
+// pose) and adds the newly set cells to the environment mass.
+__global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsigned char* __restrict__ kcells,
+                        const OrientDesc* __restrict__ descs, StepDeviceResult* r)
+{
+    const unsigned long long key = r->key;
+    if (key == ~0ULL) return;
+    const int o = int(key & 0xffffULL);
+    const int ux = int((key >> 16) & 0xffffULL), uy = int((key >> 32) & 0xffffULL), uz = int(key >> 48);
+    const OrientDesc d = descs[o];
+    const long long n = (long long)d.kx * d.ky * d.kz;
+    unsigned long long added = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (!kcells[d.cells_offset + i]) continue;
+        const int x = int(i % d.kx), y = int((i / d.kx) % d.ky), z = int(i / ((long long)d.kx * d.ky));
+        const int gx = ux + x, gy = uy + y, gz = uz + z;
+        if (gx < 0 || gy < 0 || gz < 0 || gx >= ex || gy >= ey || gz >= ez) continue;
+        unsigned char* c = env + gx + (long long)ex * (gy + (long long)ey * gz);
+        if (*c == 0) {
+            *c = 1;
+            ++added;
+        }
+    }
+    if (added) atomicAdd(&r->env_mass, added);
+}
+
+}  // namespace
+
+class Session {
+public:
+    Session(Context& ctx, const sp_session_desc& d) : ctx_(ctx), desc_(d)
+    {
+        for (int a = 0; a != 3; ++a) {
+            ext_[a] = int(d.environment_shape[a]);
+            efirst_[a] = d.environment_first[a];
+            const int g = good_length(ext_[a]);
+            if (g == 0) throw DeviceError { "FFT_LENGTH" };
+            n_[a] = g;
+        }
+        O_ = int(d.orientations);
+        kernels_.resize(O_);
+        descs_.resize(O_);
+        use_device(ctx_);
+        SP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        env_.allocate(std::size_t(ext_[0]) * ext_[1] * ext_[2]);
+        SP_HIP_CHECK(hipMemsetAsync(env_.get(), 0, env_.bytes(), stream_));
+        result_.allocate(sizeof(StepDeviceResult));
+        StepDeviceResult r0 {};
+        r0.key = ~0ULL;
+        SP_HIP_CHECK(hipMemcpyAsync(result_.get(), &r0, sizeof r0, hipMemcpyHostToDevice, stream_));
+        SP_HIP_CHECK(hipHostMalloc(&host_result_, sizeof(StepDeviceResult), hipHostMallocDefault));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+    ~Session()
+    {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        if (host_result_) (void)hipHostFree(host_result_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    void set_environment(const std::uint8_t* cells, std::uint64_t count)
+    {
+        if (count != std::uint64_t(ext_[0]) * ext_[1] * ext_[2]) throw std::invalid_argument("environment size");
+        for (std::uint64_t i = 0; i != count; ++i)
+            if (cells[i] > 1) throw std::domain_error("CORRELATION_INVALID_ENVIRONMENT");
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(env_.get(), cells, count, hipMemcpyHostToDevice, stream_));
+        std::uint64_t mass = 0;
+        for (std::uint64_t i = 0; i != count; ++i) mass += cells[i];
+        StepDeviceResult r0 {};
+        r0.key = ~0ULL;
+        r0.env_mass = mass;
+        SP_HIP_CHECK(hipMemcpyAsync(result_.get(), &r0, sizeof r0, hipMemcpyHostToDevice, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    void get_environment(std::uint8_t* cells, std::uint64_t count)
+    {
+        if (count != std::uint64_t(ext_[0]) * ext_[1] * ext_[2]) throw std::invalid_argument("environment size");
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(cells, env_.get(), count, hipMemcpyDeviceToHost, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    void set_kernel(std::uint32_t o, const std::uint8_t* cells, const std::uint32_t shape[3], const std::int64_t first[3])
+    {
+        if (int(o) >= O_) throw std::invalid_argument("orientation index");
+        Kernel& k = kernels_[o];
+        std::uint64_t n = 1;
+        for (int a = 0; a != 3; ++a) {
+            if (shape[a] == 0) throw std::invalid_argument("kernel shape");
+            k.shape[a] = int(shape[a]);
+            k.first[a] = first[a];
+            n *= shape[a];
+        }
+        k.cells.assign(cells, cells + n);
+        k.mass = 0;
+        for (auto c : k.cells) {
+            if (c > 1) throw std::domain_error("CORRELATION_INVALID_KERNEL");
+            k.mass += c;
+        }
+        k.set = true;
+        prepared_ = false;
+    }
+
+    void keep_fields(bool keep)
+    {
+        if (keep != keep_fields_) {
+            keep_fields_ = keep;
+            prepared_ = false;
+        }
+    }
+
+    // Precision chosen for the session: FP32 unless its a-priori bound, with
+    // the environment mass at its worst case (every cell), exceeds 0.4.
+    int precision() const { return precision_; }
+    double bound() const { return bound_; }
+
+    void prepare()
+    {
+        if (prepared_) return;
+        for (const auto& k : kernels_)
+            if (!k.set) throw std::invalid_argument("kernel not set");
+        std::uint64_t max_mass = 0;
+        for (const auto& k : kernels_) max_mass = std::max(max_mass, k.mass);
+        const std::uint64_t cells = std::uint64_t(n_[0]) * n_[1] * n_[2];
+        bound_ = fp32_error_bound(cells, cells, max_mass);
+        precision_ = desc_.precision == SP_PRECISION_FP64 ? 64 : desc_.precision == SP_PRECISION_FP32 ? 32
+                     : (bound_ > 0.4 ? 64 : 32);
+        // descriptors and concatenated kernels
+        std::vector<std::uint8_t> all;
+        kmax_ = { 1, 1, 1 };
+        lmax_ = { 0, 0, 0 };
+        field_offsets_.assign(O_, 0);
+        long long foff = 0;
+        for (int o = 0; o != O_; ++o) {
+            const auto& k = kernels_[o];
+            OrientDesc& d = descs_[o];
+            d.kx = k.shape[0];
+            d.ky = k.shape[1];
+            d.kz = k.shape[2];
+            d.lx = ext_[0] - k.shape[0] + 1;
+            d.ly = ext_[1] - k.shape[1] + 1;
+            d.lz = ext_[2] - k.shape[2] + 1;
+            if (d.lx <= 0 || d.ly <= 0 || d.lz <= 0) d.lx = d.ly = d.lz = 0;  // does not fit (gap_fill.cpp:382-390)
+            d.cells_offset = (long long)all.size();
+            d.mass = (long long)k.mass;
+            all.insert(all.end(), k.cells.begin(), k.cells.end());
+            if (d.lx > 0)
+                for (int a = 0; a != 3; ++a) kmax_[a] = std::max(kmax_[a], k.shape[a]);
+            lmax_[0] = std::max(lmax_[0], d.lx);
+            lmax_[1] = std::max(lmax_[1], d.ly);
+            lmax_[2] = std::max(lmax_[2], d.lz);
+            field_offsets_[o] = foff;
+            foff += (long long)d.lx * d.ly * d.lz;
+        }
+        if (kmax_[0] > n_[0] || kmax_[1] > n_[1] || kmax_[2] > n_[2]) {
+            // pruned kernel buffers assume the kernel fits the FFT grid
+            for (int a = 0; a != 3; ++a) kmax_[a] = std::min(kmax_[a], n_[a]);
+        }
+        use_device(ctx_);
+        kcells_.allocate(std::max<std::size_t>(all.size(), 1));
+        SP_HIP_CHECK(hipMemcpyAsync(kcells_.get(), all.data(), all.size(), hipMemcpyHostToDevice, stream_));
+        d_descs_.allocate(sizeof(OrientDesc) * O_);
+        SP_HIP_CHECK(hipMemcpyAsync(d_descs_.get(), descs_.data(), sizeof(OrientDesc) * O_, hipMemcpyHostToDevice, stream_));
+        d_offsets_.allocate(sizeof(long long) * O_);
+        SP_HIP_CHECK(hipMemcpyAsync(d_offsets_.get(), field_offsets_.data(), sizeof(long long) * O_,
+                                    hipMemcpyHostToDevice, stream_));
+        if (keep_fields_) fields_.allocate(std::max<long long>(foff, 1) * sizeof(int));
+        else fields_.release();
+        fields_total_ = foff;
+        const std::size_t elem = precision_ == 64 ? sizeof(double2) : sizeof(float2);
+        const std::size_t nh = std::size_t(n_[2] / 2 + 1);
+        sb_.allocate(elem * nh * n_[1] * n_[0]);
+        h1_.allocate(elem * O_ * nh * kmax_[1] * kmax_[0]);
+        h2_.allocate(elem * O_ * nh * n_[1] * kmax_[0]);
+        t_.allocate(elem * O_ * nh * n_[1] * n_[0]);
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        for (int a = 0; a != 3; ++a) {
+            if (!make_plan(n_[a], plans_[a])) throw DeviceError { "FFT_LENGTH" };
+            // twiddle tables are allocated here, never inside a graph capture
+            if (precision_ == 64) (void)twiddles<double>(n_[a]);
+            else (void)twiddles<float>(n_[a]);
+        }
+        if (graph_exec_) {
+            (void)hipGraphExecDestroy(graph_exec_);
+            graph_exec_ = nullptr;
+        }
+        if (graph_) {
+            (void)hipGraphDestroy(graph_);
+            graph_ = nullptr;
+        }
+        prepared_ = true;
+    }
+
+    template <typename T> void enqueue_step(bool stamp)
+    {
+        using T2 = cx<T>;
+        const int nx = n_[0], ny = n_[1], nz = n_[2];
+        const int nh = nz / 2 + 1;
+        const T2* twx = twiddles<T>(nx);
+        const T2* twy = twiddles<T>(ny);
+        const T2* twz = twiddles<T>(nz);
+        T2* SB = sb_.as<T2>();
+        T2* H1 = h1_.as<T2>();
+        T2* H2 = h2_.as<T2>();
+        T2* TT = t_.as<T2>();
+        const OrientDesc* dd = d_descs_.as<OrientDesc>();
+        StepDeviceResult* res = result_.as<StepDeviceResult>();
+        const std::size_t e = sizeof(T2);
+        hipLaunchKernelGGL(k_reset_result, dim3(1), dim3(64), 0, stream_, res);
+        // E1: environment R2C along z (uint8 -> spectrum planes)
+        {
+            const int W = choose_w(nz, e, 2, true, 32, 2);
+            hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((nx + W - 1) / W, ny, 1), dim3(kThreads),
+                               lds_bytes<T>(nz, 2, W / 2), stream_, env_.as<unsigned char>(), 0LL, ext_[0], ext_[1],
+                               ext_[0], ext_[1], ext_[2], SB, 0LL, nx, ny, nx, ny, nullptr, twz, plans_[2], W);
+        }
+        // E2: C2C along y (columns x, batch kz)
+        {
+            const int W = choose_w(ny, e, 2, false, 16, 1);
+            hipLaunchKernelGGL((k_c2c<T>), dim3((nx + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(ny, 2, W),
+                               stream_, SB, 0LL, (long long)nx, 1LL, (long long)nx * ny, SB, 0LL, (long long)nx, 1LL,
+                               (long long)nx * ny, nx, ny, ny, nullptr, 0, twy, plans_[1], W, 1);
+        }
+        // E3: C2C along x (rows, batch kz)
+        {
+            const int W = choose_w(nx, e, 2, false, 16, 1);
+            hipLaunchKernelGGL((k_c2c<T>), dim3((ny + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(nx, 2, W),
+                               stream_, SB, 0LL, 1LL, (long long)nx, (long long)nx * ny, SB, 0LL, 1LL, (long long)nx,
+                               (long long)nx * ny, ny, nx, nx, nullptr, 0, twx, plans_[0], W, 1);
+        }
+        const long long h1o = (long long)nh * kmax_[1] * kmax_[0];
+        const long long h2o = (long long)nh * ny * kmax_[0];
+        const long long to = (long long)nh * ny * nx;
+        // G1: kernel R2C along z over its own columns (x < kx, y < ky)
+        {
+            const int W = choose_w(nz, e, 2, true, 32, 2);
+            hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((kmax_[0] + W - 1) / W, kmax_[1], O_), dim3(kThreads),
+                               lds_bytes<T>(nz, 2, W / 2), stream_, kcells_.as<unsigned char>(), 0LL, 0, 0, 0, 0, 0, H1,
+                               h1o, kmax_[0], kmax_[1], 0, 0, dd, twz, plans_[2], W);
+        }
+        // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
+        {
+            const int W = choose_w(ny, e, 2, false, 16, 1);
+            hipLaunchKernelGGL((k_c2c<T>), dim3((kmax_[0] + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(ny, 2, W),
+                               stream_, H1, h1o, (long long)kmax_[0], 1LL, (long long)kmax_[1] * kmax_[0], H2, h2o,
+                               (long long)kmax_[0], 1LL, (long long)ny * kmax_[0], 0, 0, ny, dd, 1, twy, plans_[1], W, 1);
+        }
+        // P1: finish kernel x transform, product with conj, inverse x
+        {
+            const int W = choose_w(nx, e, 4, false, 16, 1);
+            hipLaunchKernelGGL((k_mul_x<T>), dim3((ny + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(nx, 4, W),
+                               stream_, H2, h2o, kmax_[0], ny, SB, nx, ny, TT, to, dd, 0, 0, twx, plans_[0], W);
+        }
+        // P2: inverse C2C along y for the legal columns, rows y < ly written
+        if (lmax_[0] > 0) {
+            const int W = choose_w(ny, e, 2, false, 16, 1);
+            hipLaunchKernelGGL((k_c2c<T>), dim3((lmax_[0] + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(ny, 2, W),
+                               stream_, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to, (long long)nx, 1LL,
+                               (long long)nx * ny, 0, ny, 0, dd, 2, twy, plans_[1], W, 0);
+            // P3: C2R along z + round + checks + feasible + lowest (z, y, x)
+            const int W3 = choose_w(nz, e, 2, true, 32, 2);
+            const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
+            hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], O_), dim3(kThreads),
+                               lds_bytes<T>(nz, 2, W3 / 2), stream_, TT, to, nx, ny, dd, inv_total, res,
+                               keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(), twz, plans_[2], W3);
+        }
+        if (stamp) {
+            hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
+                               ext_[2], kcells_.as<unsigned char>(), dd, res);
+        }
+    }
+
+    void enqueue(bool stamp)
+    {
+        if (precision_ == 64) enqueue_step<double>(stamp);
+        else enqueue_step<float>(stamp);
+    }
+
+    void launch(bool stamp)
+    {
+        prepare();
+        use_device(ctx_);
+        if (!desc_.capture_graph) {
+            enqueue(stamp);
+            SP_HIP_CHECK(hipGetLastError());
+            return;
+        }
+        if (!graph_exec_ || graph_stamp_ != stamp) {
+            if (graph_exec_) {
+                (void)hipGraphExecDestroy(graph_exec_);
+                graph_exec_ = nullptr;
+            }
+            if (graph_) {
+                (void)hipGraphDestroy(graph_);
+                graph_ = nullptr;
+            }
+            SP_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            enqueue(stamp);
+            SP_HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
+            SP_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+            graph_stamp_ = stamp;
+        }
+        SP_HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+    }
+
+    void read_result(sp_step_result& out)
+    {
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(host_result_, result_.get(), sizeof(StepDeviceResult), hipMemcpyDeviceToHost, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        const StepDeviceResult r = *host_result_;
+        out = sp_step_result {};
+        out.precision = precision_;
+        out.error_bound = bound_;
+        out.numeric_flags = r.flags;
+        float res;
+        std::memcpy(&res, &r.max_residual_bits, sizeof res);
+        out.max_residual = res;
+        if (r.key == ~0ULL) {
+            out.orientation = -1;
+            return;
+        }
+        const int o = int(r.key & 0xffffULL);
+        const std::int64_t u[3] = { std::int64_t((r.key >> 16) & 0xffffULL), std::int64_t((r.key >> 32) & 0xffffULL),
+                                     std::int64_t(r.key >> 48) };
+        out.orientation = o;
+        for (int a = 0; a != 3; ++a) {
+            // key = t + kernel_first = environment_first + u; t = key - kernel_first
+            out.key[a] = efirst_[a] + u[a];
+            out.translation[a] = out.key[a] - kernels_[o].first[a];
+        }
+    }
+
+    void overlap_field(std::uint32_t o, std::int32_t* out, std::uint64_t count, std::uint32_t legal[3])
+    {
+        if (!keep_fields_ || !prepared_) throw std::invalid_argument("fields not kept");
+        if (int(o) >= O_) throw std::invalid_argument("orientation index");
+        const OrientDesc& d = descs_[o];
+        legal[0] = std::uint32_t(std::max(d.lx, 0));
+        legal[1] = std::uint32_t(std::max(d.ly, 0));
+        legal[2] = std::uint32_t(std::max(d.lz, 0));
+        const std::uint64_t n = std::uint64_t(legal[0]) * legal[1] * legal[2];
+        if (count < n) throw std::length_error("capacity");
+        if (n == 0) return;
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(out, fields_.as<int>() + field_offsets_[o], n * sizeof(int), hipMemcpyDeviceToHost,
+                                    stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    hipStream_t stream() const { return stream_; }
+    std::uint64_t device_bytes() const
+    {
+        return env_.bytes() + kcells_.bytes() + sb_.bytes() + h1_.bytes() + h2_.bytes() + t_.bytes() + fields_.bytes();
+    }
+
+private:
+    struct Kernel {
+        int shape[3] {};
+        std::int64_t first[3] {};
+        std::vector<std::uint8_t> cells;
+        std::uint64_t mass {};
+        bool set {};
+    };
+    Context& ctx_;
+    sp_session_desc desc_;
+    int ext_[3] {}, n_[3] {};
+    std::int64_t efirst_[3] {};
+    int O_ {};
+    std::vector<Kernel> kernels_;
+    std::vector<OrientDesc> descs_;
+    std::vector<long long> field_offsets_;
+    std::array<int, 3> kmax_ {}, lmax_ {};
+    FftPlan plans_[3] {};
+    hipStream_t stream_ {};
+    DeviceBuffer env_, result_, kcells_, d_descs_, d_offsets_, fields_, sb_, h1_, h2_, t_;
+    StepDeviceResult* host_result_ {};
+    long long fields_total_ {};
+    bool keep_fields_ {}, prepared_ {};
+    int precision_ { 32 };
+    double bound_ {};
+    hipGraph_t graph_ {};
+    hipGraphExec_t graph_exec_ {};
+    bool graph_stamp_ {};
+};
+
+}  // namespace sphip
+
+struct sp_hip_session {
+    sphip::Session* s {};
+    sp_hip_ctx* ctx {};
+};
+
+using sphip::DeviceError;
+
+namespace {
+int fail(sp_hip_ctx* ctx, int status, const std::string& msg)
+{
+    if (ctx) ctx->c.last_error = msg;
+    return status;
+}
+
+template <typename F> int guarded(sp_hip_ctx* ctx, F&& f)
+{
+    try {
+        f();
+        return SP_OK;
+    }
+    catch (const DeviceError& e) {
+        if (e.what == "FFT_LENGTH") return fail(ctx, SP_FFT_LENGTH_UNSUPPORTED, "padded FFT length unsupported");
+        return fail(ctx, SP_DEVICE_ERROR, e.what);
+    }
+    catch (const std::domain_error& e) {
+        const std::string w = e.what();
+        if (w == "CORRELATION_INVALID_ENVIRONMENT") return fail(ctx, SP_CORRELATION_INVALID_ENVIRONMENT, w);
+        if (w == "CORRELATION_INVALID_KERNEL") return fail(ctx, SP_CORRELATION_INVALID_KERNEL, w);
+        return fail(ctx, SP_INVALID_ARGUMENT, w);
+    }
+    catch (const std::length_error& e) {
+        return fail(ctx, SP_CAPACITY, e.what());
+    }
+    catch (const std::bad_alloc&) {
+        return fail(ctx, SP_CORRELATION_ALLOCATION, "host allocation failed");
+    }
+    catch (const std::exception& e) {
+        return fail(ctx, SP_INVALID_ARGUMENT, e.what());
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int sp_hip_session_create(sp_hip_ctx* ctx, const sp_session_desc* desc, sp_hip_session** out)
+{
+    if (!ctx || !desc || !out || desc->orientations == 0 || desc->orientations > 65535) return SP_INVALID_ARGUMENT;
+    for (int a = 0; a != 3; ++a)
+        if (desc->environment_shape[a] == 0 || desc->environment_shape[a] > 65535) return SP_INVALID_ARGUMENT;
+    return guarded(ctx, [&] {
+        auto* s = new sp_hip_session;
+        s->ctx = ctx;
+        try {
+            s->s = new sphip::Session(ctx->c, *desc);
+        }
+        catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+void sp_hip_session_destroy(sp_hip_session* s)
+{
+    if (!s) return;
+    delete s->s;
+    delete s;
+}
+
+int sp_hip_session_set_environment(sp_hip_session* s, const uint8_t* cells, uint64_t count)
+{
+    if (!s || !cells) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_environment(cells, count); });
+}
+
+int sp_hip_session_get_environment(sp_hip_session* s, uint8_t* cells, uint64_t count)
+{
+    if (!s || !cells) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->get_environment(cells, count); });
+}
+
+int sp_hip_session_set_kernel(sp_hip_session* s, uint32_t o, const uint8_t* cells, const uint32_t shape[3],
+                              const int64_t first[3])
+{
+    if (!s || !cells || !shape || !first) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_kernel(o, cells, shape, first); });
+}
+
+int sp_hip_session_keep_fields(sp_hip_session* s, int keep)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->keep_fields(keep != 0); });
+}
+
+int sp_hip_session_step(sp_hip_session* s, int stamp, sp_step_result* out)
+{
+    if (!s || !out) return SP_INVALID_ARGUMENT;
+    int rc = guarded(s->ctx, [&] {
+        s->s->launch(stamp != 0);
+        s->s->read_result(*out);
+    });
+    if (rc == SP_OK && (out->numeric_flags & 3u)) return fail(s->ctx, SP_CORRELATION_NUMERIC, "step numeric check failed");
+    return rc;
+}
+
+int sp_hip_session_run(sp_hip_session* s, uint32_t steps, int stamp)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        for (uint32_t i = 0; i != steps; ++i) s->s->launch(stamp != 0);
+    });
+}
+
+int sp_hip_session_last_result(sp_hip_session* s, sp_step_result* out)
+{
+    if (!s || !out) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->read_result(*out); });
+}
+
+int sp_hip_session_overlap_field(sp_hip_session* s, uint32_t o, int32_t* out, uint64_t count, uint32_t legal[3])
+{
+    if (!s || !legal) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->overlap_field(o, out, count, legal); });
+}
+
+void* sp_hip_session_stream(sp_hip_session* s) { return s ? static_cast<void*>(s->s->stream()) : nullptr; }
+
+uint64_t sp_hip_session_device_bytes(const sp_hip_session* s) { return s ? s->s->device_bytes() : 0; }
+
+}  // extern "C"

@@ -1,0 +1,336 @@
+"""Python binding of the MI355X spectral placement step (ctypes over the C ABI
+declared in include/spectrapack_hip.h).
+
+This is the host-side mirror used by tests and the benchmark: names, argument
+meaning and failure codes follow engine/pack_compute's correlation interface
+(engine/pack_compute/include/spectrapack/compute/correlation.hpp:17-117) and the
+plain gap-fill step (engine/pack_solver/src/gap_fill.cpp:363-509).  Every call
+runs the HIP library; there is no CPU fallback — loading fails loudly when the
+library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspectrapack_hip.so")
+
+_lib = None
+
+SP_PRECISION_AUTO, SP_PRECISION_FP32, SP_PRECISION_FP64 = 0, 32, 64
+
+
+class Spec(C.Structure):
+    _fields_ = [("environment_shape", C.c_uint32 * 3), ("kernel_shape", C.c_uint32 * 3),
+                ("environment_first", C.c_int64 * 3), ("kernel_first", C.c_int64 * 3)]
+
+
+class Limits(C.Structure):
+    _fields_ = [("max_working_bytes", C.c_uint64), ("reserved_bytes", C.c_uint64),
+                ("max_padded_cells", C.c_uint64), ("max_direct_terms", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("padded_cells", C.c_uint64), ("working_bytes_peak", C.c_uint64), ("direct_terms", C.c_uint64)]
+
+
+class Report(C.Structure):
+    _fields_ = [("max_integer_residual", C.c_double), ("mass_residual", C.c_double),
+                ("max_probe_error", C.c_double), ("error_bound", C.c_double), ("precision", C.c_int32),
+                ("fft_shape", C.c_uint32 * 3)]
+
+
+class Estimate(C.Structure):
+    _fields_ = [("padded_shape", C.c_uint32 * 3), ("padded_cells", C.c_uint64), ("working_bytes", C.c_uint64)]
+
+
+class SessionDesc(C.Structure):
+    _fields_ = [("environment_shape", C.c_uint32 * 3), ("environment_first", C.c_int64 * 3),
+                ("orientations", C.c_uint32), ("precision", C.c_int32), ("capture_graph", C.c_uint32)]
+
+
+class StepResult(C.Structure):
+    _fields_ = [("orientation", C.c_int32), ("reserved", C.c_int32), ("translation", C.c_int64 * 3),
+                ("key", C.c_int64 * 3), ("max_residual", C.c_double), ("numeric_flags", C.c_uint32),
+                ("precision", C.c_int32), ("error_bound", C.c_double)]
+
+
+class Failure(Exception):
+    """A CorrelationFailure / RepresentationFailure variant (code string + stats)."""
+
+    def __init__(self, status, code, message="", stats=None):
+        super().__init__(f"{code}: {message}" if message else code)
+        self.status = status
+        self.code = code
+        self.message = message
+        self.stats = stats or {}
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run cpp-max-packer_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        L.sp_hip_status_code.restype = C.c_char_p
+        L.sp_hip_last_error.restype = C.c_char_p
+        L.sp_hip_session_stream.restype = C.c_void_p
+        L.sp_hip_session_device_bytes.restype = C.c_uint64
+        L.sp_hip_fp32_error_bound.restype = C.c_double
+        L.sp_hip_fp32_error_bound.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        _lib = L
+    return _lib
+
+
+def status_code(status: int) -> str:
+    return lib().sp_hip_status_code(int(status)).decode()
+
+
+def fp32_error_bound(fft_cells: int, env_mass: int, kernel_mass: int) -> float:
+    return lib().sp_hip_fp32_error_bound(int(fft_cells), int(env_mass), int(kernel_mass))
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class Context:
+    """The sp_hip_ctx handle (the role FftExecution plays: correlation.hpp:78-98)."""
+
+    def __init__(self, device: int = 0, bytes_cap: int = 0):
+        self._h = C.c_void_p()
+        rc = lib().sp_hip_ctx_create(C.c_int(device), C.c_uint64(bytes_cap), C.byref(self._h))
+        if rc:
+            raise Failure(rc, status_code(rc), "context creation failed")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def last_error(self) -> str:
+        return lib().sp_hip_last_error(self._h).decode()
+
+    def close(self):
+        if self._h:
+            lib().sp_hip_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _fail(self, rc, stats=None):
+        raise Failure(rc, status_code(rc), self.last_error(), stats)
+
+    def estimate(self, spec):
+        s = _spec(spec)
+        e = Estimate()
+        rc = lib().sp_hip_estimate(C.byref(s), C.byref(e))
+        if rc:
+            self._fail(rc)
+        return dict(padded_shape=tuple(e.padded_shape), padded_cells=e.padded_cells, working_bytes=e.working_bytes)
+
+    def correlate(self, spec, environment, kernel, limits=None, precision=SP_PRECISION_AUTO, proximity=False):
+        """correlate_binary_cpu / correlate_proximity_cpu on the device.
+
+        Returns dict(translation_first, shape, values, numeric, stats); raises
+        Failure with the reference's code string otherwise."""
+        s = _spec(spec)
+        env = np.ascontiguousarray(environment, dtype=np.float64 if proximity else np.uint8).ravel()
+        ker = np.ascontiguousarray(kernel, dtype=np.uint8).ravel()
+        lim = Limits(*(limits if limits is not None else (512 << 20, 0, 16777216, 200000000)))
+        es = np.array(spec[0], dtype=np.int64)
+        ks = np.array(spec[1], dtype=np.int64)
+        n = int(np.prod(es + ks - 1)) if (es > 0).all() and (ks > 0).all() else 0
+        out = np.zeros(max(n, 1), np.float64)
+        first = (C.c_int64 * 3)()
+        shape = (C.c_uint32 * 3)()
+        rep = Report()
+        st = Stats()
+        if proximity:
+            rc = lib().sp_hip_correlate_proximity(self._h, C.byref(s), _p(env, C.c_double), C.c_uint64(env.size),
+                                                  _p(ker, C.c_uint8), C.c_uint64(ker.size), C.byref(lim),
+                                                  _p(out, C.c_double), C.c_uint64(out.size), first, shape, C.byref(rep),
+                                                  C.byref(st))
+        else:
+            rc = lib().sp_hip_correlate_binary(self._h, C.byref(s), _p(env, C.c_uint8), C.c_uint64(env.size),
+                                               _p(ker, C.c_uint8), C.c_uint64(ker.size), C.byref(lim),
+                                               C.c_int(precision), _p(out, C.c_double), C.c_uint64(out.size), first,
+                                               shape, C.byref(rep), C.byref(st))
+        stats = dict(padded_cells=st.padded_cells, working_bytes_peak=st.working_bytes_peak,
+                     direct_terms=st.direct_terms)
+        if rc:
+            self._fail(rc, stats)
+        sh = tuple(shape)
+        return dict(translation_first=np.array(tuple(first)), shape=np.array(sh),
+                    values=out[: int(np.prod(sh))].copy(),
+                    numeric=dict(max_integer_residual=rep.max_integer_residual, mass_residual=rep.mass_residual,
+                                 max_probe_error=rep.max_probe_error, error_bound=rep.error_bound,
+                                 precision=rep.precision, fft_shape=tuple(rep.fft_shape)),
+                    stats=stats)
+
+
+    # ---- rasteriser (conservative_fields.hpp:100-164) ----
+    def voxelize_objects(self, vertices, triangles, quaternions, pitch, origin=(0.0, 0.0, 0.0)):
+        """voxelize_object for every quaternion: list of dict(first, shape, cells[z,y,x])."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1, 3)
+        q = np.ascontiguousarray(quaternions, dtype=np.float64).reshape(-1, 4)
+        n = q.shape[0]
+        org = np.asarray(origin, dtype=np.float64)
+        firsts = np.zeros((n, 3), np.int64)
+        shapes = np.zeros((n, 3), np.uint32)
+        offs = np.zeros(n, np.uint64)
+        req = C.c_uint64(0)
+        args = (self._h, _p(v, C.c_double), C.c_uint32(len(v)), _p(t, C.c_uint32), C.c_uint32(len(t)),
+                _p(org, C.c_double), C.c_double(pitch), _p(q, C.c_double), C.c_uint32(n), _p(firsts, C.c_int64),
+                _p(shapes, C.c_uint32), _p(offs, C.c_uint64))
+        rc = lib().sp_hip_voxelize_objects(*args, None, C.c_uint64(0), C.byref(req))
+        if rc and rc != 37:
+            self._fail(rc)
+        cells = np.zeros(max(int(req.value), 1), np.uint8)
+        rc = lib().sp_hip_voxelize_objects(*args, _p(cells, C.c_uint8), C.c_uint64(cells.size), C.byref(req))
+        if rc:
+            self._fail(rc)
+        out = []
+        for k in range(n):
+            sh = tuple(int(x) for x in shapes[k])
+            m = sh[0] * sh[1] * sh[2]
+            o = int(offs[k])
+            out.append(dict(first=tuple(int(x) for x in firsts[k]), shape=sh,
+                            cells=cells[o:o + m].reshape(sh[2], sh[1], sh[0]).copy()))
+        return out
+
+    def box_container(self, first, shape, pitch, dims, clearance=0.0, origin=(0.0, 0.0, 0.0)):
+        f = (C.c_int64 * 3)(*[int(x) for x in first])
+        s = (C.c_uint32 * 3)(*[int(x) for x in shape])
+        org = np.asarray(origin, dtype=np.float64)
+        d = np.asarray(dims, dtype=np.float64)
+        cells = np.zeros(int(np.prod([int(x) for x in shape])), np.uint8)
+        rc = lib().sp_hip_voxelize_box_container(self._h, f, s, _p(org, C.c_double), C.c_double(pitch),
+                                                 _p(d, C.c_double), C.c_double(clearance), _p(cells, C.c_uint8))
+        if rc:
+            self._fail(rc)
+        return cells.reshape(int(shape[2]), int(shape[1]), int(shape[0]))
+
+    def mesh_container(self, vertices, triangles, first, shape, pitch, clearance=0.0, origin=(0.0, 0.0, 0.0)):
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1, 3)
+        f = (C.c_int64 * 3)(*[int(x) for x in first])
+        s = (C.c_uint32 * 3)(*[int(x) for x in shape])
+        org = np.asarray(origin, dtype=np.float64)
+        cells = np.zeros(int(np.prod([int(x) for x in shape])), np.uint8)
+        rc = lib().sp_hip_voxelize_mesh_container(self._h, _p(v, C.c_double), C.c_uint32(len(v)), _p(t, C.c_uint32),
+                                                  C.c_uint32(len(t)), f, s, _p(org, C.c_double), C.c_double(pitch),
+                                                  C.c_double(clearance), _p(cells, C.c_uint8))
+        if rc:
+            self._fail(rc)
+        return cells.reshape(int(shape[2]), int(shape[1]), int(shape[0]))
+
+
+def _spec(spec) -> Spec:
+    es, ks, ef, kf = spec
+    return Spec((C.c_uint32 * 3)(*[int(v) for v in es]), (C.c_uint32 * 3)(*[int(v) for v in ks]),
+                (C.c_int64 * 3)(*[int(v) for v in ef]), (C.c_int64 * 3)(*[int(v) for v in kf]))
+
+
+class Session:
+    """Batched placement step with the occupancy resident in HBM."""
+
+    def __init__(self, ctx: Context, env_shape, env_first, orientations, precision=SP_PRECISION_AUTO,
+                 capture_graph=True):
+        self.ctx = ctx
+        self.env_shape = tuple(int(v) for v in env_shape)  # (x, y, z)
+        self.env_first = tuple(int(v) for v in env_first)
+        self.orientations = int(orientations)
+        d = SessionDesc((C.c_uint32 * 3)(*self.env_shape), (C.c_int64 * 3)(*self.env_first),
+                        C.c_uint32(self.orientations), C.c_int32(precision), C.c_uint32(1 if capture_graph else 0))
+        self._h = C.c_void_p()
+        rc = lib().sp_hip_session_create(ctx.handle, C.byref(d), C.byref(self._h))
+        if rc:
+            ctx._fail(rc)
+        self.kernel_shapes = [None] * self.orientations
+
+    def close(self):
+        if self._h:
+            lib().sp_hip_session_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            self.ctx._fail(rc)
+
+    def set_environment(self, cells):
+        """cells: uint8 array [z, y, x] (X-fastest)."""
+        a = np.ascontiguousarray(cells, dtype=np.uint8).ravel()
+        self._check(lib().sp_hip_session_set_environment(self._h, _p(a, C.c_uint8), C.c_uint64(a.size)))
+
+    def environment(self):
+        ex, ey, ez = self.env_shape
+        a = np.zeros(ex * ey * ez, np.uint8)
+        self._check(lib().sp_hip_session_get_environment(self._h, _p(a, C.c_uint8), C.c_uint64(a.size)))
+        return a.reshape(ez, ey, ex)
+
+    def set_kernel(self, o, cells, shape, first):
+        """cells [z, y, x] uint8; shape and first as (x, y, z)."""
+        a = np.ascontiguousarray(cells, dtype=np.uint8).ravel()
+        sh = (C.c_uint32 * 3)(*[int(v) for v in shape])
+        fi = (C.c_int64 * 3)(*[int(v) for v in first])
+        self._check(lib().sp_hip_session_set_kernel(self._h, C.c_uint32(o), _p(a, C.c_uint8), sh, fi))
+        self.kernel_shapes[o] = tuple(int(v) for v in shape)
+
+    def keep_fields(self, keep=True):
+        self._check(lib().sp_hip_session_keep_fields(self._h, C.c_int(1 if keep else 0)))
+
+    @staticmethod
+    def _result(r: StepResult):
+        return dict(orientation=r.orientation, translation=tuple(r.translation), key=tuple(r.key),
+                    max_residual=r.max_residual, numeric_flags=r.numeric_flags, precision=r.precision,
+                    error_bound=r.error_bound)
+
+    def step(self, stamp=True):
+        r = StepResult()
+        self._check(lib().sp_hip_session_step(self._h, C.c_int(1 if stamp else 0), C.byref(r)))
+        return self._result(r)
+
+    def run(self, steps, stamp=True):
+        self._check(lib().sp_hip_session_run(self._h, C.c_uint32(steps), C.c_int(1 if stamp else 0)))
+
+    def last_result(self):
+        r = StepResult()
+        self._check(lib().sp_hip_session_last_result(self._h, C.byref(r)))
+        return self._result(r)
+
+    def overlap_field(self, o):
+        ex, ey, ez = self.env_shape
+        kx, ky, kz = self.kernel_shapes[o]
+        n = max((ex - kx + 1) * (ey - ky + 1) * (ez - kz + 1), 1)
+        a = np.zeros(n, np.int32)
+        legal = (C.c_uint32 * 3)()
+        self._check(lib().sp_hip_session_overlap_field(self._h, C.c_uint32(o), _p(a, C.c_int32), C.c_uint64(a.size),
+                                                       legal))
+        lx, ly, lz = tuple(legal)
+        if lx * ly * lz == 0:
+            return None
+        return a[: lx * ly * lz].reshape(lz, ly, lx)
+
+    def stream(self):
+        return lib().sp_hip_session_stream(self._h)
+
+    def device_bytes(self):
+        return lib().sp_hip_session_device_bytes(self._h)

@@ -1,0 +1,218 @@
+/* spectrapack_hip.h — C ABI of the MI355X spectral placement step.
+ *
+ * Plain C: POD structs, pointers and sizes, int status codes; no exceptions
+ * cross this boundary.  Status codes map one-to-one onto the reference's
+ * failure code strings (sp_hip_status_code returns the string).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   engine/pack_compute/include/spectrapack/compute/correlation.hpp:17-117
+ *     CorrelationSpec, CorrelationLimits, CorrelationStats, NumericReport,
+ *     CorrelationFailure/Result, FftExecution, estimate_correlation_cpu,
+ *     correlate_binary_cpu, correlate_proximity_cpu.
+ *   engine/pack_solver/src/gap_fill.cpp:363-509 (per-rotation free masks and
+ *     the lowest (z, y, x) choice) — the batched placement step.
+ *   engine/pack_geometry/include/spectrapack/geometry/conservative_fields.hpp:100-164
+ *     voxelize_object / voxelize_container — the rasteriser.
+ *   engine/pack_solver/src/orientation_cube.hpp:10-80, orientations.cpp:44-172 —
+ *     orientation catalogs.
+ */
+#ifndef SPECTRAPACK_HIP_H
+#define SPECTRAPACK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (correlation.hpp:52-56 CorrelationFailure::code values) */
+enum sp_status {
+    SP_OK = 0,
+    SP_CORRELATION_MEMORY_LIMIT = 1,
+    SP_CORRELATION_SHAPE_OVERFLOW = 2,
+    SP_CORRELATION_INDEX_OVERFLOW = 3,
+    SP_CORRELATION_INVALID_FIELD = 4,
+    SP_CORRELATION_INVALID_ENVIRONMENT = 5,
+    SP_CORRELATION_INVALID_KERNEL = 6,
+    SP_CORRELATION_NUMERIC_RANGE = 7,
+    SP_CORRELATION_PADDED_CELL_LIMIT = 8,
+    SP_CORRELATION_NUMERIC = 9,
+    SP_CORRELATION_DIRECT_TERM_LIMIT = 10,
+    SP_CORRELATION_ALLOCATION = 11,
+    SP_CORRELATION_OPERATION = 12,
+    SP_CORRELATION_FLOATING_ENVIRONMENT = 13,
+    SP_CORRELATION_EXECUTION_CLOSED = 14,
+    SP_CPU_THREAD_COUNT_UNSUPPORTED = 15,
+    SP_FFT_THREAD_START_FAILURE = 16,
+    SP_OPERATION_CANCELLED = 17,
+    SP_DEADLINE_EXCEEDED = 18,
+    /* Codes of the GPU build itself (no reference counterpart). */
+    SP_DEVICE_ERROR = 32,     /* a HIP runtime call failed; see sp_hip_last_error */
+    SP_INVALID_ARGUMENT = 33, /* null pointer, bad handle or unsupported option */
+    SP_FIELD_INPUT = 34,      /* conservative_fields.cpp FIELD_INPUT */
+    SP_FIELD_INDEX_OVERFLOW = 35,
+    SP_FFT_LENGTH_UNSUPPORTED = 36, /* a padded length above the kernels' maximum */
+    SP_CAPACITY = 37          /* caller buffer too small; required size reported */
+};
+
+/* Returns the reference's code string for a status ("CORRELATION_NUMERIC", ...). */
+const char* sp_hip_status_code(int status);
+
+/* ---- context: correlation.hpp:78-98 FftExecution (one coordinator owns it and
+ * serialises calls).  Owns a HIP stream, device buffers and FFT tables. */
+typedef struct sp_hip_ctx sp_hip_ctx;
+int sp_hip_ctx_create(int device, uint64_t bytes_cap, sp_hip_ctx** out);
+void sp_hip_ctx_destroy(sp_hip_ctx* ctx);
+const char* sp_hip_last_error(const sp_hip_ctx* ctx);
+
+/* correlation.hpp:22-27 (X-fastest storage). */
+typedef struct {
+    uint32_t environment_shape[3];
+    uint32_t kernel_shape[3];
+    int64_t environment_first[3];
+    int64_t kernel_first[3];
+} sp_correlation_spec;
+
+/* correlation.hpp:29-36 */
+typedef struct {
+    uint64_t max_working_bytes;
+    uint64_t reserved_bytes;
+    uint64_t max_padded_cells;
+    uint64_t max_direct_terms;
+} sp_correlation_limits;
+
+/* correlation.hpp:38-44 */
+typedef struct {
+    uint64_t padded_cells;
+    uint64_t working_bytes_peak;
+    uint64_t direct_terms;
+} sp_correlation_stats;
+
+/* correlation.hpp:46-50, plus the precision the GPU used and its bound. */
+typedef struct {
+    double max_integer_residual;
+    double mass_residual;
+    double max_probe_error;
+    double error_bound;  /* a-priori rounding bound of the chosen precision */
+    int32_t precision;   /* 32 or 64 */
+    uint32_t fft_shape[3];
+} sp_numeric_report;
+
+/* correlation.hpp:100-103 */
+typedef struct {
+    uint32_t padded_shape[3];
+    uint64_t padded_cells;
+    uint64_t working_bytes;
+} sp_correlation_estimate;
+
+enum sp_precision { SP_PRECISION_AUTO = 0, SP_PRECISION_FP32 = 32, SP_PRECISION_FP64 = 64 };
+
+/* correlation.cpp:596-641 estimate_correlation_cpu: padded shape E+K-1 and the
+ * device bytes the GPU correlation needs. */
+int sp_hip_estimate(const sp_correlation_spec* spec, sp_correlation_estimate* out);
+
+/* correlation.hpp:108-111 correlate_binary_cpu: full-linear correlation of a
+ * uint8 environment {0,1} with a uint8 kernel {0,1}.  out_values receives
+ * prod(E+K-1) integer-valued doubles, X-fastest; out_first/out_shape the
+ * translation_first = E.first - K.first - (K-1) and shape E+K-1.  Host buffers. */
+int sp_hip_correlate_binary(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const uint8_t* environment,
+                            uint64_t environment_count, const uint8_t* kernel, uint64_t kernel_count,
+                            const sp_correlation_limits* limits, int precision, double* out_values,
+                            uint64_t out_capacity, int64_t out_first[3], uint32_t out_shape[3],
+                            sp_numeric_report* report, sp_correlation_stats* stats);
+
+/* correlation.hpp:113-117 correlate_proximity_cpu: environment values in [0,1]
+ * (double), FP64 on the device (the reference pins these to 1e-10 relative). */
+int sp_hip_correlate_proximity(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const double* environment,
+                               uint64_t environment_count, const uint8_t* kernel, uint64_t kernel_count,
+                               const sp_correlation_limits* limits, double* out_values, uint64_t out_capacity,
+                               int64_t out_first[3], uint32_t out_shape[3], sp_numeric_report* report,
+                               sp_correlation_stats* stats);
+
+/* ---- batched placement step (gap_fill.cpp:363-509) -------------------------
+ * A session keeps the environment occupancy (uint8, X-fastest) resident in HBM
+ * together with O oriented kernels.  One step correlates every orientation in
+ * one graph launch, forms each feasible mask (rounded overlap == 0 over legal
+ * translations), picks the lowest kernel-window corner in (z, y, x) order with
+ * the earliest orientation winning ties, and (optionally) stamps that kernel. */
+typedef struct sp_hip_session sp_hip_session;
+
+typedef struct {
+    uint32_t environment_shape[3];
+    int64_t environment_first[3];
+    uint32_t orientations;
+    int32_t precision;      /* sp_precision */
+    uint32_t capture_graph; /* 1: replay each step as one hipGraph */
+} sp_session_desc;
+
+typedef struct {
+    int32_t orientation;     /* -1 when no orientation has a free translation */
+    int32_t reserved;
+    int64_t translation[3];  /* t: kernel cell i sits at world cell t + kernel_first + i */
+    int64_t key[3];          /* (t + kernel_first) as x, y, z */
+    double max_residual;     /* max |v - round(v)| over legal translations, all orientations */
+    uint32_t numeric_flags;  /* bit0: residual >= 0.25, bit1: range check failed */
+    int32_t precision;
+    double error_bound;      /* a-priori bound of the step's precision */
+} sp_step_result;
+
+int sp_hip_session_create(sp_hip_ctx* ctx, const sp_session_desc* desc, sp_hip_session** out);
+void sp_hip_session_destroy(sp_hip_session* s);
+/* Host occupancy grid (uint8 {0,1}, prod(environment_shape) cells). */
+int sp_hip_session_set_environment(sp_hip_session* s, const uint8_t* cells, uint64_t count);
+int sp_hip_session_get_environment(sp_hip_session* s, uint8_t* cells, uint64_t count);
+/* Kernel o: uint8 {0,1} cells of shape kernel_shape (X-fastest) whose cell 0
+ * is world cell kernel_first.  All orientations must be set before a step. */
+int sp_hip_session_set_kernel(sp_hip_session* s, uint32_t orientation, const uint8_t* cells,
+                              const uint32_t kernel_shape[3], const int64_t kernel_first[3]);
+/* One step; stamp != 0 ORs the chosen kernel into the occupancy on the device. */
+int sp_hip_session_step(sp_hip_session* s, int stamp, sp_step_result* out);
+/* Enqueue `steps` steps without reading results back (for throughput timing);
+ * sp_hip_session_step_result then reads the last one. */
+int sp_hip_session_run(sp_hip_session* s, uint32_t steps, int stamp);
+int sp_hip_session_last_result(sp_hip_session* s, sp_step_result* out);
+/* Legal overlap counts of orientation o from the last step (int32, X-fastest,
+ * shape environment_shape - kernel_shape + 1); requires keep_fields. */
+int sp_hip_session_keep_fields(sp_hip_session* s, int keep);
+int sp_hip_session_overlap_field(sp_hip_session* s, uint32_t orientation, int32_t* out, uint64_t count,
+                                 uint32_t legal_shape[3]);
+/* Stream the session launches on (hipStream_t), for events around its work. */
+void* sp_hip_session_stream(sp_hip_session* s);
+/* Device bytes held by the session. */
+uint64_t sp_hip_session_device_bytes(const sp_hip_session* s);
+
+/* ---- conservative rasteriser (conservative_fields.hpp:100-164) -------------
+ * Mesh: vertex_count x 3 doubles (local frame), triangle_count x 3 indices.
+ *
+ * voxelize_object (conservative_fields.cpp:919-1003) for `count` canonical
+ * quaternions (x, y, z, w) at the lattice origin (which must be 0,0,0):
+ * per orientation the trimmed window (firsts/shapes, 3 per orientation) and
+ * {0,1} cells (X-fastest) concatenated at offsets[k].  When cells is null or
+ * capacity < required, returns SP_CAPACITY with every window and *required
+ * filled. */
+int sp_hip_voxelize_objects(sp_hip_ctx* ctx, const double* vertices, uint32_t vertex_count, const uint32_t* triangles,
+                            uint32_t triangle_count, const double origin[3], double pitch, const double* quaternions,
+                            uint32_t count, int64_t* firsts, uint32_t* shapes, uint64_t* offsets, uint8_t* cells,
+                            uint64_t capacity, uint64_t* required);
+/* voxelize_container, analytic box (conservative_fields.cpp:1146-1198):
+ * 1 = blocked, 0 = certified inside the box shrunk by the clearance. */
+int sp_hip_voxelize_box_container(sp_hip_ctx* ctx, const int64_t first[3], const uint32_t shape[3], const double origin[3],
+                                  double pitch, const double dims[3], double clearance, uint8_t* cells);
+/* voxelize_container, closed mesh (conservative_fields.cpp:1200-1275) at zero
+ * wall clearance: only cells certified inside the solid are free (0). */
+int sp_hip_voxelize_mesh_container(sp_hip_ctx* ctx, const double* vertices, uint32_t vertex_count,
+                                   const uint32_t* triangles, uint32_t triangle_count, const int64_t first[3],
+                                   const uint32_t shape[3], const double origin[3], double pitch, double clearance,
+                                   uint8_t* cells);
+
+/* A-priori FP32 rounding bound for a correlation of an environment of mass
+ * env_mass with a kernel of mass kernel_mass on an FFT grid of fft_cells cells
+ * (see DESIGN: probabilistic rounding model).  FP64 is chosen when > 0.4. */
+double sp_hip_fp32_error_bound(uint64_t fft_cells, uint64_t env_mass, uint64_t kernel_mass);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

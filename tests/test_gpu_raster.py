@@ -1,0 +1,65 @@
+"""GPU rasteriser against the oracle, bit-exact: oriented kernels (cube and
+SO(3) catalogs; cuboid, star, L-bracket, torus knot), analytic box and mesh
+containers."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def compare_objects(ctx, oracle, mesh, catalog, pitch):
+    g = ctx.voxelize_objects(mesh.vertices, mesh.triangles, catalog, pitch)
+    for q, k in zip(catalog, g):
+        f, s, c = oracle.voxelize_object(mesh.vertices, mesh.triangles, q, pitch)
+        assert tuple(f) == k["first"] and tuple(s) == k["shape"], q
+        assert np.array_equal(c, k["cells"]), q
+
+
+def test_box_cube_catalog(ctx, oracle):
+    from spectrapack_amd import workloads as W
+    print("[raster] box cube catalog", flush=True)
+    compare_objects(ctx, oracle, W.cuboid(10.0, 6.0, 4.0), oracle.cube_catalog(), 1.0)
+
+
+def test_box_rotated(ctx, oracle):
+    from spectrapack_amd import workloads as W
+    print("[raster] box rotated", flush=True)
+    compare_objects(ctx, oracle, W.cuboid(10.0, 6.0, 4.0), W.so3_catalog(12, 3), 1.0)
+
+
+def test_star_cube_and_so3(ctx, oracle):
+    from spectrapack_amd import workloads as W
+    print("[raster] star", flush=True)
+    m = W.star(W.DEFAULT_SEED)
+    compare_objects(ctx, oracle, m, oracle.cube_catalog()[:8], 1.0)
+    compare_objects(ctx, oracle, m, W.so3_catalog(8, 1), 1.0)
+
+
+def test_bracket_so3(ctx, oracle):
+    from spectrapack_amd import workloads as W
+    print("[raster] bracket", flush=True)
+    w = W.bracket_in_sphere(orientations=12)
+    compare_objects(ctx, oracle, w.part, w.catalog[:12], w.pitch)
+    compare_objects(ctx, oracle, w.part, oracle.cube_catalog()[:6], w.pitch)
+
+
+def test_box_container(ctx, oracle):
+    from spectrapack_amd import workloads as W
+    print("[raster] box container", flush=True)
+    for dims, pitch, clearance in [((64.0, 64.0, 64.0), 1.0, 0.0), ((20.0, 13.5, 9.25), 0.75, 0.0),
+                                   ((30.0, 30.0, 10.0), 1.0, 1.5)]:
+        first, shape = W.box_window(dims, pitch)
+        g = ctx.box_container(first, shape, pitch, dims, clearance)
+        c = oracle.box_container(first, shape, pitch, dims, clearance)
+        assert np.array_equal(g, c)
+
+
+def test_sphere_container(ctx, oracle):
+    from spectrapack_amd import workloads as W
+    print("[raster] sphere container", flush=True)
+    sph = W.sphere(30.0, n_seg=32, n_lat=15)
+    for pitch in (1.0, 0.7):
+        first, shape = W.mesh_window(sph, pitch)
+        g = ctx.mesh_container(sph.vertices, sph.triangles, first, shape, pitch)
+        c = oracle.mesh_container(sph.vertices, sph.triangles, first, shape, pitch)
+        assert np.array_equal(g, c)
