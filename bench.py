@@ -1,0 +1,211 @@
+"""Placement-step throughput on one MI355X.
+
+One step = for every orientation of the catalog, the overlap counts of the
+oriented part over all legal translations of the current occupancy (spectral
+correlation), the feasible mask (rounded count == 0), the lowest (z, y, x)
+choice with the earliest orientation winning ties, and stamping the chosen part
+into the occupancy — all on the device as one graph launch.  The default
+workload is the L-bracket in an r=120 mm sphere voxelised at 256^3 with 96
+seeded SO(3) orientations.  Kernel rasterisation runs once per run and is
+reported separately.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cpp-max-packer_amd"))
+
+METRIC = "placement steps/s at 256³×96 orientations; parts packed vs plain search"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def step_model_bytes(nx, ny, nz, orientations, precision):
+    """Fixed roofline model of one step: B = 48 Nc + O (120 Nc + 4 N) bytes in
+    FP32 (every spectrum term doubles in FP64 and the field read is 8 N)."""
+    N = nx * ny * nz
+    nc = (nx // 2 + 1) * ny * nz
+    if precision == 64:
+        return 96 * nc + orientations * (240 * nc + 8 * N)
+    return 48 * nc + orientations * (120 * nc + 4 * N)
+
+
+def kernel_bytes(name, n, kernels, ext, elem):
+    """Algorithmic bytes one launch of each step kernel must move (all orientations)."""
+    nx, ny, nz = n
+    nh = nz // 2 + 1
+    ex, ey, ez = ext
+    fit = [k for k in kernels if all(ext[a] >= k["shape"][a] for a in range(3))]
+    if name == "env_r2c_z":
+        return ex * ey * ez + nh * ny * nx * elem
+    if name in ("env_c2c_y", "env_c2c_x"):
+        return 2 * nh * ny * nx * elem
+    if name == "kernel_r2c_z":
+        return sum(k["shape"][0] * k["shape"][1] * k["shape"][2] + nh * k["shape"][1] * k["shape"][0] * elem for k in fit)
+    if name == "kernel_c2c_y":
+        return sum(nh * k["shape"][1] * k["shape"][0] * elem + nh * ny * k["shape"][0] * elem for k in fit)
+    if name == "mul_x":
+        return sum(nh * ny * (k["shape"][0] + nx + (ex - k["shape"][0] + 1)) * elem for k in fit)
+    if name == "inv_c2c_y":
+        return sum(nh * (ny + ey - k["shape"][1] + 1) * (ex - k["shape"][0] + 1) * elem for k in fit)
+    if name == "c2r_z_argmin":
+        return sum(nh * (ey - k["shape"][1] + 1) * (ex - k["shape"][0] + 1) * elem for k in fit)
+    return 0
+
+
+def cpu_baseline(workload, env, kernels, threads, sample_orientations):
+    """The oracle's FP64 full-linear restatement of the same correlations."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if not os.path.exists(O.LIB_PATH):
+        O.build()
+    t0 = time.perf_counter()
+    done, _ = O.correlate_orientations(env, workload.env_first, kernels, 0, sample_orientations, threads)
+    dt = time.perf_counter() - t0
+    per_step = dt / max(done, 1) * len(kernels) if threads == 1 else dt * len(kernels) / max(done, 1)
+    return dict(value=1.0 / per_step, unit="steps/s", cores=threads, kind="port",
+                sample=f"{done} of {len(kernels)} orientation correlations of one step (FP64 full-linear restatement, "
+                       f"P=E+K-1, {threads} thread(s)) in {dt:.1f} s; steps/s = {done}/({dt:.1f} s x {len(kernels)})")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="bracket", choices=["box", "star", "bracket", "knot"])
+    ap.add_argument("--precision", type=int, default=0, choices=[0, 32, 64])
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--profile-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: the placement step runs on a single GPU in this build "
+              "(multi-GPU orientation sharding is a design note only); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import spectrapack_amd as sp
+    from spectrapack_amd import workloads as W
+
+    log(f"[bench] workload={args.workload} steps={args.steps} warmup={args.warmup}")
+    w = W.WORKLOADS[args.workload]()
+    ctx = sp.Context(0)
+    # container occupancy and oriented kernels, on the device
+    t0 = time.perf_counter()
+    if w.container is not None:
+        env = ctx.mesh_container(w.container.vertices, w.container.triangles, w.env_first, w.env_shape, w.pitch)
+    else:
+        env = ctx.box_container(w.env_first, w.env_shape, w.pitch, w.box_dims)
+    t_container = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    kernels = ctx.voxelize_objects(w.part.vertices, w.part.triangles, w.catalog, w.pitch)
+    t_raster = time.perf_counter() - t0
+    log(f"[bench] container {t_container:.3f}s raster {len(kernels)} orientations {t_raster:.3f}s")
+
+    s = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=args.precision or
+                   (32 if w.precision == "fp32" else 0), capture_graph=not args.no_graph)
+    s.set_environment(env)
+    for i, k in enumerate(kernels):
+        s.set_kernel(i, k["cells"], k["shape"], k["first"])
+    r = s.step(stamp=True)  # prepares buffers and captures the graph
+    prec = r["precision"]
+    log(f"[bench] precision fp{prec} bound={r['error_bound']:.3g} device bytes={s.device_bytes() / 2**30:.2f} GiB")
+    placed = 1 if r["orientation"] >= 0 else 0
+    for _ in range(args.warmup):
+        r = s.step(stamp=True)
+        placed += r["orientation"] >= 0
+    stream = torch.cuda.ExternalStream(s.stream())
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    e0.record(stream)
+    s.run(args.steps, stamp=True)
+    e1.record(stream)
+    e1.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    torch.cuda.synchronize()
+    last = s.last_result()
+    ms_step = e0.elapsed_time(e1) / args.steps
+    log(f"[bench] {args.steps} steps: {ms_step:.3f} ms/step (events), wall {t_wall / args.steps * 1e3:.3f} ms/step; "
+        f"last choice {last['orientation']} residual {last['max_residual']:.3g}")
+
+    # per-kernel breakdown (no graph, events between kernels)
+    prof = {}
+    for _ in range(args.profile_steps):
+        for name, ms in s.profile_step(stamp=True):
+            prof.setdefault(name, []).append(ms)
+    prof = {k: sum(v) / len(v) for k, v in prof.items()}
+    n = tuple(int(sp_len) for sp_len in _fft_shape(w.env_shape))
+    elem = 8 if prec == 32 else 16
+    dominant = max(prof, key=prof.get)
+    dom_bytes = kernel_bytes(dominant, n, kernels, w.env_shape, elem)
+    dom_ms = prof[dominant]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    step_bytes = step_model_bytes(*n, len(kernels), prec)
+    step_achieved = step_bytes / (ms_step * 1e-3) / 1e9
+    log("[bench] kernels: " + ", ".join(f"{k} {v:.3f} ms" for k, v in prof.items()))
+
+    out = {
+        "metric": METRIC,
+        "value": 1000.0 / ms_step,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"f{prec}",
+        "data": "synthetic (seeded meshes and catalog, seed 20261203)",
+        "config": {"workload": w.name, "grid": list(w.env_shape), "fft_grid": list(n), "orientations": len(kernels),
+                   "pitch_mm": w.pitch, "precision": f"fp{prec}", "error_bound": r["error_bound"],
+                   "graph": not args.no_graph, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
+        "step_roofline": {"model_bytes_per_step": step_bytes, "achieved": step_achieved, "peak": PEAK_HBM_GBS,
+                          "unit": "GB/s", "frac": step_achieved / PEAK_HBM_GBS},
+        "kernel_ms": prof,
+        "raster_s": t_raster,
+        "container_s": t_container,
+        "parts_placed": int(placed + args.steps if last["orientation"] >= 0 else placed),
+        "max_residual_last_step": last["max_residual"],
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline sample ...")
+        out["cpu_baseline"] = cpu_baseline(w, env, kernels, args.cpu_threads, max(1, args.cpu_threads))
+        out["cpu_baseline"]["host"] = platform.processor() or platform.machine()
+    print(json.dumps(out), flush=True)
+
+
+def _fft_shape(env_shape):
+    """The FFT grid the session uses: the smallest 2^a 3^b 5^c length >= each extent."""
+    def good(n):
+        m = max(n, 1)
+        while True:
+            r = m
+            for p in (2, 3, 5):
+                while r % p == 0:
+                    r //= p
+            if r == 1:
+                return m
+            m += 1
+    return [good(int(e)) for e in env_shape]
+
+
+if __name__ == "__main__":
+    main()

@@ -241,6 +241,18 @@ public:
         prepared_ = true;
     }
 
+    // Optional per-kernel timing: when prof_ is set, an event is recorded after
+    // every launch (outside graph capture only).
+    void mark(const char* name)
+    {
+        if (!prof_) return;
+        hipEvent_t e;
+        SP_HIP_CHECK(hipEventCreate(&e));
+        SP_HIP_CHECK(hipEventRecord(e, stream_));
+        prof_events_.push_back(e);
+        prof_names_.push_back(name);
+    }
+
     template <typename T> void enqueue_step(bool stamp)
     {
         using T2 = cx<T>;
@@ -256,13 +268,16 @@ public:
         const OrientDesc* dd = d_descs_.as<OrientDesc>();
         StepDeviceResult* res = result_.as<StepDeviceResult>();
         const std::size_t e = sizeof(T2);
+        mark("start");
         hipLaunchKernelGGL(k_reset_result, dim3(1), dim3(64), 0, stream_, res);
+        mark("reset");
         // E1: environment R2C along z (uint8 -> spectrum planes)
         {
             const int W = choose_w(nz, e, 2, true, 32, 2);
             hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((nx + W - 1) / W, ny, 1), dim3(kThreads),
                                lds_bytes<T>(nz, 2, W / 2), stream_, env_.as<unsigned char>(), 0LL, ext_[0], ext_[1],
                                ext_[0], ext_[1], ext_[2], SB, 0LL, nx, ny, nx, ny, nullptr, twz, plans_[2], W);
+            mark("env_r2c_z");
         }
         // E2: C2C along y (columns x, batch kz)
         {
@@ -270,6 +285,7 @@ public:
             hipLaunchKernelGGL((k_c2c<T>), dim3((nx + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(ny, 2, W),
                                stream_, SB, 0LL, (long long)nx, 1LL, (long long)nx * ny, SB, 0LL, (long long)nx, 1LL,
                                (long long)nx * ny, nx, ny, ny, nullptr, 0, twy, plans_[1], W, 1);
+            mark("env_c2c_y");
         }
         // E3: C2C along x (rows, batch kz)
         {
@@ -277,6 +293,7 @@ public:
             hipLaunchKernelGGL((k_c2c<T>), dim3((ny + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(nx, 2, W),
                                stream_, SB, 0LL, 1LL, (long long)nx, (long long)nx * ny, SB, 0LL, 1LL, (long long)nx,
                                (long long)nx * ny, ny, nx, nx, nullptr, 0, twx, plans_[0], W, 1);
+            mark("env_c2c_x");
         }
         const long long h1o = (long long)nh * kmax_[1] * kmax_[0];
         const long long h2o = (long long)nh * ny * kmax_[0];
@@ -287,6 +304,7 @@ public:
             hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((kmax_[0] + W - 1) / W, kmax_[1], O_), dim3(kThreads),
                                lds_bytes<T>(nz, 2, W / 2), stream_, kcells_.as<unsigned char>(), 0LL, 0, 0, 0, 0, 0, H1,
                                h1o, kmax_[0], kmax_[1], 0, 0, dd, twz, plans_[2], W);
+            mark("kernel_r2c_z");
         }
         // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
         {
@@ -294,12 +312,14 @@ public:
             hipLaunchKernelGGL((k_c2c<T>), dim3((kmax_[0] + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(ny, 2, W),
                                stream_, H1, h1o, (long long)kmax_[0], 1LL, (long long)kmax_[1] * kmax_[0], H2, h2o,
                                (long long)kmax_[0], 1LL, (long long)ny * kmax_[0], 0, 0, ny, dd, 1, twy, plans_[1], W, 1);
+            mark("kernel_c2c_y");
         }
         // P1: finish kernel x transform, product with conj, inverse x
         {
             const int W = choose_w(nx, e, 4, false, 16, 1);
             hipLaunchKernelGGL((k_mul_x<T>), dim3((ny + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(nx, 4, W),
                                stream_, H2, h2o, kmax_[0], ny, SB, nx, ny, TT, to, dd, 0, 0, twx, plans_[0], W);
+            mark("mul_x");
         }
         // P2: inverse C2C along y for the legal columns, rows y < ly written
         if (lmax_[0] > 0) {
@@ -307,16 +327,19 @@ public:
             hipLaunchKernelGGL((k_c2c<T>), dim3((lmax_[0] + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(ny, 2, W),
                                stream_, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to, (long long)nx, 1LL,
                                (long long)nx * ny, 0, ny, 0, dd, 2, twy, plans_[1], W, 0);
+            mark("inv_c2c_y");
             // P3: C2R along z + round + checks + feasible + lowest (z, y, x)
             const int W3 = choose_w(nz, e, 2, true, 32, 2);
             const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
             hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], O_), dim3(kThreads),
                                lds_bytes<T>(nz, 2, W3 / 2), stream_, TT, to, nx, ny, dd, inv_total, res,
                                keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(), twz, plans_[2], W3);
+            mark("c2r_z_argmin");
         }
         if (stamp) {
             hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
                                ext_[2], kcells_.as<unsigned char>(), dd, res);
+            mark("stamp");
         }
     }
 
@@ -351,6 +374,34 @@ public:
             graph_stamp_ = stamp;
         }
         SP_HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+    }
+
+    // One step without the graph, timing each kernel between events.
+    std::vector<std::pair<std::string, float>> profile_step(bool stamp)
+    {
+        prepare();
+        use_device(ctx_);
+        prof_ = true;
+        prof_events_.clear();
+        prof_names_.clear();
+        try {
+            enqueue(stamp);
+        }
+        catch (...) {
+            prof_ = false;
+            throw;
+        }
+        prof_ = false;
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        std::vector<std::pair<std::string, float>> out;
+        for (std::size_t i = 1; i < prof_events_.size(); ++i) {
+            float ms = 0;
+            SP_HIP_CHECK(hipEventElapsedTime(&ms, prof_events_[i - 1], prof_events_[i]));
+            out.emplace_back(prof_names_[i], ms);
+        }
+        for (auto e : prof_events_) (void)hipEventDestroy(e);
+        prof_events_.clear();
+        return out;
     }
 
     void read_result(sp_step_result& out)
@@ -432,6 +483,9 @@ private:
     hipGraph_t graph_ {};
     hipGraphExec_t graph_exec_ {};
     bool graph_stamp_ {};
+    bool prof_ {};
+    std::vector<hipEvent_t> prof_events_;
+    std::vector<const char*> prof_names_;
 };
 
 }  // namespace sphip
@@ -560,6 +614,23 @@ int sp_hip_session_overlap_field(sp_hip_session* s, uint32_t o, int32_t* out, ui
 {
     if (!s || !legal) return SP_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] { s->s->overlap_field(o, out, count, legal); });
+}
+
+int sp_hip_session_profile_step(sp_hip_session* s, int stamp, float* kernel_ms, char* names, uint32_t capacity,
+                                uint32_t* count)
+{
+    if (!s || !count) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        const auto prof = s->s->profile_step(stamp != 0);
+        *count = uint32_t(prof.size());
+        for (uint32_t i = 0; i < prof.size() && i < capacity; ++i) {
+            if (kernel_ms) kernel_ms[i] = prof[i].second;
+            if (names) {
+                std::strncpy(names + 32 * i, prof[i].first.c_str(), 31);
+                names[32 * i + 31] = 0;
+            }
+        }
+    });
 }
 
 void* sp_hip_session_stream(sp_hip_session* s) { return s ? static_cast<void*>(s->s->stream()) : nullptr; }

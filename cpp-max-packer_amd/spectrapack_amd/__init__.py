@@ -329,6 +329,17 @@ class Session:
             return None
         return a[: lx * ly * lz].reshape(lz, ly, lx)
 
+    def profile_step(self, stamp=True):
+        """One step without the graph; per-kernel milliseconds from HIP events."""
+        cap = 32
+        ms = np.zeros(cap, np.float32)
+        names = C.create_string_buffer(32 * cap)
+        n = C.c_uint32(0)
+        self._check(lib().sp_hip_session_profile_step(self._h, C.c_int(1 if stamp else 0), _p(ms, C.c_float), names,
+                                                      C.c_uint32(cap), C.byref(n)))
+        raw = names.raw
+        return [(raw[32 * i:32 * i + 32].split(b"\0")[0].decode(), float(ms[i])) for i in range(n.value)]
+
     def stream(self):
         return lib().sp_hip_session_stream(self._h)
 

@@ -177,6 +177,11 @@ int sp_hip_session_last_result(sp_hip_session* s, sp_step_result* out);
 int sp_hip_session_keep_fields(sp_hip_session* s, int keep);
 int sp_hip_session_overlap_field(sp_hip_session* s, uint32_t orientation, int32_t* out, uint64_t count,
                                  uint32_t legal_shape[3]);
+/* Runs one step without the graph, timing each kernel with HIP events:
+ * kernel_ms[i] and names (32 chars each) for up to capacity kernels; *count
+ * receives the number of kernels. */
+int sp_hip_session_profile_step(sp_hip_session* s, int stamp, float* kernel_ms, char* names, uint32_t capacity,
+                                uint32_t* count);
 /* Stream the session launches on (hipStream_t), for events around its work. */
 void* sp_hip_session_stream(sp_hip_session* s);
 /* Device bytes held by the session. */
