@@ -1,0 +1,329 @@
+// Register-FFT versions of the step's passes for power-of-R lengths (256 and
+// 512).  Same layouts and semantics as fft_kernels.hpp; W pencils per block,
+// P = N/R threads per pencil.
+#pragma once
+
+#include "fft_kernels.hpp"
+#include "sp_regfft.hpp"
+
+namespace sphip {
+
+template <typename T> __device__ __forceinline__ void load_tw_lds(cx<T>* dst, const cx<T>* tw, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = tw[i];
+}
+
+// Strided pencils (consecutive pencils contiguous in memory).  Same arguments
+// as k_c2c.  Lane mapping: pencil = lane % W (fastest), t = lane / W.
+template <typename T, int N, int R, int W>
+__global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long in_o, long long in_sn, long long in_sw,
+                                                      long long in_sb, cx<T>* out, long long out_o, long long out_sn,
+                                                      long long out_sw, long long out_sb, int count_w, int n_in,
+                                                      int n_out, const OrientDesc* descs, int mode,
+                                                      const cx<T>* __restrict__ tw, int fwd)
+{
+    using T2 = cx<T>;
+    using S = RegShape<N, R>;
+    constexpr int P = S::P;
+    extern __shared__ unsigned char smem_raw[];
+    T2* tws = reinterpret_cast<T2*>(smem_raw);
+    T2* lds = tws + N;
+    const int o = blockIdx.z;
+    int cw = count_w, ni = n_in, no = n_out;
+    if (descs) {
+        const OrientDesc d = descs[o];
+        if (mode == 1) {
+            if (d.lx <= 0) return;
+            cw = d.kx;
+            ni = d.ky;
+        }
+        else if (mode == 2) {
+            cw = d.lx;
+            no = d.ly;
+        }
+    }
+    const int p0 = blockIdx.x * W;
+    if (p0 >= cw) return;
+    const int w = threadIdx.x % W, t = threadIdx.x / W;
+    const bool live = p0 + w < cw;
+    const cx<T>* src = in + o * in_o + blockIdx.y * in_sb + (p0 + w) * in_sw;
+    T2* dst = out + o * out_o + blockIdx.y * out_sb + (p0 + w) * out_sw;
+    load_tw_lds<T>(tws, tw, N);
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int n = t + P * q;
+        v[q] = (live && n < ni) ? src[n * in_sn] : T2 { 0, 0 };
+    }
+    __syncthreads();
+    reg_fft<T, N, R>(v, lds + w * S::stride, tws, t, fwd != 0);
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            const int n = t + P * s;
+            if (n < no) dst[n * out_sn] = v[s];
+        }
+    }
+}
+
+// Contiguous pencils (rows).  Lane mapping: t = lane % P (fastest), pencil = lane / P.
+template <typename T, int N, int R, int W>
+__global__ void __launch_bounds__(W*(N / R)) rf_rows(cx<T>* data, int rows, long long row_pitch, long long batch_pitch,
+                                                      const cx<T>* __restrict__ tw, int fwd)
+{
+    using T2 = cx<T>;
+    using S = RegShape<N, R>;
+    constexpr int P = S::P;
+    extern __shared__ unsigned char smem_raw[];
+    T2* tws = reinterpret_cast<T2*>(smem_raw);
+    T2* lds = tws + N;
+    const int t = threadIdx.x % P, w = threadIdx.x / P;
+    const int r = blockIdx.x * W + w;
+    const bool live = r < rows;
+    T2* row = data + blockIdx.y * batch_pitch + (long long)r * row_pitch;
+    load_tw_lds<T>(tws, tw, N);
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = live ? row[t + P * q] : T2 { 0, 0 };
+    __syncthreads();
+    reg_fft<T, N, R>(v, lds + w * S::stride, tws, t, fwd != 0);
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) row[t + P * s] = v[s];
+    }
+}
+
+// The product pass, batched over orientations: a block owns W spectral rows
+// (kz, y0..y0+W) of the environment spectrum, keeps them in registers, and
+// for every orientation finishes the kernel's x transform from its partially
+// transformed row H[kz][y][x < kx], multiplies by the conjugate, runs the
+// inverse x transform and stores x < lx into out[o][kz][y][x].
+template <typename T, int N, int R, int W>
+__global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px,
+                                                       const cx<T>* __restrict__ SB, int ny, cx<T>* __restrict__ out,
+                                                       long long out_o, const OrientDesc* __restrict__ descs,
+                                                       const int* __restrict__ fit, int nfit, const cx<T>* __restrict__ tw)
+{
+    using T2 = cx<T>;
+    using S = RegShape<N, R>;
+    constexpr int P = S::P;
+    extern __shared__ unsigned char smem_raw[];
+    T2* tws = reinterpret_cast<T2*>(smem_raw);
+    T2* lds = tws + N;
+    const int t = threadIdx.x % P, w = threadIdx.x / P;
+    const int kz = blockIdx.y;
+    const int y = blockIdx.x * W + w;
+    const bool live = y < ny;
+    const long long row = (long long)kz * ny + y;
+    load_tw_lds<T>(tws, tw, N);
+    T2 sb[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) sb[s] = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
+    T2* pl = lds + w * S::stride;
+    for (int i = 0; i < nfit; ++i) {
+        const int o = fit[i];
+        const int kx = descs[o].kx, lx = descs[o].lx;
+        const T2* hrow = H + o * h_o + row * h_px;
+        T2 v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int x = t + P * q;
+            v[q] = (live && x < kx) ? hrow[x] : T2 { 0, 0 };
+        }
+        reg_fft<T, N, R>(v, pl, tws, t, true);
+#pragma unroll
+        for (int s = 0; s < R; ++s) v[s] = cmulc(sb[s], v[s]);
+        reg_fft<T, N, R>(v, pl, tws, t, false);
+        if (live) {
+            T2* orow = out + o * out_o + row * N;
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                const int x = t + P * s;
+                if (x < lx) orow[x] = v[s];
+            }
+        }
+    }
+}
+
+// R2C along z with column pairs: W real columns (x) per block at row y, two
+// columns per complex pencil.  Input uint8 or real; descs selects kernel mode.
+template <typename T, int N, int R, int W, typename In>
+__global__ void __launch_bounds__((W / 2) * (N / R)) rf_r2c_z(const In* __restrict__ in, int in_px, int in_py, int ex,
+                                                               int ey, int ez, cx<T>* __restrict__ out, long long out_bs,
+                                                               int out_px, int out_py, int cx_, int cy_,
+                                                               const OrientDesc* descs, const cx<T>* __restrict__ tw)
+{
+    using T2 = cx<T>;
+    using S = RegShape<N, R>;
+    constexpr int P = S::P;
+    constexpr int PW = W / 2;
+    extern __shared__ unsigned char smem_raw[];
+    T2* tws = reinterpret_cast<T2*>(smem_raw);
+    T2* lds = tws + N;
+    const int b = blockIdx.z;
+    int ex_ = ex, ey_ = ey, ez_ = ez, px = in_px, py = in_py, cxx = cx_, cyy = cy_;
+    const In* src = in;
+    if (descs) {
+        const OrientDesc d = descs[b];
+        if (d.lx <= 0) return;
+        ex_ = px = cxx = d.kx;
+        ey_ = py = cyy = d.ky;
+        ez_ = d.kz;
+        src = in + d.cells_offset;
+    }
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * W;
+    if (y >= cyy || x0 >= cxx) return;
+    const int p = threadIdx.x % PW, t = threadIdx.x / PW;
+    load_tw_lds<T>(tws, tw, N);
+    const int xa = x0 + p, xb = x0 + p + PW;
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int z = t + P * q;
+        const bool zin = y < ey_ && z < ez_;
+        const T a = (zin && xa < ex_) ? T(to_real(src[xa + (long long)px * (y + (long long)py * z)])) : T(0);
+        const T bb = (zin && xb < ex_) ? T(to_real(src[xb + (long long)px * (y + (long long)py * z)])) : T(0);
+        v[q] = T2 { a, bb };
+    }
+    __syncthreads();
+    T2* pl = lds + p * S::stride;
+    reg_fft<T, N, R>(v, pl, tws, t, true);
+    // unpack needs X[N-k]: exchange the spectrum through LDS
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < R; ++s) pl[S::pad(t + P * s)] = v[s];
+    __syncthreads();
+    T2* dst = out + b * out_bs;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int k = t + P * s;
+        if (k > N / 2) continue;
+        const T2 xk = v[s];
+        const T2 xn = pl[S::pad((N - k) & (N - 1))];
+        const T2 A = { T(0.5) * (xk.x + xn.x), T(0.5) * (xk.y - xn.y) };
+        const T2 B = { T(0.5) * (xk.y + xn.y), T(0.5) * (xn.x - xk.x) };
+        const long long base = (long long)out_px * (y + (long long)out_py * k);
+        if (xa < cxx) dst[base + xa] = A;
+        if (xb < cxx) dst[base + xb] = B;
+    }
+}
+
+// C2R along z of column pairs + rounding, the reference's per-value checks,
+// the feasible mask and the lowest (z, y, x) key (see k_c2r_z_argmin).
+template <typename T, int N, int R, int W>
+__global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>* __restrict__ in, long long in_o,
+                                                                      int nx, int ny, const OrientDesc* __restrict__ descs,
+                                                                      T inv_total, StepDeviceResult* __restrict__ result,
+                                                                      int* __restrict__ fields,
+                                                                      const long long* __restrict__ field_offsets,
+                                                                      const cx<T>* __restrict__ tw)
+{
+    using T2 = cx<T>;
+    using S = RegShape<N, R>;
+    constexpr int P = S::P;
+    constexpr int PW = W / 2;
+    extern __shared__ unsigned char smem_raw[];
+    T2* tws = reinterpret_cast<T2*>(smem_raw);
+    T2* lds = tws + N;
+    __shared__ unsigned long long best;
+    __shared__ unsigned int res_bits, flags;
+    const int o = blockIdx.z;
+    const OrientDesc d = descs[o];
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * W;
+    if (d.lx <= 0 || d.ly <= 0 || d.lz <= 0 || y >= d.ly || x0 >= d.lx) return;
+    if (threadIdx.x == 0) {
+        best = ~0ULL;
+        res_bits = 0;
+        flags = 0;
+    }
+    const int p = threadIdx.x % PW, t = threadIdx.x / PW;
+    load_tw_lds<T>(tws, tw, N);
+    const int xa = x0 + p, xb = x0 + p + PW;
+    const cx<T>* src = in + o * in_o + (long long)nx * y + x0;
+    const long long plane = (long long)nx * ny;
+    // Stage the half spectrum once: rows k in [0, N/2] of the W columns, then
+    // every thread builds its Hermitian-extended inputs from LDS.
+    constexpr int NH = N / 2 + 1;
+    constexpr int SROW = W + 1;
+    T2* stage = lds;  // NH * SROW elements, reused by the FFT exchange afterwards
+    const int lim = d.lx - x0;
+    {
+        // all loads of this thread first (memory-level parallelism), then LDS
+        constexpr int BLOCK = (W / 2) * (N / R);
+        constexpr int ITER = (NH * W + BLOCK - 1) / BLOCK;
+        T2 tmp[ITER];
+#pragma unroll
+        for (int j = 0; j < ITER; ++j) {
+            const int i = threadIdx.x + j * BLOCK;
+            const int k = i / W, c = i % W;
+            tmp[j] = (i < NH * W && c < lim) ? src[k * plane + c] : T2 { 0, 0 };
+        }
+#pragma unroll
+        for (int j = 0; j < ITER; ++j) {
+            const int i = threadIdx.x + j * BLOCK;
+            if (i < NH * W) stage[(i / W) * SROW + (i % W)] = tmp[j];
+        }
+    }
+    __syncthreads();
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int k = t + P * q;
+        const bool mirror = k > N / 2;
+        const int kk = mirror ? N - k : k;
+        const T2 a = stage[kk * SROW + p];
+        const T2 b = stage[kk * SROW + p + PW];
+        v[q] = mirror ? T2 { a.x + b.y, b.x - a.y } : T2 { a.x - b.y, a.y + b.x };
+    }
+    __syncthreads();
+    reg_fft<T, N, R>(v, lds + p * S::stride, tws, t, false);
+    const T max_out = T(d.mass < (long long)result->env_mass ? d.mass : (long long)result->env_mass);
+    unsigned long long mine = ~0ULL;
+    float my_res = 0.f;
+    unsigned int my_flags = 0;
+    int* fo = fields ? fields + field_offsets[o] : nullptr;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int z = t + P * s;
+        if (z >= d.lz) continue;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int x = half ? xb : xa;
+            if (x >= d.lx) continue;
+            const T val = (half ? v[s].y : v[s].x) * inv_total;
+            const T rv = round(val);
+            const T res = fabs(val - rv);
+            my_res = fmaxf(my_res, float(res));
+            if (!(res < T(0.25))) my_flags |= 1u;
+            if (rv < T(0) || rv > max_out) my_flags |= 2u;
+            if (fo) fo[x + (long long)d.lx * (y + (long long)d.ly * z)] = int(rv);
+            if (rv == T(0)) {
+                const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
+                                               ((unsigned long long)x << 16) | (unsigned long long)o;
+                mine = key < mine ? key : mine;
+            }
+        }
+    }
+    // wave-level reduction, then one LDS atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long om = __shfl_down(mine, off, 64);
+        mine = om < mine ? om : mine;
+        my_res = fmaxf(my_res, __shfl_down(my_res, off, 64));
+        my_flags |= __shfl_down(my_flags, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (mine != ~0ULL) atomicMin(&best, mine);
+        if (my_res > 0.f) atomicMax(&res_bits, __float_as_uint(my_res));
+        if (my_flags) atomicOr(&flags, my_flags);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (best != ~0ULL) atomicMin(&result->key, best);
+        if (res_bits) atomicMax(&result->max_residual_bits, res_bits);
+        if (flags) atomicOr(&result->flags, flags);
+    }
+}
+
+}  // namespace sphip

@@ -1,0 +1,103 @@
+// Register-resident FFTs for lengths N = R^k (R = 8 or 16) on gfx950.
+//
+// A pencil of N points is owned by P = N/R threads, each holding R values in
+// registers.  The transform is a Stockham sequence of k radix-R passes; the
+// first pass reads x[t + P*q] straight from global memory (coalesced across t),
+// the last pass leaves X[t + P*s] in thread t's registers (natural order,
+// stride P), and passes exchange through LDS only between them — one exchange
+// for N = 256 (16 x 16), two for N = 512 (8 x 8 x 8).  The output distribution
+// of a forward transform equals the input distribution of the next one, so a
+// product of spectra followed by an inverse transform needs no exchange.
+#pragma once
+
+#include "sp_fft.hpp"
+
+namespace sphip {
+
+// Radix-16 in registers as 4 x 4 with the w16 twiddles in between.
+template <typename T> __device__ __forceinline__ void dft16(cx<T>* v, bool fwd)
+{
+    using T2 = cx<T>;
+    const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173), h = T(0.70710678118654752440);
+    T2 y[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        T2 a[4] = { v[n2], v[4 + n2], v[8 + n2], v[12 + n2] };
+        Dft<T, 4>::run(a, fwd);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) y[n2 * 4 + k1] = a[k1];
+    }
+    // twiddle y[n2][k1] *= w16^(n2*k1)
+    const T sg = fwd ? T(-1) : T(1);
+    // w16^m = cos(2 pi m/16) + sg*i*sin(2 pi m/16)
+    auto tw = [&](T2 a, T c, T s) { return T2 { a.x * c - a.y * sg * s, a.x * sg * s + a.y * c }; };
+    y[1 * 4 + 1] = tw(y[1 * 4 + 1], c1, s1);   // m=1
+    y[1 * 4 + 2] = tw(y[1 * 4 + 2], h, h);     // m=2
+    y[1 * 4 + 3] = tw(y[1 * 4 + 3], s1, c1);   // m=3
+    y[2 * 4 + 1] = tw(y[2 * 4 + 1], h, h);     // m=2
+    y[2 * 4 + 2] = mul_mi(y[2 * 4 + 2], fwd);  // m=4
+    y[2 * 4 + 3] = tw(y[2 * 4 + 3], -h, h);    // m=6
+    y[3 * 4 + 1] = tw(y[3 * 4 + 1], s1, c1);   // m=3
+    y[3 * 4 + 2] = tw(y[3 * 4 + 2], -h, h);    // m=6
+    y[3 * 4 + 3] = tw(y[3 * 4 + 3], -c1, -s1); // m=9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        T2 a[4] = { y[0 * 4 + k1], y[1 * 4 + k1], y[2 * 4 + k1], y[3 * 4 + k1] };
+        Dft<T, 4>::run(a, fwd);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = a[k2];
+    }
+}
+
+template <typename T, int R> __device__ __forceinline__ void dft_reg(cx<T>* v, bool fwd)
+{
+    if constexpr (R == 16) dft16<T>(v, fwd);
+    else Dft<T, R>::run(v, fwd);
+}
+
+template <int N, int R> struct RegShape {
+    static constexpr int P = N / R;
+    static constexpr int passes = (N == R) ? 1 : (N == R * R) ? 2 : (N == R * R * R) ? 3 : (N == R * R * R * R) ? 4 : -1;
+    static_assert(passes > 0, "N must be a power of R");
+    // LDS elements of one pencil: one pad slot every 16 plus one, so lanes of
+    // consecutive pencils and stride-16 accesses land in distinct banks.
+    static constexpr int stride = N + N / 16 + 1;
+    __device__ static constexpr int pad(int i) { return i + (i >> 4); }
+};
+
+// Runs the transform for one pencil.  v: in = x[t + P*q], out = X[t + P*s].
+// lds: this pencil's region (RegShape::stride elements); tw: exp(-2 pi i k/N)
+// table (LDS).  Every thread of the block must call it (barriers inside).
+template <typename T, int N, int R>
+__device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, int t, bool fwd)
+{
+    using T2 = cx<T>;
+    using S = RegShape<N, R>;
+    int l = 1;
+#pragma unroll
+    for (int p = 0; p < S::passes; ++p) {
+        const int m = N / (l * R);
+        const int j = t % l, k = t / l;
+        if (p > 0) {
+            // gather this pass's inputs: src[(q*m + k)*l + j]
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[q] = lds[S::pad((q * m + k) * l + j)];
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                T2 w = tw[(j * q * m) & (N - 1)];
+                if (!fwd) w.y = -w.y;
+                v[q] = cmul(v[q], w);
+            }
+        }
+        dft_reg<T, R>(v, fwd);
+        if (p + 1 < S::passes) {
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < R; ++s) lds[S::pad((k * R + s) * l + j)] = v[s];
+        }
+        l *= R;
+    }
+}
+
+}  // namespace sphip

@@ -11,6 +11,7 @@
 
 #include "common.hpp"
 #include "fft_kernels.hpp"
+#include "rf_kernels.hpp"
 
 namespace sphip {
 
@@ -69,6 +70,105 @@ __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsign
         }
     }
     if (added) atomicAdd(&r->env_mass, added);
+}
+
+
+// ---- register-FFT dispatch for power-of-R lengths -------------------------
+template <typename T, int N, int R> constexpr std::size_t rf_lds(int pencils)
+{
+    return sizeof(cx<T>) * (std::size_t(N) + std::size_t(pencils) * RegShape<N, R>::stride);
+}
+
+inline bool rf_supported(int n) { return n == 256 || n == 512; }
+
+template <typename T>
+void rf_cols_launch(int n, hipStream_t st, int count_w_max, int batches, int orients, const cx<T>* in, long long in_o,
+                    long long in_sn, long long in_sw, long long in_sb, cx<T>* out, long long out_o, long long out_sn,
+                    long long out_sw, long long out_sb, int count_w, int n_in, int n_out, const OrientDesc* descs,
+                    int mode, const cx<T>* tw, int fwd)
+{
+    if (n == 256) {
+        constexpr int N = 256, R = 16, W = 32;
+        hipLaunchKernelGGL((rf_cols<T, N, R, W>), dim3((count_w_max + W - 1) / W, batches, orients), dim3(W * (N / R)),
+                           (rf_lds<T, N, R>(W)), st, in, in_o, in_sn, in_sw, in_sb, out, out_o, out_sn, out_sw, out_sb,
+                           count_w, n_in, n_out, descs, mode, tw, fwd);
+    }
+    else {
+        constexpr int N = 512, R = 8, W = 16;
+        hipLaunchKernelGGL((rf_cols<T, N, R, W>), dim3((count_w_max + W - 1) / W, batches, orients), dim3(W * (N / R)),
+                           (rf_lds<T, N, R>(W)), st, in, in_o, in_sn, in_sw, in_sb, out, out_o, out_sn, out_sw, out_sb,
+                           count_w, n_in, n_out, descs, mode, tw, fwd);
+    }
+}
+
+template <typename T>
+void rf_rows_launch(int n, hipStream_t st, cx<T>* data, int rows, long long row_pitch, int batches, long long batch_pitch,
+                    const cx<T>* tw, int fwd)
+{
+    if (n == 256) {
+        constexpr int N = 256, R = 16, W = 16;
+        hipLaunchKernelGGL((rf_rows<T, N, R, W>), dim3((rows + W - 1) / W, batches), dim3(W * (N / R)), (rf_lds<T, N, R>(W)),
+                           st, data, rows, row_pitch, batch_pitch, tw, fwd);
+    }
+    else {
+        constexpr int N = 512, R = 8, W = 4;
+        hipLaunchKernelGGL((rf_rows<T, N, R, W>), dim3((rows + W - 1) / W, batches), dim3(W * (N / R)), (rf_lds<T, N, R>(W)),
+                           st, data, rows, row_pitch, batch_pitch, tw, fwd);
+    }
+}
+
+template <typename T>
+void rf_mul_launch(int n, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px, const cx<T>* SB,
+                   cx<T>* out, long long out_o, const OrientDesc* descs, const int* fit, int count, const cx<T>* tw)
+{
+    if (n == 256) {
+        constexpr int N = 256, R = 16, W = 16;
+        hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st,
+                           H, h_o, h_px, SB, ny, out, out_o, descs, fit, count, tw);
+    }
+    else {
+        constexpr int N = 512, R = 8, W = 4;
+        hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st,
+                           H, h_o, h_px, SB, ny, out, out_o, descs, fit, count, tw);
+    }
+}
+
+template <typename T, typename In>
+void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batches, const In* in, int in_px, int in_py,
+                   int ex, int ey, int ez, cx<T>* out, long long out_bs, int out_px, int out_py, int cx_, int cy_,
+                   const OrientDesc* descs, const cx<T>* tw)
+{
+    if (n == 256) {
+        constexpr int N = 256, R = 16, W = 32;
+        hipLaunchKernelGGL((rf_r2c_z<T, N, R, W, In>), dim3((cols_max + W - 1) / W, rows_max, batches),
+                           dim3((W / 2) * (N / R)), (rf_lds<T, N, R>(W / 2)), st, in, in_px, in_py, ex, ey, ez, out,
+                           out_bs, out_px, out_py, cx_, cy_, descs, tw);
+    }
+    else {
+        constexpr int N = 512, R = 8, W = 16;
+        hipLaunchKernelGGL((rf_r2c_z<T, N, R, W, In>), dim3((cols_max + W - 1) / W, rows_max, batches),
+                           dim3((W / 2) * (N / R)), (rf_lds<T, N, R>(W / 2)), st, in, in_px, in_py, ex, ey, ez, out,
+                           out_bs, out_px, out_py, cx_, cy_, descs, tw);
+    }
+}
+
+template <typename T>
+void rf_c2r_launch(int n, hipStream_t st, int lx_max, int ly_max, int orients, const cx<T>* in, long long in_o, int nx,
+                   int ny, const OrientDesc* descs, T inv_total, StepDeviceResult* res, int* fields,
+                   const long long* offsets, const cx<T>* tw)
+{
+    if (n == 256) {
+        constexpr int N = 256, R = 16, W = 64;
+        const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
+        hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
+                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, tw);
+    }
+    else {
+        constexpr int N = 512, R = 8, W = 16;
+        const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
+        hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
+                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, tw);
+    }
 }
 
 }  // namespace
@@ -151,6 +251,23 @@ public:
         prepared_ = false;
     }
 
+    // Selects the register-FFT passes for power-of-R lengths (default) or the
+    // generic LDS Stockham passes everywhere (for cross-checking the two).
+    void set_fast_path(bool on)
+    {
+        if (on != use_rf_) {
+            use_rf_ = on;
+            if (graph_exec_) {
+                (void)hipGraphExecDestroy(graph_exec_);
+                graph_exec_ = nullptr;
+            }
+            if (graph_) {
+                (void)hipGraphDestroy(graph_);
+                graph_ = nullptr;
+            }
+        }
+    }
+
     void keep_fields(bool keep)
     {
         if (keep != keep_fields_) {
@@ -211,6 +328,13 @@ public:
         SP_HIP_CHECK(hipMemcpyAsync(kcells_.get(), all.data(), all.size(), hipMemcpyHostToDevice, stream_));
         d_descs_.allocate(sizeof(OrientDesc) * O_);
         SP_HIP_CHECK(hipMemcpyAsync(d_descs_.get(), descs_.data(), sizeof(OrientDesc) * O_, hipMemcpyHostToDevice, stream_));
+        std::vector<int> fit;
+        for (int o = 0; o != O_; ++o)
+            if (descs_[o].lx > 0) fit.push_back(o);
+        nfit_ = int(fit.size());
+        d_fit_.allocate(sizeof(int) * std::max<std::size_t>(fit.size(), 1));
+        if (!fit.empty())
+            SP_HIP_CHECK(hipMemcpyAsync(d_fit_.get(), fit.data(), sizeof(int) * fit.size(), hipMemcpyHostToDevice, stream_));
         d_offsets_.allocate(sizeof(long long) * O_);
         SP_HIP_CHECK(hipMemcpyAsync(d_offsets_.get(), field_offsets_.data(), sizeof(long long) * O_,
                                     hipMemcpyHostToDevice, stream_));
@@ -272,7 +396,12 @@ public:
         hipLaunchKernelGGL(k_reset_result, dim3(1), dim3(64), 0, stream_, res);
         mark("reset");
         // E1: environment R2C along z (uint8 -> spectrum planes)
-        {
+        if (rf_supported(nz) && use_rf_) {
+            rf_r2c_launch<T, unsigned char>(nz, stream_, nx, ny, 1, env_.as<unsigned char>(), ext_[0], ext_[1], ext_[0],
+                                            ext_[1], ext_[2], SB, 0LL, nx, ny, nx, ny, nullptr, twz);
+            mark("env_r2c_z");
+        }
+        else {
             const int W = choose_w(nz, e, 2, true, 32, 2);
             hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((nx + W - 1) / W, ny, 1), dim3(kThreads),
                                lds_bytes<T>(nz, 2, W / 2), stream_, env_.as<unsigned char>(), 0LL, ext_[0], ext_[1],
@@ -280,7 +409,12 @@ public:
             mark("env_r2c_z");
         }
         // E2: C2C along y (columns x, batch kz)
-        {
+        if (rf_supported(ny) && use_rf_) {
+            rf_cols_launch<T>(ny, stream_, nx, nh, 1, SB, 0LL, (long long)nx, 1LL, (long long)nx * ny, SB, 0LL,
+                              (long long)nx, 1LL, (long long)nx * ny, nx, ny, ny, nullptr, 0, twy, 1);
+            mark("env_c2c_y");
+        }
+        else {
             const int W = choose_w(ny, e, 2, false, 16, 1);
             hipLaunchKernelGGL((k_c2c<T>), dim3((nx + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(ny, 2, W),
                                stream_, SB, 0LL, (long long)nx, 1LL, (long long)nx * ny, SB, 0LL, (long long)nx, 1LL,
@@ -288,7 +422,11 @@ public:
             mark("env_c2c_y");
         }
         // E3: C2C along x (rows, batch kz)
-        {
+        if (rf_supported(nx) && use_rf_) {
+            rf_rows_launch<T>(nx, stream_, SB, ny, (long long)nx, nh, (long long)nx * ny, twx, 1);
+            mark("env_c2c_x");
+        }
+        else {
             const int W = choose_w(nx, e, 2, false, 16, 1);
             hipLaunchKernelGGL((k_c2c<T>), dim3((ny + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(nx, 2, W),
                                stream_, SB, 0LL, 1LL, (long long)nx, (long long)nx * ny, SB, 0LL, 1LL, (long long)nx,
@@ -299,7 +437,12 @@ public:
         const long long h2o = (long long)nh * ny * kmax_[0];
         const long long to = (long long)nh * ny * nx;
         // G1: kernel R2C along z over its own columns (x < kx, y < ky)
-        {
+        if (rf_supported(nz) && use_rf_) {
+            rf_r2c_launch<T, unsigned char>(nz, stream_, kmax_[0], kmax_[1], O_, kcells_.as<unsigned char>(), 0, 0, 0, 0, 0,
+                                            H1, h1o, kmax_[0], kmax_[1], 0, 0, dd, twz);
+            mark("kernel_r2c_z");
+        }
+        else {
             const int W = choose_w(nz, e, 2, true, 32, 2);
             hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((kmax_[0] + W - 1) / W, kmax_[1], O_), dim3(kThreads),
                                lds_bytes<T>(nz, 2, W / 2), stream_, kcells_.as<unsigned char>(), 0LL, 0, 0, 0, 0, 0, H1,
@@ -307,7 +450,12 @@ public:
             mark("kernel_r2c_z");
         }
         // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
-        {
+        if (rf_supported(ny) && use_rf_) {
+            rf_cols_launch<T>(ny, stream_, kmax_[0], nh, O_, H1, h1o, (long long)kmax_[0], 1LL, (long long)kmax_[1] * kmax_[0],
+                              H2, h2o, (long long)kmax_[0], 1LL, (long long)ny * kmax_[0], 0, 0, ny, dd, 1, twy, 1);
+            mark("kernel_c2c_y");
+        }
+        else {
             const int W = choose_w(ny, e, 2, false, 16, 1);
             hipLaunchKernelGGL((k_c2c<T>), dim3((kmax_[0] + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(ny, 2, W),
                                stream_, H1, h1o, (long long)kmax_[0], 1LL, (long long)kmax_[1] * kmax_[0], H2, h2o,
@@ -315,7 +463,11 @@ public:
             mark("kernel_c2c_y");
         }
         // P1: finish kernel x transform, product with conj, inverse x
-        {
+        if (rf_supported(nx) && use_rf_) {
+            rf_mul_launch<T>(nx, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, d_fit_.as<int>(), nfit_, twx);
+            mark("mul_x");
+        }
+        else {
             const int W = choose_w(nx, e, 4, false, 16, 1);
             hipLaunchKernelGGL((k_mul_x<T>), dim3((ny + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(nx, 4, W),
                                stream_, H2, h2o, kmax_[0], ny, SB, nx, ny, TT, to, dd, 0, 0, twx, plans_[0], W);
@@ -323,17 +475,31 @@ public:
         }
         // P2: inverse C2C along y for the legal columns, rows y < ly written
         if (lmax_[0] > 0) {
-            const int W = choose_w(ny, e, 2, false, 16, 1);
-            hipLaunchKernelGGL((k_c2c<T>), dim3((lmax_[0] + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(ny, 2, W),
-                               stream_, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to, (long long)nx, 1LL,
-                               (long long)nx * ny, 0, ny, 0, dd, 2, twy, plans_[1], W, 0);
+            // P2: inverse C2C along y for the legal columns, rows y < ly written
+            if (rf_supported(ny) && use_rf_) {
+                rf_cols_launch<T>(ny, stream_, lmax_[0], nh, O_, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
+                                  (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, twy, 0);
+            }
+            else {
+                const int W = choose_w(ny, e, 2, false, 16, 1);
+                hipLaunchKernelGGL((k_c2c<T>), dim3((lmax_[0] + W - 1) / W, nh, O_), dim3(kThreads),
+                                   lds_bytes<T>(ny, 2, W), stream_, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT,
+                                   to, (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, twy, plans_[1], W, 0);
+            }
             mark("inv_c2c_y");
             // P3: C2R along z + round + checks + feasible + lowest (z, y, x)
-            const int W3 = choose_w(nz, e, 2, true, 32, 2);
             const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
-            hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], O_), dim3(kThreads),
-                               lds_bytes<T>(nz, 2, W3 / 2), stream_, TT, to, nx, ny, dd, inv_total, res,
-                               keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(), twz, plans_[2], W3);
+            if (rf_supported(nz) && use_rf_) {
+                rf_c2r_launch<T>(nz, stream_, lmax_[0], lmax_[1], O_, TT, to, nx, ny, dd, inv_total, res,
+                                 keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(), twz);
+            }
+            else {
+                const int W3 = choose_w(nz, e, 2, true, 32, 2);
+                hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], O_), dim3(kThreads),
+                                   lds_bytes<T>(nz, 2, W3 / 2), stream_, TT, to, nx, ny, dd, inv_total, res,
+                                   keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(), twz, plans_[2],
+                                   W3);
+            }
             mark("c2r_z_argmin");
         }
         if (stamp) {
@@ -474,7 +640,8 @@ private:
     std::array<int, 3> kmax_ {}, lmax_ {};
     FftPlan plans_[3] {};
     hipStream_t stream_ {};
-    DeviceBuffer env_, result_, kcells_, d_descs_, d_offsets_, fields_, sb_, h1_, h2_, t_;
+    DeviceBuffer env_, result_, kcells_, d_descs_, d_offsets_, d_fit_, fields_, sb_, h1_, h2_, t_;
+    int nfit_ {};
     StepDeviceResult* host_result_ {};
     long long fields_total_ {};
     bool keep_fields_ {}, prepared_ {};
@@ -484,6 +651,7 @@ private:
     hipGraphExec_t graph_exec_ {};
     bool graph_stamp_ {};
     bool prof_ {};
+    bool use_rf_ { true };
     std::vector<hipEvent_t> prof_events_;
     std::vector<const char*> prof_names_;
 };
@@ -577,6 +745,12 @@ int sp_hip_session_set_kernel(sp_hip_session* s, uint32_t o, const uint8_t* cell
 {
     if (!s || !cells || !shape || !first) return SP_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] { s->s->set_kernel(o, cells, shape, first); });
+}
+
+int sp_hip_session_set_fast_path(sp_hip_session* s, int on)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_fast_path(on != 0); });
 }
 
 int sp_hip_session_keep_fields(sp_hip_session* s, int keep)
