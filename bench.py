@@ -88,7 +88,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 4, "dispatches": 3}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the placement step runs on a single GPU in this build "
               "(multi-GPU orientation sharding is a design note only); run with --gpus 1.", file=sys.stderr)
@@ -158,6 +170,13 @@ def main():
     step_achieved = step_bytes / (ms_step * 1e-3) / 1e9
     log("[bench] kernels: " + ", ".join(f"{k} {v:.3f} ms" for k, v in prof.items()))
 
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tj = json.load(f)
+        if tj.get("precision") == prec and tj.get("kernel") == dominant:
+            traffic = tj.get("hbm_bytes_per_launch")
     out = {
         "metric": METRIC,
         "value": 1000.0 / ms_step,
@@ -175,7 +194,7 @@ def main():
                    "pitch_mm": w.pitch, "precision": f"fp{prec}", "error_bound": r["error_bound"],
                    "graph": not args.no_graph, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
         "step_roofline": {"model_bytes_per_step": step_bytes, "achieved": step_achieved, "peak": PEAK_HBM_GBS,
                           "unit": "GB/s", "frac": step_achieved / PEAK_HBM_GBS},

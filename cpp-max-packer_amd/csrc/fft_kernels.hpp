@@ -29,6 +29,21 @@ struct StepDeviceResult {
     unsigned long long pad;
 };
 
+// Rejected translations (gap_fill.cpp:517-520): excl[0] = count, then sorted
+// packed keys.  A rejected key is never chosen again.
+__device__ __forceinline__ bool excluded(const unsigned long long* excl, unsigned long long key)
+{
+    const unsigned long long n = excl[0];
+    if (n == 0) return false;
+    unsigned long long lo = 0, hi = n;
+    while (lo < hi) {
+        const unsigned long long mid = (lo + hi) / 2;
+        if (excl[1 + mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && excl[1 + lo] == key;
+}
+
 template <typename T> __device__ __forceinline__ void load_twiddles(cx<T>* lds_tw, const cx<T>* tw, int n)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) lds_tw[i] = tw[i];
@@ -255,6 +270,7 @@ __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ 
                                                       const OrientDesc* __restrict__ descs, T inv_total,
                                                       StepDeviceResult* __restrict__ result, int* __restrict__ fields,
                                                       const long long* __restrict__ field_offsets,
+                                                      const unsigned long long* __restrict__ excl,
                                                       const cx<T>* __restrict__ tw, FftPlan plan, int W)
 {
     using T2 = cx<T>;
@@ -305,7 +321,7 @@ __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ 
         if (rv == T(0)) {
             const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                            ((unsigned long long)x << 16) | (unsigned long long)o;
-            mine = key < mine ? key : mine;
+            if (!excluded(excl, key)) mine = key < mine ? key : mine;
         }
     }
     if (mine != ~0ULL) atomicMin(&best, mine);

@@ -217,6 +217,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
                                                                       T inv_total, StepDeviceResult* __restrict__ result,
                                                                       int* __restrict__ fields,
                                                                       const long long* __restrict__ field_offsets,
+                                                                      const unsigned long long* __restrict__ excl,
                                                                       const cx<T>* __restrict__ tw)
 {
     using T2 = cx<T>;
@@ -302,7 +303,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
             if (rv == T(0)) {
                 const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                                ((unsigned long long)x << 16) | (unsigned long long)o;
-                mine = key < mine ? key : mine;
+                if (!excluded(excl, key)) mine = key < mine ? key : mine;
             }
         }
     }

@@ -18,6 +18,7 @@ namespace sphip {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr std::size_t kMaxExcluded = 65536;
 constexpr std::size_t kLdsBudget = 64 * 1024;
 
 // Largest tile width (power of two, <= cap) whose LDS footprint fits.
@@ -44,6 +45,11 @@ __global__ void k_reset_result(StepDeviceResult* r)
         r->flags = 0;
         r->max_residual_bits = 0;
     }
+}
+
+__global__ void k_set_key(StepDeviceResult* r, unsigned long long key)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) r->key = key;
 }
 
 // ORs the chosen kernel into the occupancy (gap_fill.cpp:98-121 for the chosen
@@ -155,19 +161,19 @@ void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batche
 template <typename T>
 void rf_c2r_launch(int n, hipStream_t st, int lx_max, int ly_max, int orients, const cx<T>* in, long long in_o, int nx,
                    int ny, const OrientDesc* descs, T inv_total, StepDeviceResult* res, int* fields,
-                   const long long* offsets, const cx<T>* tw)
+                   const long long* offsets, const unsigned long long* excl, const cx<T>* tw)
 {
     if (n == 256) {
         constexpr int N = 256, R = 16, W = 64;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
         hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
-                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, tw);
+                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, tw);
     }
     else {
         constexpr int N = 512, R = 8, W = 16;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
         hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
-                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, tw);
+                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, tw);
     }
 }
 
@@ -196,6 +202,8 @@ public:
         r0.key = ~0ULL;
         SP_HIP_CHECK(hipMemcpyAsync(result_.get(), &r0, sizeof r0, hipMemcpyHostToDevice, stream_));
         SP_HIP_CHECK(hipHostMalloc(&host_result_, sizeof(StepDeviceResult), hipHostMallocDefault));
+        excl_.allocate(sizeof(unsigned long long) * (1 + kMaxExcluded));
+        SP_HIP_CHECK(hipMemsetAsync(excl_.get(), 0, sizeof(unsigned long long), stream_));
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
     }
     ~Session()
@@ -266,6 +274,52 @@ public:
                 graph_ = nullptr;
             }
         }
+    }
+
+    // gap_fill.cpp:517-520: a rejected proposal's translation stays blocked
+    // for that orientation.
+    void exclude(std::uint32_t o, const std::int64_t t[3])
+    {
+        if (int(o) >= O_ || !kernels_[o].set) throw std::invalid_argument("orientation index");
+        unsigned long long key = o;
+        const int shift[3] = { 16, 32, 48 };
+        for (int a = 0; a != 3; ++a) {
+            const std::int64_t u = t[a] + kernels_[o].first[a] - efirst_[a];
+            if (u < 0 || u > 0xffff) throw std::invalid_argument("translation outside the window");
+            key |= (unsigned long long)u << shift[a];
+        }
+        auto it = std::lower_bound(excluded_.begin(), excluded_.end(), key);
+        if (it != excluded_.end() && *it == key) return;
+        if (excluded_.size() >= kMaxExcluded) throw std::length_error("too many rejected translations");
+        excluded_.insert(it, key);
+        std::vector<unsigned long long> buf(1 + excluded_.size());
+        buf[0] = excluded_.size();
+        std::copy(excluded_.begin(), excluded_.end(), buf.begin() + 1);
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(excl_.get(), buf.data(), buf.size() * sizeof(unsigned long long),
+                                    hipMemcpyHostToDevice, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    // Stamp kernel o at translation t (outside the step graph).
+    void stamp_at(std::uint32_t o, const std::int64_t t[3])
+    {
+        prepare();
+        if (int(o) >= O_ || descs_[o].lx <= 0) throw std::invalid_argument("orientation index");
+        unsigned long long key = o;
+        const int shift[3] = { 16, 32, 48 };
+        for (int a = 0; a != 3; ++a) {
+            const std::int64_t u = t[a] + kernels_[o].first[a] - efirst_[a];
+            if (u < 0 || u > 0xffff) throw std::invalid_argument("translation outside the window");
+            key |= (unsigned long long)u << shift[a];
+        }
+        use_device(ctx_);
+        StepDeviceResult* res = result_.as<StepDeviceResult>();
+        hipLaunchKernelGGL(k_set_key, dim3(1), dim3(64), 0, stream_, res, key);
+        hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
+                           ext_[2], kcells_.as<unsigned char>(), d_descs_.as<OrientDesc>(), res);
+        SP_HIP_CHECK(hipGetLastError());
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
     void keep_fields(bool keep)
@@ -491,14 +545,15 @@ public:
             const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
             if (rf_supported(nz) && use_rf_) {
                 rf_c2r_launch<T>(nz, stream_, lmax_[0], lmax_[1], O_, TT, to, nx, ny, dd, inv_total, res,
-                                 keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(), twz);
+                                 keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
+                                 excl_.as<unsigned long long>(), twz);
             }
             else {
                 const int W3 = choose_w(nz, e, 2, true, 32, 2);
                 hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], O_), dim3(kThreads),
                                    lds_bytes<T>(nz, 2, W3 / 2), stream_, TT, to, nx, ny, dd, inv_total, res,
-                                   keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(), twz, plans_[2],
-                                   W3);
+                                   keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
+                                   excl_.as<unsigned long long>(), twz, plans_[2], W3);
             }
             mark("c2r_z_argmin");
         }
@@ -640,7 +695,8 @@ private:
     std::array<int, 3> kmax_ {}, lmax_ {};
     FftPlan plans_[3] {};
     hipStream_t stream_ {};
-    DeviceBuffer env_, result_, kcells_, d_descs_, d_offsets_, d_fit_, fields_, sb_, h1_, h2_, t_;
+    DeviceBuffer env_, result_, kcells_, d_descs_, d_offsets_, d_fit_, fields_, sb_, h1_, h2_, t_, excl_;
+    std::vector<unsigned long long> excluded_;
     int nfit_ {};
     StepDeviceResult* host_result_ {};
     long long fields_total_ {};
@@ -751,6 +807,18 @@ int sp_hip_session_set_fast_path(sp_hip_session* s, int on)
 {
     if (!s) return SP_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] { s->s->set_fast_path(on != 0); });
+}
+
+int sp_hip_session_exclude(sp_hip_session* s, uint32_t orientation, const int64_t translation[3])
+{
+    if (!s || !translation) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->exclude(orientation, translation); });
+}
+
+int sp_hip_session_stamp(sp_hip_session* s, uint32_t orientation, const int64_t translation[3])
+{
+    if (!s || !translation) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->stamp_at(orientation, translation); });
 }
 
 int sp_hip_session_keep_fields(sp_hip_session* s, int keep)

@@ -1,0 +1,78 @@
+// The plain greedy spectral placement loop on one MI355X — the role
+// engine/pack_solver/src/gap_fill.cpp:241-588 (fill_gaps) plays, recomputing
+// every orientation's feasible mask from the current occupancy each step
+// instead of patching it with blocked-offset correlations.  Each proposal is
+// the lowest kernel-window corner in (z, y, x) order over all orientations,
+// the earliest orientation winning ties (gap_fill.cpp:461-498); the caller's
+// verdict decides whether it is stamped (accepted), blocked for that
+// orientation (rejected) or ends the run (stop), as at gap_fill.cpp:512-523.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <span>
+#include <string_view>
+#include <vector>
+
+#include "spectrapack/compute/correlation.hpp"
+#include "spectrapack/runtime/operation_control.hpp"
+
+struct sp_hip_session;
+
+namespace spectrapack::solver::gpu {
+
+enum class GapVerdict { accepted, rejected, stop };
+
+// One oriented kernel: {0,1} cells (X-fastest) of its trimmed window.
+struct OrientedKernel {
+    std::array<double, 4> rotation_xyzw {};
+    compute::Index3 first {};
+    compute::Shape3 shape {};
+    std::vector<std::uint8_t> cells;
+};
+
+struct Environment {
+    compute::Index3 first {};
+    compute::Shape3 shape {};
+    double pitch_mm {};
+    std::vector<std::uint8_t> occupied;  // X-fastest, 1 = blocked
+};
+
+struct Proposal {
+    std::uint32_t orientation {};
+    std::array<double, 4> rotation_xyzw {};
+    compute::Index3 translation {};           // kernel cell i sits at world cell translation + first + i
+    std::array<double, 3> translation_mm {};  // pitch * translation (gap_fill.cpp:507-509)
+};
+
+struct PlacementRequest {
+    std::uint64_t capacity {};
+    int precision {};  // 0 auto (a-priori bound), 32 or 64
+};
+
+struct PlacementStats {
+    std::uint64_t steps {}, proposals {}, accepted {};
+    int precision {};
+    double error_bound {}, max_residual {};
+    std::string_view outcome { "GAP_FILL_NOT_RUN" };
+};
+
+// Runs steps until capacity, exhaustion, a stop verdict or the control's
+// stop/deadline (polled between graph launches).  Outcome strings follow
+// gap_fill.cpp: GAP_FILL_CAPACITY, GAP_FILL_EXHAUSTED, GAP_FILL_STOPPED,
+// GAP_FILL_INTERRUPTED, GAP_FILL_CORRELATION (device or numeric failure).
+PlacementStats place_parts(const Environment& environment, std::span<const OrientedKernel> kernels,
+                           const PlacementRequest& request, const runtime::OperationControl& control,
+                           const std::function<GapVerdict(const Proposal&)>& accept,
+                           compute::FftExecution* execution = nullptr, std::vector<std::uint8_t>* final_occupancy = nullptr);
+
+// Rasterises a closed mesh (local frame) for every rotation on the device
+// (conservative_fields.cpp:919-1003).  Empty on failure.
+std::vector<OrientedKernel> voxelize_kernels(std::span<const std::array<double, 3>> vertices,
+                                             std::span<const std::array<std::uint32_t, 3>> triangles,
+                                             std::span<const std::array<double, 4>> rotations, double pitch_mm,
+                                             compute::FftExecution& execution);
+
+}  // namespace spectrapack::solver::gpu

@@ -297,6 +297,14 @@ class Session:
     def set_fast_path(self, on=True):
         self._check(lib().sp_hip_session_set_fast_path(self._h, C.c_int(1 if on else 0)))
 
+    def stamp(self, orientation, translation):
+        t = (C.c_int64 * 3)(*[int(v) for v in translation])
+        self._check(lib().sp_hip_session_stamp(self._h, C.c_uint32(orientation), t))
+
+    def exclude(self, orientation, translation):
+        t = (C.c_int64 * 3)(*[int(v) for v in translation])
+        self._check(lib().sp_hip_session_exclude(self._h, C.c_uint32(orientation), t))
+
     def keep_fields(self, keep=True):
         self._check(lib().sp_hip_session_keep_fields(self._h, C.c_int(1 if keep else 0)))
 

@@ -168,6 +168,11 @@ int sp_hip_session_set_kernel(sp_hip_session* s, uint32_t orientation, const uin
                               const uint32_t kernel_shape[3], const int64_t kernel_first[3]);
 /* One step; stamp != 0 ORs the chosen kernel into the occupancy on the device. */
 int sp_hip_session_step(sp_hip_session* s, int stamp, sp_step_result* out);
+/* gap_fill.cpp:512-523 verdicts outside the graph: stamp kernel `orientation` at
+ * `translation` (an accepted proposal), or keep that translation blocked for
+ * that orientation in every later step (a rejected proposal). */
+int sp_hip_session_stamp(sp_hip_session* s, uint32_t orientation, const int64_t translation[3]);
+int sp_hip_session_exclude(sp_hip_session* s, uint32_t orientation, const int64_t translation[3]);
 /* Enqueue `steps` steps without reading results back (for throughput timing);
  * sp_hip_session_step_result then reads the last one. */
 int sp_hip_session_run(sp_hip_session* s, uint32_t steps, int stamp);

@@ -224,14 +224,18 @@ static std::vector<KernelField> kernels_of(std::uint32_t n, const std::uint8_t* 
 // legal overlap counts, concatenated in orientation order.
 int orc_place_step(const std::uint8_t* env, const std::int64_t* ef, const std::uint32_t* es, std::uint32_t n,
                    const std::uint8_t* const* kcells, const std::uint32_t* ks, const std::int64_t* kf, int direct,
-                   int* orientation, std::int64_t* translation, std::int32_t* fields, char* code, int code_len)
+                   int* orientation, std::int64_t* translation, std::int32_t* fields, char* code, int code_len,
+                   const std::int64_t* excluded, std::uint32_t n_excluded)
 {
     const std::vector<std::uint8_t> e(env, env + std::size_t(es[0]) * es[1] * es[2]);
     const auto kernels = kernels_of(n, kcells, ks, kf);
     StepChoice c;
     std::vector<std::vector<std::int32_t>> f;
+    std::vector<std::pair<int, std::array<std::int64_t, 3>>> ex;
+    for (std::uint32_t i = 0; i != n_excluded; ++i)
+        ex.push_back({ int(excluded[4 * i]), { excluded[4 * i + 1], excluded[4 * i + 2], excluded[4 * i + 3] } });
     const auto err = place_step(e, { ef[0], ef[1], ef[2] }, { es[0], es[1], es[2] }, kernels, direct != 0, c,
-                                fields ? &f : nullptr);
+                                fields ? &f : nullptr, ex.empty() ? nullptr : &ex);
     put_code(err, code, code_len);
     if (!err.empty()) return 1;
     *orientation = c.orientation;

@@ -6,6 +6,7 @@
 // the reference; this is an independent mixed-radix (2,3,4,5 and generic odd
 // radices) Cooley-Tukey with Bluestein's algorithm for lengths with a prime
 // factor above 13, the same family of methods pocketfft publishes.
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <memory>
@@ -34,6 +35,7 @@ struct Plan {
     bool bluestein {};
     std::vector<std::size_t> factors;       // mixed-radix factors, product n
     std::vector<C> tw_fwd, tw_inv;         // per-stage twiddles
+    std::vector<std::vector<C>> roots_fwd, roots_inv;  // per-stage radix roots
     // Bluestein
     std::size_t m {};
     std::vector<C> chirp_fwd, chirp_inv;   // w_k = exp(-+ i pi k^2 / n)
@@ -91,7 +93,20 @@ std::shared_ptr<Plan> make_plan(std::size_t n)
     p->n = n;
     std::size_t largest;
     p->factors = factorize(n, largest);
-    if (largest > 13) {
+    // Bluestein only when it is cheaper than the generic-radix passes, with
+    // pocketfft's published cost model (n * sum of factors, odd factors above
+    // 5 weighted 1.1; Bluestein as 2 transforms of the next 2^a length times a
+    // 1.5 fudge factor).
+    auto cost = [](std::size_t len) {
+        std::size_t lg;
+        const auto fs = factorize(len, lg);
+        double c = 0;
+        for (std::size_t f : fs) c += f <= 5 ? double(f) : 1.1 * double(f);
+        return c * double(len);
+    };
+    std::size_t m2 = 1;
+    while (m2 < 2 * n - 1) m2 <<= 1;
+    if (largest > 5 && 1.5 * 2 * cost(m2) < cost(n)) {
         p->bluestein = true;
         std::size_t m = 1;
         while (m < 2 * n - 1) m <<= 1;
@@ -121,6 +136,13 @@ std::shared_ptr<Plan> make_plan(std::size_t n)
     // Stage twiddles for a decimation-in-time Stockham formulation.
     std::size_t l = 1;
     for (std::size_t r : p->factors) {
+        std::vector<C> rf(r), ri(r);
+        for (std::size_t q = 0; q != r; ++q) {
+            rf[q] = twiddle(q, r, true);
+            ri[q] = twiddle(q, r, false);
+        }
+        p->roots_fwd.push_back(std::move(rf));
+        p->roots_inv.push_back(std::move(ri));
         const std::size_t span = l * r;
         for (std::size_t j = 0; j != l; ++j)
             for (std::size_t q = 1; q != r; ++q) {
@@ -142,11 +164,11 @@ void run_mixed(const Plan& p, C* data, bool forward, std::vector<C>& scratch)
     std::size_t l = 1;
     std::size_t tw_off = 0;
     const auto& tw = forward ? p.tw_fwd : p.tw_inv;
-    std::vector<C> roots, tmp;
-    for (std::size_t r : p.factors) {
+    std::vector<C> tmp;
+    for (std::size_t stage = 0; stage != p.factors.size(); ++stage) {
+        const std::size_t r = p.factors[stage];
         const std::size_t m = n / (l * r);  // remaining
-        roots.resize(r);
-        for (std::size_t q = 0; q != r; ++q) roots[q] = twiddle(q, r, forward);
+        const auto& roots = forward ? p.roots_fwd[stage] : p.roots_inv[stage];
         tmp.resize(r);
         for (std::size_t j = 0; j != l; ++j) {
             for (std::size_t k = 0; k != m; ++k) {
@@ -157,22 +179,31 @@ void run_mixed(const Plan& p, C* data, bool forward, std::vector<C>& scratch)
                     tmp[q] = v;
                 }
                 // radix-r DFT
-                for (std::size_t s = 0; s != r; ++s) {
-                    C acc = tmp[0];
-                    if (r == 2) {
-                        acc = s == 0 ? tmp[0] + tmp[1] : tmp[0] - tmp[1];
+                if (r == 2) {
+                    dst[(k * r) * l + j] = tmp[0] + tmp[1];
+                    dst[(k * r + 1) * l + j] = tmp[0] - tmp[1];
+                }
+                else if (r == 4) {
+                    const C a = tmp[0] + tmp[2], b = tmp[0] - tmp[2];
+                    const C c = tmp[1] + tmp[3];
+                    C d = tmp[1] - tmp[3];
+                    d = forward ? C(d.imag(), -d.real()) : C(-d.imag(), d.real());
+                    dst[(k * r) * l + j] = a + c;
+                    dst[(k * r + 1) * l + j] = b + d;
+                    dst[(k * r + 2) * l + j] = a - c;
+                    dst[(k * r + 3) * l + j] = b - d;
+                }
+                else {
+                    for (std::size_t s = 0; s != r; ++s) {
+                        C acc = tmp[0];
+                        std::size_t idx = s;
+                        for (std::size_t q = 1; q != r; ++q) {
+                            acc += tmp[q] * roots[idx];
+                            idx += s;
+                            if (idx >= r) idx -= r;
+                        }
+                        dst[(k * r + s) * l + j] = acc;
                     }
-                    else if (r == 4) {
-                        const C a = tmp[0] + tmp[2], b = tmp[0] - tmp[2];
-                        const C c = tmp[1] + tmp[3];
-                        C d = tmp[1] - tmp[3];
-                        d = forward ? C(d.imag(), -d.real()) : C(-d.imag(), d.real());
-                        acc = s == 0 ? a + c : s == 1 ? b + d : s == 2 ? a - c : b - d;
-                    }
-                    else {
-                        for (std::size_t q = 1; q != r; ++q) acc += tmp[q] * roots[(q * s) % r];
-                    }
-                    dst[(k * r + s) * l + j] = acc;
                 }
             }
         }
@@ -215,13 +246,31 @@ void fft_axis(std::complex<double>* data, const std::array<std::uint32_t, 3>& sh
     if (axis == 0) { outer_a = shape[1]; sa = s1; outer_b = shape[2]; sb = s2; }
     else if (axis == 1) { outer_a = shape[0]; sa = s0; outer_b = shape[2]; sb = s2; }
     else { outer_a = shape[0]; sa = s0; outer_b = shape[1]; sb = s1; }
-    std::vector<C> line(n), scratch;
+    std::vector<C> scratch;
+    if (axis == 0) {
+        std::vector<C> line(n);
+        for (std::size_t b = 0; b != outer_b; ++b)
+            for (std::size_t a = 0; a != outer_a; ++a) {
+                C* base = data + a * sa + b * sb;
+                for (std::size_t i = 0; i != n; ++i) line[i] = base[i];
+                run(*plan, line.data(), forward, scratch);
+                for (std::size_t i = 0; i != n; ++i) base[i] = factor == 1.0 ? line[i] : line[i] * factor;
+            }
+        return;
+    }
+    // strided axes: gather a block of adjacent lines (contiguous along x) at once
+    constexpr std::size_t kBlock = 16;
+    std::vector<C> lines(kBlock * n);
     for (std::size_t b = 0; b != outer_b; ++b)
-        for (std::size_t a = 0; a != outer_a; ++a) {
-            C* base = data + a * sa + b * sb;
-            for (std::size_t i = 0; i != n; ++i) line[i] = base[i * stride];
-            run(*plan, line.data(), forward, scratch);
-            for (std::size_t i = 0; i != n; ++i) base[i * stride] = factor == 1.0 ? line[i] : line[i] * factor;
+        for (std::size_t a0 = 0; a0 < outer_a; a0 += kBlock) {
+            const std::size_t cnt = std::min(kBlock, outer_a - a0);
+            C* base = data + a0 * sa + b * sb;
+            for (std::size_t i = 0; i != n; ++i)
+                for (std::size_t l = 0; l != cnt; ++l) lines[l * n + i] = base[l * sa + i * stride];
+            for (std::size_t l = 0; l != cnt; ++l) run(*plan, lines.data() + l * n, forward, scratch);
+            for (std::size_t i = 0; i != n; ++i)
+                for (std::size_t l = 0; l != cnt; ++l)
+                    base[l * sa + i * stride] = factor == 1.0 ? lines[l * n + i] : lines[l * n + i] * factor;
         }
 }
 

@@ -233,7 +233,7 @@ def _kernel_arrays(kernels):
     return cells, ptrs, ks, kf
 
 
-def place_step(env, env_first, kernels, direct=False, fields=False):
+def place_step(env, env_first, kernels, direct=False, fields=False, excluded=()):
     """One plain step (gap_fill.cpp:363-509 rule).  env is [z,y,x] uint8.
 
     kernels: list of dicts with first (x,y,z), shape (x,y,z), cells [z,y,x].
@@ -251,9 +251,11 @@ def place_step(env, env_first, kernels, direct=False, fields=False):
     o = C.c_int(-1)
     t = np.zeros(3, np.int64)
     code = C.create_string_buffer(64)
+    ex = np.ascontiguousarray(np.asarray([[o_, *t_] for o_, t_ in excluded] or [[0, 0, 0, 0]], dtype=np.int64))
     rc = lib().orc_place_step(_p(env, C.c_uint8), _p(ef, C.c_int64), _p(es, C.c_uint32), C.c_uint32(len(kernels)),
                               ptrs, _p(ks, C.c_uint32), _p(kf, C.c_int64), C.c_int(1 if direct else 0), C.byref(o),
-                              _p(t, C.c_int64), _p(out, C.c_int32) if fields else None, code, 64)
+                              _p(t, C.c_int64), _p(out, C.c_int32) if fields else None, code, 64, _p(ex, C.c_int64),
+                              C.c_uint32(len(excluded)))
     if rc:
         raise Failure(code.value.decode(), {})
     flds = None

@@ -177,7 +177,8 @@ bool free_mask(const std::array<std::int64_t, 3>& env_first, const std::array<st
 // direct=true the integer oracle), then the lowest (z,y,x) rule.
 std::string place_step(const std::vector<std::uint8_t>& env, const std::array<std::int64_t, 3>& env_first,
                        const std::array<std::uint32_t, 3>& env_shape, const std::vector<KernelField>& kernels,
-                       bool direct, StepChoice& choice, std::vector<std::vector<std::int32_t>>* legal_fields = nullptr);
+                       bool direct, StepChoice& choice, std::vector<std::vector<std::int32_t>>* legal_fields = nullptr,
+                       const std::vector<std::pair<int, std::array<std::int64_t, 3>>>* excluded = nullptr);
 // gap_fill.cpp:98-121 for a kernel at translation t.
 void stamp(std::vector<std::uint8_t>& env, const std::array<std::int64_t, 3>& env_first,
            const std::array<std::uint32_t, 3>& env_shape, const KernelField& k, const std::array<std::int64_t, 3>& t);

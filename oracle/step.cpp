@@ -164,7 +164,8 @@ bool free_mask(const std::array<std::int64_t, 3>& ef, const std::array<std::uint
 // strict comparison keeps the earliest rotation on ties.
 std::string place_step(const std::vector<std::uint8_t>& env, const std::array<std::int64_t, 3>& ef,
                        const std::array<std::uint32_t, 3>& es, const std::vector<KernelField>& kernels, bool direct,
-                       StepChoice& choice, std::vector<std::vector<std::int32_t>>* legal_fields)
+                       StepChoice& choice, std::vector<std::vector<std::int32_t>>* legal_fields,
+                       const std::vector<std::pair<int, std::array<std::int64_t, 3>>>* excluded)
 {
     choice = StepChoice {};
     if (legal_fields) legal_fields->assign(kernels.size(), {});
@@ -224,6 +225,14 @@ std::string place_step(const std::vector<std::uint8_t>& env, const std::array<st
                             c.values[off[0] + std::size_t(c.shape[0]) * (off[1] + std::size_t(c.shape[1]) * off[2])]);
                     }
         }
+        // gap_fill.cpp:517: a rejected proposal's translation is no longer free
+        if (excluded)
+            for (const auto& [eo, et] : *excluded) {
+                if (eo != int(o)) continue;
+                const std::int64_t u[3] = { et[0] - low[0], et[1] - low[1], et[2] - low[2] };
+                if (u[0] < 0 || u[1] < 0 || u[2] < 0 || u[0] >= legal[0] || u[1] >= legal[1] || u[2] >= legal[2]) continue;
+                mask[u[0] + std::size_t(legal[0]) * (u[1] + std::size_t(legal[1]) * u[2])] = 0;
+            }
         std::size_t cursor = 0;
         for (; cursor != mask.size(); ++cursor)
             if (mask[cursor]) break;
