@@ -62,6 +62,28 @@ def kernel_bytes(name, n, kernels, ext, elem):
     return 0
 
 
+def fp_deviation(sp, ctx, w, env, kernels, bound):
+    """One step from the container occupancy in FP32 and in FP64: the largest
+    |FP32 - FP64| over every legal overlap value of every orientation."""
+    import numpy as np
+    raws = {}
+    for prec in (32, 64):
+        s = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=prec, capture_graph=False)
+        s.set_environment(env)
+        for i, k in enumerate(kernels):
+            s.set_kernel(i, k["cells"], k["shape"], k["first"])
+        s.keep_raw(True)
+        s.step(stamp=False)
+        raws[prec] = [s.raw_values(i) for i in range(len(kernels))]
+        s.close()
+    dev = 0.0
+    for a, b in zip(raws[32], raws[64]):
+        if a.size:
+            dev = max(dev, float(np.max(np.abs(a - b))))
+    return {"max_abs_dev": dev, "fp32_bound": bound, "orientations": len(kernels),
+            "reference": "the device FP64 path on the same occupancy (itself bit-exact to the FP64 oracle on integers)"}
+
+
 def cpu_baseline(workload, env, kernels, threads, sample_orientations):
     """The oracle's FP64 full-linear restatement of the same correlations."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -88,6 +110,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--no-deviation", action="store_true", help="skip the FP32-vs-FP64 deviation step")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -204,6 +227,10 @@ def main():
         "parts_placed": int(placed + args.steps if last["orientation"] >= 0 else placed),
         "max_residual_last_step": last["max_residual"],
     }
+    if not args.no_deviation:
+        log("[bench] FP32 vs FP64 deviation ...")
+        out["fp32_vs_fp64"] = fp_deviation(sp, ctx, w, env, kernels, r["error_bound"])
+        log(f"[bench] max |fp32 - fp64| = {out['fp32_vs_fp64']['max_abs_dev']:.3g} (bound {r['error_bound']:.3g})")
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline sample ...")
         out["cpu_baseline"] = cpu_baseline(w, env, kernels, args.cpu_threads, max(1, args.cpu_threads))

@@ -271,7 +271,8 @@ __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ 
                                                       StepDeviceResult* __restrict__ result, int* __restrict__ fields,
                                                       const long long* __restrict__ field_offsets,
                                                       const unsigned long long* __restrict__ excl,
-                                                      const cx<T>* __restrict__ tw, FftPlan plan, int W)
+                                                      double* __restrict__ raw, const cx<T>* __restrict__ tw,
+                                                      FftPlan plan, int W)
 {
     using T2 = cx<T>;
     extern __shared__ unsigned char smem_raw[];
@@ -318,6 +319,7 @@ __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ 
         if (!(res < T(0.25))) my_flags |= 1u;
         if (rv < T(0) || rv > max_out) my_flags |= 2u;
         if (fo) fo[x + (long long)d.lx * (y + (long long)d.ly * z)] = int(rv);
+        if (raw) raw[field_offsets[o] + x + (long long)d.lx * (y + (long long)d.ly * z)] = double(v);
         if (rv == T(0)) {
             const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                            ((unsigned long long)x << 16) | (unsigned long long)o;

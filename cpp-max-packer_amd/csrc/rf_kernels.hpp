@@ -218,6 +218,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
                                                                       int* __restrict__ fields,
                                                                       const long long* __restrict__ field_offsets,
                                                                       const unsigned long long* __restrict__ excl,
+                                                                      double* __restrict__ raw,
                                                                       const cx<T>* __restrict__ tw)
 {
     using T2 = cx<T>;
@@ -300,6 +301,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
             if (!(res < T(0.25))) my_flags |= 1u;
             if (rv < T(0) || rv > max_out) my_flags |= 2u;
             if (fo) fo[x + (long long)d.lx * (y + (long long)d.ly * z)] = int(rv);
+            if (raw) raw[field_offsets[o] + x + (long long)d.lx * (y + (long long)d.ly * z)] = double(val);
             if (rv == T(0)) {
                 const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                                ((unsigned long long)x << 16) | (unsigned long long)o;

@@ -161,19 +161,19 @@ void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batche
 template <typename T>
 void rf_c2r_launch(int n, hipStream_t st, int lx_max, int ly_max, int orients, const cx<T>* in, long long in_o, int nx,
                    int ny, const OrientDesc* descs, T inv_total, StepDeviceResult* res, int* fields,
-                   const long long* offsets, const unsigned long long* excl, const cx<T>* tw)
+                   const long long* offsets, const unsigned long long* excl, double* raw, const cx<T>* tw)
 {
     if (n == 256) {
         constexpr int N = 256, R = 16, W = 64;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
         hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
-                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, tw);
+                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, raw, tw);
     }
     else {
         constexpr int N = 512, R = 8, W = 16;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
         hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
-                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, tw);
+                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, raw, tw);
     }
 }
 
@@ -330,6 +330,30 @@ public:
         }
     }
 
+    // Unrounded legal values (as double) of the next steps, for measuring the
+    // FP32 deviation from FP64 on the same occupancy.
+    void keep_raw(bool keep)
+    {
+        if (keep != keep_raw_) {
+            keep_raw_ = keep;
+            prepared_ = false;
+        }
+    }
+
+    void raw_values(std::uint32_t o, double* out, std::uint64_t count)
+    {
+        if (!keep_raw_ || !prepared_) throw std::invalid_argument("raw values not kept");
+        if (int(o) >= O_) throw std::invalid_argument("orientation index");
+        const OrientDesc& d = descs_[o];
+        const std::uint64_t n = std::uint64_t(std::max(d.lx, 0)) * std::max(d.ly, 0) * std::max(d.lz, 0);
+        if (count < n) throw std::length_error("capacity");
+        if (n == 0) return;
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(out, raw_.as<double>() + field_offsets_[o], n * sizeof(double), hipMemcpyDeviceToHost,
+                                    stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
     // Precision chosen for the session: FP32 unless its a-priori bound, with
     // the environment mass at its worst case (every cell), exceeds 0.4.
     int precision() const { return precision_; }
@@ -394,6 +418,8 @@ public:
                                     hipMemcpyHostToDevice, stream_));
         if (keep_fields_) fields_.allocate(std::max<long long>(foff, 1) * sizeof(int));
         else fields_.release();
+        if (keep_raw_) raw_.allocate(std::max<long long>(foff, 1) * sizeof(double));
+        else raw_.release();
         fields_total_ = foff;
         const std::size_t elem = precision_ == 64 ? sizeof(double2) : sizeof(float2);
         const std::size_t nh = std::size_t(n_[2] / 2 + 1);
@@ -546,14 +572,15 @@ public:
             if (rf_supported(nz) && use_rf_) {
                 rf_c2r_launch<T>(nz, stream_, lmax_[0], lmax_[1], O_, TT, to, nx, ny, dd, inv_total, res,
                                  keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
-                                 excl_.as<unsigned long long>(), twz);
+                                 excl_.as<unsigned long long>(), keep_raw_ ? raw_.as<double>() : nullptr, twz);
             }
             else {
                 const int W3 = choose_w(nz, e, 2, true, 32, 2);
                 hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], O_), dim3(kThreads),
                                    lds_bytes<T>(nz, 2, W3 / 2), stream_, TT, to, nx, ny, dd, inv_total, res,
                                    keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
-                                   excl_.as<unsigned long long>(), twz, plans_[2], W3);
+                                   excl_.as<unsigned long long>(), keep_raw_ ? raw_.as<double>() : nullptr, twz,
+                                   plans_[2], W3);
             }
             mark("c2r_z_argmin");
         }
@@ -700,7 +727,8 @@ private:
     int nfit_ {};
     StepDeviceResult* host_result_ {};
     long long fields_total_ {};
-    bool keep_fields_ {}, prepared_ {};
+    bool keep_fields_ {}, keep_raw_ {}, prepared_ {};
+    DeviceBuffer raw_;
     int precision_ { 32 };
     double bound_ {};
     hipGraph_t graph_ {};
@@ -819,6 +847,18 @@ int sp_hip_session_stamp(sp_hip_session* s, uint32_t orientation, const int64_t 
 {
     if (!s || !translation) return SP_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] { s->s->stamp_at(orientation, translation); });
+}
+
+int sp_hip_session_keep_raw(sp_hip_session* s, int keep)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->keep_raw(keep != 0); });
+}
+
+int sp_hip_session_raw_values(sp_hip_session* s, uint32_t orientation, double* out, uint64_t count)
+{
+    if (!s || !out) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->raw_values(orientation, out, count); });
 }
 
 int sp_hip_session_keep_fields(sp_hip_session* s, int keep)

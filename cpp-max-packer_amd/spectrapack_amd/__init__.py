@@ -305,6 +305,17 @@ class Session:
         t = (C.c_int64 * 3)(*[int(v) for v in translation])
         self._check(lib().sp_hip_session_exclude(self._h, C.c_uint32(orientation), t))
 
+    def keep_raw(self, keep=True):
+        self._check(lib().sp_hip_session_keep_raw(self._h, C.c_int(1 if keep else 0)))
+
+    def raw_values(self, o):
+        ex, ey, ez = self.env_shape
+        kx, ky, kz = self.kernel_shapes[o]
+        n = max((ex - kx + 1) * (ey - ky + 1) * (ez - kz + 1), 1)
+        a = np.zeros(n, np.float64)
+        self._check(lib().sp_hip_session_raw_values(self._h, C.c_uint32(o), _p(a, C.c_double), C.c_uint64(a.size)))
+        return a
+
     def keep_fields(self, keep=True):
         self._check(lib().sp_hip_session_keep_fields(self._h, C.c_int(1 if keep else 0)))
 

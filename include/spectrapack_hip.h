@@ -190,6 +190,10 @@ int sp_hip_session_profile_step(sp_hip_session* s, int stamp, float* kernel_ms, 
 /* 1 (default): register-resident FFT passes for 256- and 512-point axes;
  * 0: the generic LDS Stockham passes for every axis (same results). */
 int sp_hip_session_set_fast_path(sp_hip_session* s, int on);
+/* Unrounded legal values (double) of the last step, for measuring the FP32
+ * deviation from FP64 on the same occupancy; requires keep_raw. */
+int sp_hip_session_keep_raw(sp_hip_session* s, int keep);
+int sp_hip_session_raw_values(sp_hip_session* s, uint32_t orientation, double* out, uint64_t count);
 /* Stream the session launches on (hipStream_t), for events around its work. */
 void* sp_hip_session_stream(sp_hip_session* s);
 /* Device bytes held by the session. */
