@@ -275,34 +275,34 @@ int run(sphip::Context& C, const sp_correlation_spec& s, const Geometry& g, cons
             hipLaunchKernelGGL((k_r2c_z<T, E>), dim3((nx + W - 1) / W, ny, 1), dim3(kThreads), lds_bytes<T>(nz, 2, W / 2),
                                st_, denv, 0LL, int(s.environment_shape[0]), int(s.environment_shape[1]),
                                int(s.environment_shape[0]), int(s.environment_shape[1]), int(s.environment_shape[2]), S,
-                               0LL, nx, ny, nx, ny, nullptr, twz, pz, W);
+                               0LL, nx, ny, nx, ny, nullptr, nullptr, twz, pz, W);
         else
             hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((nx + W - 1) / W, ny, 1), dim3(kThreads),
                                lds_bytes<T>(nz, 2, W / 2), st_, dker, 0LL, int(s.kernel_shape[0]), int(s.kernel_shape[1]),
                                int(s.kernel_shape[0]), int(s.kernel_shape[1]), int(s.kernel_shape[2]), S, 0LL, nx, ny, nx,
-                               ny, nullptr, twz, pz, W);
+                               ny, nullptr, nullptr, twz, pz, W);
         const int Wy = choose_w(ny, e, 2, false, 16, 1);
         hipLaunchKernelGGL((k_c2c<T>), dim3((nx + Wy - 1) / Wy, nh, 1), dim3(kThreads), lds_bytes<T>(ny, 2, Wy), st_, S, 0LL,
                            (long long)nx, 1LL, (long long)nx * ny, S, 0LL, (long long)nx, 1LL, (long long)nx * ny, nx, ny,
-                           ny, nullptr, 0, twy, py, Wy, 1);
+                           ny, nullptr, 0, nullptr, twy, py, Wy, 1);
         if (which == 0) {
             const int Wx = choose_w(nx, e, 2, false, 16, 1);
             hipLaunchKernelGGL((k_c2c<T>), dim3((ny + Wx - 1) / Wx, nh, 1), dim3(kThreads), lds_bytes<T>(nx, 2, Wx), st_, S,
                                0LL, 1LL, (long long)nx, (long long)nx * ny, S, 0LL, 1LL, (long long)nx,
-                               (long long)nx * ny, ny, nx, nx, nullptr, 0, twx, px, Wx, 1);
+                               (long long)nx * ny, ny, nx, nx, nullptr, 0, nullptr, twx, px, Wx, 1);
         }
     }
     // kernel x transform + product with conj + inverse x, into SK
     {
         const int W = choose_w(nx, e, 4, false, 16, 1);
         hipLaunchKernelGGL((k_mul_x<T>), dim3((ny + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(nx, 4, W), st_, SK, 0LL,
-                           nx, ny, SB, nx, ny, SK, 0LL, nullptr, 0, nx, twx, px, W);
+                           nx, ny, SB, nx, ny, SK, 0LL, nullptr, 0, nx, nullptr, twx, px, W);
     }
     {
         const int W = choose_w(ny, e, 2, false, 16, 1);
         hipLaunchKernelGGL((k_c2c<T>), dim3((nx + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(ny, 2, W), st_, SK, 0LL,
                            (long long)nx, 1LL, (long long)nx * ny, SK, 0LL, (long long)nx, 1LL, (long long)nx * ny, nx,
-                           ny, ny, nullptr, 0, twy, py, W, 0);
+                           ny, ny, nullptr, 0, nullptr, twy, py, W, 0);
         const int W3 = choose_w(nz, e, 2, true, 32, 2);
         const T scale = T(1.0 / (double(nx) * double(ny) * double(nz)));
         hipLaunchKernelGGL((k_c2r_z_store<T>), dim3((nx + W3 - 1) / W3, ny, 1), dim3(kThreads), lds_bytes<T>(nz, 2, W3 / 2),

@@ -29,6 +29,35 @@ struct StepDeviceResult {
     unsigned long long pad;
 };
 
+// Active orientations of a step: act[0] = count, act[1] = regions on,
+// act[2..4] = regions per axis, act[5..] = orientation per grid z slot.
+// Null: every slot is its own orientation.
+constexpr int kActHeader = 5;
+__device__ __forceinline__ int slot_orientation(const int* act)
+{
+    if (!act) return int(blockIdx.z);
+    if (int(blockIdx.z) >= act[0]) return -1;
+    return act[kActHeader + blockIdx.z];
+}
+
+// What the argmin pass keeps: the active list, an optional mask over
+// regions of the environment (a kernel corner u lies in region
+// floor(u * regions / extent) per axis) and optional per-orientation minima.
+struct StepFilter {
+    const int* act;
+    const unsigned char* region_mask;
+    int ex, ey, ez;
+    unsigned long long* orient_best;
+};
+
+__device__ __forceinline__ bool region_allowed(const StepFilter& f, int ux, int uy, int uz)
+{
+    if (!f.act || !f.act[1]) return true;
+    const int* a = f.act;
+    const int rx = ux * a[2] / f.ex, ry = uy * a[3] / f.ey, rz = uz * a[4] / f.ez;
+    return f.region_mask[rx + a[2] * (ry + a[3] * rz)] != 0;
+}
+
 // Rejected translations (gap_fill.cpp:517-520): excl[0] = count, then sorted
 // packed keys.  A rejected key is never chosen again.
 __device__ __forceinline__ bool excluded(const unsigned long long* excl, unsigned long long key)
@@ -82,7 +111,7 @@ template <typename T, typename In>
 __global__ void __launch_bounds__(256) k_r2c_z(const In* __restrict__ in, long long in_bs, int in_px, int in_py,
                                                int ex, int ey, int ez, cx<T>* __restrict__ out, long long out_bs,
                                                int out_px, int out_py, int cx_, int cy_, const OrientDesc* descs,
-                                               const cx<T>* __restrict__ tw, FftPlan plan, int W)
+                                               const int* act, const cx<T>* __restrict__ tw, FftPlan plan, int W)
 {
     using T2 = cx<T>;
     extern __shared__ unsigned char smem_raw[];
@@ -92,7 +121,8 @@ __global__ void __launch_bounds__(256) k_r2c_z(const In* __restrict__ in, long l
     T2* tws = reinterpret_cast<T2*>(smem_raw);
     T2* A = tws + nz;
     T2* B = A + nz * sw;
-    const int b = blockIdx.z;
+    const int b = descs ? slot_orientation(act) : int(blockIdx.z);
+    if (b < 0) return;
     int ex_ = ex, ey_ = ey, ez_ = ez, px = in_px, py = in_py, cxx = cx_, cyy = cy_;
     const In* src = in + b * in_bs;
     if (descs) {
@@ -145,8 +175,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_c2c(const cx<T>* in, long long in_o, long long in_sn, long long in_sw,
                                              long long in_sb, cx<T>* out, long long out_o, long long out_sn,
                                              long long out_sw, long long out_sb, int count_w, int n_in, int n_out,
-                                             const OrientDesc* descs, int mode, const cx<T>* __restrict__ tw,
-                                             FftPlan plan, int W, int fwd)
+                                             const OrientDesc* descs, int mode, const int* act,
+                                             const cx<T>* __restrict__ tw, FftPlan plan, int W, int fwd)
 {
     using T2 = cx<T>;
     extern __shared__ unsigned char smem_raw[];
@@ -155,7 +185,8 @@ __global__ void __launch_bounds__(256) k_c2c(const cx<T>* in, long long in_o, lo
     T2* tws = reinterpret_cast<T2*>(smem_raw);
     T2* A = tws + n;
     T2* B = A + n * sw;
-    const int o = blockIdx.z;
+    const int o = descs ? slot_orientation(act) : int(blockIdx.z);
+    if (o < 0) return;
     int cw = count_w, ni = n_in, no = n_out;
     if (descs) {
         const OrientDesc d = descs[o];
@@ -214,7 +245,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, int h_py,
                                                const cx<T>* __restrict__ SB, int nx, int ny, cx<T>* __restrict__ out,
                                                long long out_o, const OrientDesc* descs, int k_spectral,
-                                               int lx_all, const cx<T>* __restrict__ tw, FftPlan plan, int W)
+                                               int lx_all, const int* act, const cx<T>* __restrict__ tw, FftPlan plan,
+                                               int W)
 {
     using T2 = cx<T>;
     extern __shared__ unsigned char smem_raw[];
@@ -225,7 +257,8 @@ __global__ void __launch_bounds__(256) k_mul_x(const cx<T>* __restrict__ H, long
     T2* K1 = K0 + n * sw;
     T2* B0 = K1 + n * sw;
     T2* B1 = B0 + n * sw;
-    const int o = blockIdx.z;
+    const int o = descs ? slot_orientation(act) : int(blockIdx.z);
+    if (o < 0) return;
     const int kz = blockIdx.y;
     const int y0 = blockIdx.x * W;
     int kx = nx, lx = lx_all;
@@ -271,8 +304,8 @@ __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ 
                                                       StepDeviceResult* __restrict__ result, int* __restrict__ fields,
                                                       const long long* __restrict__ field_offsets,
                                                       const unsigned long long* __restrict__ excl,
-                                                      double* __restrict__ raw, const cx<T>* __restrict__ tw,
-                                                      FftPlan plan, int W)
+                                                      double* __restrict__ raw, StepFilter filt,
+                                                      const cx<T>* __restrict__ tw, FftPlan plan, int W)
 {
     using T2 = cx<T>;
     extern __shared__ unsigned char smem_raw[];
@@ -284,7 +317,8 @@ __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ 
     T2* B = A + nz * sw;
     __shared__ unsigned long long best;
     __shared__ unsigned int res_bits, flags;
-    const int o = blockIdx.z;
+    const int o = slot_orientation(filt.act);
+    if (o < 0) return;
     const OrientDesc d = descs[o];
     const int y = blockIdx.y;
     const int x0 = blockIdx.x * W;
@@ -323,7 +357,8 @@ __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ 
         if (rv == T(0)) {
             const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                            ((unsigned long long)x << 16) | (unsigned long long)o;
-            if (!excluded(excl, key)) mine = key < mine ? key : mine;
+            if (!excluded(excl, key) && region_allowed(filt, x, y, z))
+                mine = key < mine ? key : mine;
         }
     }
     if (mine != ~0ULL) atomicMin(&best, mine);
@@ -331,7 +366,10 @@ __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ 
     if (my_flags) atomicOr(&flags, my_flags);
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (best != ~0ULL) atomicMin(&result->key, best);
+        if (best != ~0ULL) {
+            atomicMin(&result->key, best);
+            if (filt.orient_best) atomicMin(&filt.orient_best[o], best);
+        }
         if (res_bits) atomicMax(&result->max_residual_bits, res_bits);
         if (flags) atomicOr(&result->flags, flags);
     }

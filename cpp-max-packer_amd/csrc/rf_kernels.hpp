@@ -19,7 +19,7 @@ template <typename T, int N, int R, int W>
 __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long in_o, long long in_sn, long long in_sw,
                                                       long long in_sb, cx<T>* out, long long out_o, long long out_sn,
                                                       long long out_sw, long long out_sb, int count_w, int n_in,
-                                                      int n_out, const OrientDesc* descs, int mode,
+                                                      int n_out, const OrientDesc* descs, int mode, const int* act,
                                                       const cx<T>* __restrict__ tw, int fwd)
 {
     using T2 = cx<T>;
@@ -28,7 +28,8 @@ __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long 
     extern __shared__ unsigned char smem_raw[];
     T2* tws = reinterpret_cast<T2*>(smem_raw);
     T2* lds = tws + N;
-    const int o = blockIdx.z;
+    const int o = descs ? slot_orientation(act) : int(blockIdx.z);
+    if (o < 0) return;
     int cw = count_w, ni = n_in, no = n_out;
     if (descs) {
         const OrientDesc d = descs[o];
@@ -102,7 +103,7 @@ template <typename T, int N, int R, int W>
 __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px,
                                                        const cx<T>* __restrict__ SB, int ny, cx<T>* __restrict__ out,
                                                        long long out_o, const OrientDesc* __restrict__ descs,
-                                                       const int* __restrict__ fit, int nfit, const cx<T>* __restrict__ tw)
+                                                       const int* __restrict__ act, const cx<T>* __restrict__ tw)
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
@@ -120,9 +121,11 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
 #pragma unroll
     for (int s = 0; s < R; ++s) sb[s] = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
     T2* pl = lds + w * S::stride;
-    for (int i = 0; i < nfit; ++i) {
-        const int o = fit[i];
+    const int count = act[0];
+    for (int i = 0; i < count; ++i) {
+        const int o = act[kActHeader + i];
         const int kx = descs[o].kx, lx = descs[o].lx;
+        if (lx <= 0) continue;  // uniform across the block
         const T2* hrow = H + o * h_o + row * h_px;
         T2 v[R];
 #pragma unroll
@@ -151,7 +154,8 @@ template <typename T, int N, int R, int W, typename In>
 __global__ void __launch_bounds__((W / 2) * (N / R)) rf_r2c_z(const In* __restrict__ in, int in_px, int in_py, int ex,
                                                                int ey, int ez, cx<T>* __restrict__ out, long long out_bs,
                                                                int out_px, int out_py, int cx_, int cy_,
-                                                               const OrientDesc* descs, const cx<T>* __restrict__ tw)
+                                                               const OrientDesc* descs, const int* act,
+                                                               const cx<T>* __restrict__ tw)
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
@@ -160,7 +164,8 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_r2c_z(const In* __restri
     extern __shared__ unsigned char smem_raw[];
     T2* tws = reinterpret_cast<T2*>(smem_raw);
     T2* lds = tws + N;
-    const int b = blockIdx.z;
+    const int b = descs ? slot_orientation(act) : int(blockIdx.z);
+    if (b < 0) return;
     int ex_ = ex, ey_ = ey, ez_ = ez, px = in_px, py = in_py, cxx = cx_, cyy = cy_;
     const In* src = in;
     if (descs) {
@@ -218,7 +223,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
                                                                       int* __restrict__ fields,
                                                                       const long long* __restrict__ field_offsets,
                                                                       const unsigned long long* __restrict__ excl,
-                                                                      double* __restrict__ raw,
+                                                                      double* __restrict__ raw, StepFilter filt,
                                                                       const cx<T>* __restrict__ tw)
 {
     using T2 = cx<T>;
@@ -230,7 +235,8 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
     T2* lds = tws + N;
     __shared__ unsigned long long best;
     __shared__ unsigned int res_bits, flags;
-    const int o = blockIdx.z;
+    const int o = slot_orientation(filt.act);
+    if (o < 0) return;
     const OrientDesc d = descs[o];
     const int y = blockIdx.y;
     const int x0 = blockIdx.x * W;
@@ -305,7 +311,8 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
             if (rv == T(0)) {
                 const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                                ((unsigned long long)x << 16) | (unsigned long long)o;
-                if (!excluded(excl, key)) mine = key < mine ? key : mine;
+                if (!excluded(excl, key) && region_allowed(filt, x, y, z))
+                    mine = key < mine ? key : mine;
             }
         }
     }
@@ -323,7 +330,10 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (best != ~0ULL) atomicMin(&result->key, best);
+        if (best != ~0ULL) {
+            atomicMin(&result->key, best);
+            if (filt.orient_best) atomicMin(&filt.orient_best[o], best);
+        }
         if (res_bits) atomicMax(&result->max_residual_bits, res_bits);
         if (flags) atomicOr(&result->flags, flags);
     }

@@ -7,11 +7,13 @@
 // a hipGraph and replayed per step.
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <numeric>
 
 #include "common.hpp"
 #include "fft_kernels.hpp"
 #include "rf_kernels.hpp"
+#include "session.hpp"
 
 namespace sphip {
 
@@ -38,13 +40,15 @@ template <typename T> std::size_t lds_bytes(int n, int buffers, int pencils)
     return sizeof(cx<T>) * (std::size_t(n) + std::size_t(buffers) * n * (pencils + 1));
 }
 
-__global__ void k_reset_result(StepDeviceResult* r)
+__global__ void k_reset_result(StepDeviceResult* r, unsigned long long* orient_best, int orientations)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         r->key = ~0ULL;
         r->flags = 0;
         r->max_residual_bits = 0;
     }
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < orientations; o += gridDim.x * blockDim.x)
+        orient_best[o] = ~0ULL;
 }
 
 __global__ void k_set_key(StepDeviceResult* r, unsigned long long key)
@@ -91,19 +95,19 @@ template <typename T>
 void rf_cols_launch(int n, hipStream_t st, int count_w_max, int batches, int orients, const cx<T>* in, long long in_o,
                     long long in_sn, long long in_sw, long long in_sb, cx<T>* out, long long out_o, long long out_sn,
                     long long out_sw, long long out_sb, int count_w, int n_in, int n_out, const OrientDesc* descs,
-                    int mode, const cx<T>* tw, int fwd)
+                    int mode, const int* act, const cx<T>* tw, int fwd)
 {
     if (n == 256) {
         constexpr int N = 256, R = 16, W = 32;
         hipLaunchKernelGGL((rf_cols<T, N, R, W>), dim3((count_w_max + W - 1) / W, batches, orients), dim3(W * (N / R)),
                            (rf_lds<T, N, R>(W)), st, in, in_o, in_sn, in_sw, in_sb, out, out_o, out_sn, out_sw, out_sb,
-                           count_w, n_in, n_out, descs, mode, tw, fwd);
+                           count_w, n_in, n_out, descs, mode, act, tw, fwd);
     }
     else {
         constexpr int N = 512, R = 8, W = 16;
         hipLaunchKernelGGL((rf_cols<T, N, R, W>), dim3((count_w_max + W - 1) / W, batches, orients), dim3(W * (N / R)),
                            (rf_lds<T, N, R>(W)), st, in, in_o, in_sn, in_sw, in_sb, out, out_o, out_sn, out_sw, out_sb,
-                           count_w, n_in, n_out, descs, mode, tw, fwd);
+                           count_w, n_in, n_out, descs, mode, act, tw, fwd);
     }
 }
 
@@ -125,55 +129,56 @@ void rf_rows_launch(int n, hipStream_t st, cx<T>* data, int rows, long long row_
 
 template <typename T>
 void rf_mul_launch(int n, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px, const cx<T>* SB,
-                   cx<T>* out, long long out_o, const OrientDesc* descs, const int* fit, int count, const cx<T>* tw)
+                   cx<T>* out, long long out_o, const OrientDesc* descs, const int* act, const cx<T>* tw)
 {
     if (n == 256) {
         constexpr int N = 256, R = 16, W = 16;
         hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st,
-                           H, h_o, h_px, SB, ny, out, out_o, descs, fit, count, tw);
+                           H, h_o, h_px, SB, ny, out, out_o, descs, act, tw);
     }
     else {
         constexpr int N = 512, R = 8, W = 4;
         hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st,
-                           H, h_o, h_px, SB, ny, out, out_o, descs, fit, count, tw);
+                           H, h_o, h_px, SB, ny, out, out_o, descs, act, tw);
     }
 }
 
 template <typename T, typename In>
 void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batches, const In* in, int in_px, int in_py,
                    int ex, int ey, int ez, cx<T>* out, long long out_bs, int out_px, int out_py, int cx_, int cy_,
-                   const OrientDesc* descs, const cx<T>* tw)
+                   const OrientDesc* descs, const int* act, const cx<T>* tw)
 {
     if (n == 256) {
         constexpr int N = 256, R = 16, W = 32;
         hipLaunchKernelGGL((rf_r2c_z<T, N, R, W, In>), dim3((cols_max + W - 1) / W, rows_max, batches),
                            dim3((W / 2) * (N / R)), (rf_lds<T, N, R>(W / 2)), st, in, in_px, in_py, ex, ey, ez, out,
-                           out_bs, out_px, out_py, cx_, cy_, descs, tw);
+                           out_bs, out_px, out_py, cx_, cy_, descs, act, tw);
     }
     else {
         constexpr int N = 512, R = 8, W = 16;
         hipLaunchKernelGGL((rf_r2c_z<T, N, R, W, In>), dim3((cols_max + W - 1) / W, rows_max, batches),
                            dim3((W / 2) * (N / R)), (rf_lds<T, N, R>(W / 2)), st, in, in_px, in_py, ex, ey, ez, out,
-                           out_bs, out_px, out_py, cx_, cy_, descs, tw);
+                           out_bs, out_px, out_py, cx_, cy_, descs, act, tw);
     }
 }
 
 template <typename T>
 void rf_c2r_launch(int n, hipStream_t st, int lx_max, int ly_max, int orients, const cx<T>* in, long long in_o, int nx,
                    int ny, const OrientDesc* descs, T inv_total, StepDeviceResult* res, int* fields,
-                   const long long* offsets, const unsigned long long* excl, double* raw, const cx<T>* tw)
+                   const long long* offsets, const unsigned long long* excl, double* raw, StepFilter filt,
+                   const cx<T>* tw)
 {
     if (n == 256) {
         constexpr int N = 256, R = 16, W = 64;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
         hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
-                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, raw, tw);
+                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, raw, filt, tw);
     }
     else {
         constexpr int N = 512, R = 8, W = 16;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
         hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
-                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, raw, tw);
+                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, raw, filt, tw);
     }
 }
 
@@ -208,8 +213,7 @@ public:
     }
     ~Session()
     {
-        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-        if (graph_) (void)hipGraphDestroy(graph_);
+        drop_graphs();
         if (host_result_) (void)hipHostFree(host_result_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
@@ -265,15 +269,70 @@ public:
     {
         if (on != use_rf_) {
             use_rf_ = on;
-            if (graph_exec_) {
-                (void)hipGraphExecDestroy(graph_exec_);
-                graph_exec_ = nullptr;
-            }
-            if (graph_) {
-                (void)hipGraphDestroy(graph_);
-                graph_ = nullptr;
+            drop_graphs();
+        }
+    }
+
+    // Restricts the next steps to a list of orientations and, optionally, to
+    // a mask over regions of the environment.  No list and no mask: every
+    // orientation everywhere (the plain search).  Orientations whose kernel
+    // does not fit are dropped from the list.
+    void set_active(const std::uint32_t* orients, std::uint32_t count, const std::uint8_t* mask,
+                    const std::uint32_t regions[3])
+    {
+        prepare();
+        if (count == 0 && !mask) {
+            subset_ = false;
+            return;
+        }
+        std::vector<int> list;
+        std::vector<char> seen(O_, 0);
+        if (count) {
+            for (std::uint32_t i = 0; i != count; ++i) {
+                if (orients[i] >= std::uint32_t(O_)) throw std::invalid_argument("orientation index");
+                if (seen[orients[i]]) throw std::invalid_argument("orientation listed twice");
+                seen[orients[i]] = 1;
+                if (descs_[orients[i]].lx > 0) list.push_back(int(orients[i]));
             }
         }
+        else
+            for (int o = 0; o != O_; ++o)
+                if (descs_[o].lx > 0) list.push_back(o);
+        int r[3] = { 1, 1, 1 };
+        if (mask) {
+            for (int a = 0; a != 3; ++a) {
+                if (!regions || regions[a] == 0 || regions[a] > std::uint32_t(kMaxRegionsPerAxis))
+                    throw std::invalid_argument("regions per axis");
+                r[a] = int(regions[a]);
+            }
+        }
+        std::vector<int> head(kActHeader + list.size());
+        head[0] = int(list.size());
+        head[1] = mask ? 1 : 0;
+        head[2] = r[0];
+        head[3] = r[1];
+        head[4] = r[2];
+        std::copy(list.begin(), list.end(), head.begin() + kActHeader);
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(act_sub_.get(), head.data(), head.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+        if (mask)
+            SP_HIP_CHECK(hipMemcpyAsync(region_.get(), mask, std::size_t(r[0]) * r[1] * r[2], hipMemcpyHostToDevice,
+                                        stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        subset_ = true;
+        sub_slots_ = 8;
+        while (sub_slots_ < int(list.size())) sub_slots_ *= 2;
+        sub_slots_ = std::max(1, std::min(sub_slots_, std::max(nfit_, 1)));
+    }
+
+    // Lowest feasible key of each orientation in the last step (~0: none).
+    void orientation_minima(std::uint64_t* out, std::uint32_t count)
+    {
+        if (!prepared_) throw std::invalid_argument("no step run");
+        if (count < std::uint32_t(O_)) throw std::length_error("capacity");
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(out, orient_best_.get(), sizeof(std::uint64_t) * O_, hipMemcpyDeviceToHost, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
     // gap_fill.cpp:517-520: a rejected proposal's translation stays blocked
@@ -406,13 +465,20 @@ public:
         SP_HIP_CHECK(hipMemcpyAsync(kcells_.get(), all.data(), all.size(), hipMemcpyHostToDevice, stream_));
         d_descs_.allocate(sizeof(OrientDesc) * O_);
         SP_HIP_CHECK(hipMemcpyAsync(d_descs_.get(), descs_.data(), sizeof(OrientDesc) * O_, hipMemcpyHostToDevice, stream_));
-        std::vector<int> fit;
+        // active lists: [count, regions on, regions x, y, z, orientation per slot]
+        std::vector<int> fit { 0, 0, 1, 1, 1 };
         for (int o = 0; o != O_; ++o)
             if (descs_[o].lx > 0) fit.push_back(o);
-        nfit_ = int(fit.size());
-        d_fit_.allocate(sizeof(int) * std::max<std::size_t>(fit.size(), 1));
-        if (!fit.empty())
-            SP_HIP_CHECK(hipMemcpyAsync(d_fit_.get(), fit.data(), sizeof(int) * fit.size(), hipMemcpyHostToDevice, stream_));
+        nfit_ = int(fit.size()) - kActHeader;
+        fit[0] = nfit_;
+        act_full_.allocate(sizeof(int) * fit.size());
+        SP_HIP_CHECK(hipMemcpyAsync(act_full_.get(), fit.data(), sizeof(int) * fit.size(), hipMemcpyHostToDevice, stream_));
+        act_sub_.allocate(sizeof(int) * (kActHeader + O_));
+        SP_HIP_CHECK(hipMemcpyAsync(act_sub_.get(), fit.data(), sizeof(int) * fit.size(), hipMemcpyHostToDevice, stream_));
+        subset_ = false;
+        sub_slots_ = std::max(nfit_, 1);
+        if (!region_.get()) region_.allocate(std::size_t(kMaxRegionsPerAxis) * kMaxRegionsPerAxis * kMaxRegionsPerAxis);
+        orient_best_.allocate(sizeof(unsigned long long) * O_);
         d_offsets_.allocate(sizeof(long long) * O_);
         SP_HIP_CHECK(hipMemcpyAsync(d_offsets_.get(), field_offsets_.data(), sizeof(long long) * O_,
                                     hipMemcpyHostToDevice, stream_));
@@ -434,14 +500,7 @@ public:
             if (precision_ == 64) (void)twiddles<double>(n_[a]);
             else (void)twiddles<float>(n_[a]);
         }
-        if (graph_exec_) {
-            (void)hipGraphExecDestroy(graph_exec_);
-            graph_exec_ = nullptr;
-        }
-        if (graph_) {
-            (void)hipGraphDestroy(graph_);
-            graph_ = nullptr;
-        }
+        drop_graphs();
         prepared_ = true;
     }
 
@@ -457,7 +516,7 @@ public:
         prof_names_.push_back(name);
     }
 
-    template <typename T> void enqueue_step(bool stamp)
+    template <typename T> void enqueue_step(bool stamp, const int* act, int slots)
     {
         using T2 = cx<T>;
         const int nx = n_[0], ny = n_[1], nz = n_[2];
@@ -473,32 +532,32 @@ public:
         StepDeviceResult* res = result_.as<StepDeviceResult>();
         const std::size_t e = sizeof(T2);
         mark("start");
-        hipLaunchKernelGGL(k_reset_result, dim3(1), dim3(64), 0, stream_, res);
+        hipLaunchKernelGGL(k_reset_result, dim3(1), dim3(256), 0, stream_, res, orient_best_.as<unsigned long long>(), O_);
         mark("reset");
         // E1: environment R2C along z (uint8 -> spectrum planes)
         if (rf_supported(nz) && use_rf_) {
             rf_r2c_launch<T, unsigned char>(nz, stream_, nx, ny, 1, env_.as<unsigned char>(), ext_[0], ext_[1], ext_[0],
-                                            ext_[1], ext_[2], SB, 0LL, nx, ny, nx, ny, nullptr, twz);
+                                            ext_[1], ext_[2], SB, 0LL, nx, ny, nx, ny, nullptr, nullptr, twz);
             mark("env_r2c_z");
         }
         else {
             const int W = choose_w(nz, e, 2, true, 32, 2);
             hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((nx + W - 1) / W, ny, 1), dim3(kThreads),
                                lds_bytes<T>(nz, 2, W / 2), stream_, env_.as<unsigned char>(), 0LL, ext_[0], ext_[1],
-                               ext_[0], ext_[1], ext_[2], SB, 0LL, nx, ny, nx, ny, nullptr, twz, plans_[2], W);
+                               ext_[0], ext_[1], ext_[2], SB, 0LL, nx, ny, nx, ny, nullptr, nullptr, twz, plans_[2], W);
             mark("env_r2c_z");
         }
         // E2: C2C along y (columns x, batch kz)
         if (rf_supported(ny) && use_rf_) {
             rf_cols_launch<T>(ny, stream_, nx, nh, 1, SB, 0LL, (long long)nx, 1LL, (long long)nx * ny, SB, 0LL,
-                              (long long)nx, 1LL, (long long)nx * ny, nx, ny, ny, nullptr, 0, twy, 1);
+                              (long long)nx, 1LL, (long long)nx * ny, nx, ny, ny, nullptr, 0, nullptr, twy, 1);
             mark("env_c2c_y");
         }
         else {
             const int W = choose_w(ny, e, 2, false, 16, 1);
             hipLaunchKernelGGL((k_c2c<T>), dim3((nx + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(ny, 2, W),
                                stream_, SB, 0LL, (long long)nx, 1LL, (long long)nx * ny, SB, 0LL, (long long)nx, 1LL,
-                               (long long)nx * ny, nx, ny, ny, nullptr, 0, twy, plans_[1], W, 1);
+                               (long long)nx * ny, nx, ny, ny, nullptr, 0, nullptr, twy, plans_[1], W, 1);
             mark("env_c2c_y");
         }
         // E3: C2C along x (rows, batch kz)
@@ -510,76 +569,79 @@ public:
             const int W = choose_w(nx, e, 2, false, 16, 1);
             hipLaunchKernelGGL((k_c2c<T>), dim3((ny + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(nx, 2, W),
                                stream_, SB, 0LL, 1LL, (long long)nx, (long long)nx * ny, SB, 0LL, 1LL, (long long)nx,
-                               (long long)nx * ny, ny, nx, nx, nullptr, 0, twx, plans_[0], W, 1);
+                               (long long)nx * ny, ny, nx, nx, nullptr, 0, nullptr, twx, plans_[0], W, 1);
             mark("env_c2c_x");
         }
         const long long h1o = (long long)nh * kmax_[1] * kmax_[0];
         const long long h2o = (long long)nh * ny * kmax_[0];
         const long long to = (long long)nh * ny * nx;
-        // G1: kernel R2C along z over its own columns (x < kx, y < ky)
-        if (rf_supported(nz) && use_rf_) {
-            rf_r2c_launch<T, unsigned char>(nz, stream_, kmax_[0], kmax_[1], O_, kcells_.as<unsigned char>(), 0, 0, 0, 0, 0,
-                                            H1, h1o, kmax_[0], kmax_[1], 0, 0, dd, twz);
+        if (slots > 0) {
+            // G1: kernel R2C along z over its own columns (x < kx, y < ky)
+            if (rf_supported(nz) && use_rf_) {
+                rf_r2c_launch<T, unsigned char>(nz, stream_, kmax_[0], kmax_[1], slots, kcells_.as<unsigned char>(), 0, 0, 0,
+                                                0, 0, H1, h1o, kmax_[0], kmax_[1], 0, 0, dd, act, twz);
+            }
+            else {
+                const int W = choose_w(nz, e, 2, true, 32, 2);
+                hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((kmax_[0] + W - 1) / W, kmax_[1], slots),
+                                   dim3(kThreads), lds_bytes<T>(nz, 2, W / 2), stream_, kcells_.as<unsigned char>(), 0LL, 0,
+                                   0, 0, 0, 0, H1, h1o, kmax_[0], kmax_[1], 0, 0, dd, act, twz, plans_[2], W);
+            }
             mark("kernel_r2c_z");
-        }
-        else {
-            const int W = choose_w(nz, e, 2, true, 32, 2);
-            hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((kmax_[0] + W - 1) / W, kmax_[1], O_), dim3(kThreads),
-                               lds_bytes<T>(nz, 2, W / 2), stream_, kcells_.as<unsigned char>(), 0LL, 0, 0, 0, 0, 0, H1,
-                               h1o, kmax_[0], kmax_[1], 0, 0, dd, twz, plans_[2], W);
-            mark("kernel_r2c_z");
-        }
-        // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
-        if (rf_supported(ny) && use_rf_) {
-            rf_cols_launch<T>(ny, stream_, kmax_[0], nh, O_, H1, h1o, (long long)kmax_[0], 1LL, (long long)kmax_[1] * kmax_[0],
-                              H2, h2o, (long long)kmax_[0], 1LL, (long long)ny * kmax_[0], 0, 0, ny, dd, 1, twy, 1);
-            mark("kernel_c2c_y");
-        }
-        else {
-            const int W = choose_w(ny, e, 2, false, 16, 1);
-            hipLaunchKernelGGL((k_c2c<T>), dim3((kmax_[0] + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(ny, 2, W),
-                               stream_, H1, h1o, (long long)kmax_[0], 1LL, (long long)kmax_[1] * kmax_[0], H2, h2o,
-                               (long long)kmax_[0], 1LL, (long long)ny * kmax_[0], 0, 0, ny, dd, 1, twy, plans_[1], W, 1);
-            mark("kernel_c2c_y");
-        }
-        // P1: finish kernel x transform, product with conj, inverse x
-        if (rf_supported(nx) && use_rf_) {
-            rf_mul_launch<T>(nx, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, d_fit_.as<int>(), nfit_, twx);
-            mark("mul_x");
-        }
-        else {
-            const int W = choose_w(nx, e, 4, false, 16, 1);
-            hipLaunchKernelGGL((k_mul_x<T>), dim3((ny + W - 1) / W, nh, O_), dim3(kThreads), lds_bytes<T>(nx, 4, W),
-                               stream_, H2, h2o, kmax_[0], ny, SB, nx, ny, TT, to, dd, 0, 0, twx, plans_[0], W);
-            mark("mul_x");
-        }
-        // P2: inverse C2C along y for the legal columns, rows y < ly written
-        if (lmax_[0] > 0) {
-            // P2: inverse C2C along y for the legal columns, rows y < ly written
+            // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
             if (rf_supported(ny) && use_rf_) {
-                rf_cols_launch<T>(ny, stream_, lmax_[0], nh, O_, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
-                                  (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, twy, 0);
+                rf_cols_launch<T>(ny, stream_, kmax_[0], nh, slots, H1, h1o, (long long)kmax_[0], 1LL,
+                                  (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)kmax_[0], 1LL,
+                                  (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, 1);
             }
             else {
                 const int W = choose_w(ny, e, 2, false, 16, 1);
-                hipLaunchKernelGGL((k_c2c<T>), dim3((lmax_[0] + W - 1) / W, nh, O_), dim3(kThreads),
+                hipLaunchKernelGGL((k_c2c<T>), dim3((kmax_[0] + W - 1) / W, nh, slots), dim3(kThreads),
+                                   lds_bytes<T>(ny, 2, W), stream_, H1, h1o, (long long)kmax_[0], 1LL,
+                                   (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)kmax_[0], 1LL,
+                                   (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, plans_[1], W, 1);
+            }
+            mark("kernel_c2c_y");
+            // P1: finish kernel x transform, product with conj, inverse x
+            if (rf_supported(nx) && use_rf_) {
+                rf_mul_launch<T>(nx, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twx);
+            }
+            else {
+                const int W = choose_w(nx, e, 4, false, 16, 1);
+                hipLaunchKernelGGL((k_mul_x<T>), dim3((ny + W - 1) / W, nh, slots), dim3(kThreads),
+                                   lds_bytes<T>(nx, 4, W), stream_, H2, h2o, kmax_[0], ny, SB, nx, ny, TT, to, dd, 0, 0,
+                                   act, twx, plans_[0], W);
+            }
+            mark("mul_x");
+        }
+        if (slots > 0 && lmax_[0] > 0) {
+            // P2: inverse C2C along y for the legal columns, rows y < ly written
+            if (rf_supported(ny) && use_rf_) {
+                rf_cols_launch<T>(ny, stream_, lmax_[0], nh, slots, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
+                                  (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, act, twy, 0);
+            }
+            else {
+                const int W = choose_w(ny, e, 2, false, 16, 1);
+                hipLaunchKernelGGL((k_c2c<T>), dim3((lmax_[0] + W - 1) / W, nh, slots), dim3(kThreads),
                                    lds_bytes<T>(ny, 2, W), stream_, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT,
-                                   to, (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, twy, plans_[1], W, 0);
+                                   to, (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, act, twy, plans_[1], W, 0);
             }
             mark("inv_c2c_y");
             // P3: C2R along z + round + checks + feasible + lowest (z, y, x)
             const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
+            const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
+                                    orient_best_.as<unsigned long long>() };
             if (rf_supported(nz) && use_rf_) {
-                rf_c2r_launch<T>(nz, stream_, lmax_[0], lmax_[1], O_, TT, to, nx, ny, dd, inv_total, res,
+                rf_c2r_launch<T>(nz, stream_, lmax_[0], lmax_[1], slots, TT, to, nx, ny, dd, inv_total, res,
                                  keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
-                                 excl_.as<unsigned long long>(), keep_raw_ ? raw_.as<double>() : nullptr, twz);
+                                 excl_.as<unsigned long long>(), keep_raw_ ? raw_.as<double>() : nullptr, filt, twz);
             }
             else {
                 const int W3 = choose_w(nz, e, 2, true, 32, 2);
-                hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], O_), dim3(kThreads),
+                hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], slots), dim3(kThreads),
                                    lds_bytes<T>(nz, 2, W3 / 2), stream_, TT, to, nx, ny, dd, inv_total, res,
                                    keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
-                                   excl_.as<unsigned long long>(), keep_raw_ ? raw_.as<double>() : nullptr, twz,
+                                   excl_.as<unsigned long long>(), keep_raw_ ? raw_.as<double>() : nullptr, filt, twz,
                                    plans_[2], W3);
             }
             mark("c2r_z_argmin");
@@ -591,37 +653,100 @@ public:
         }
     }
 
-    void enqueue(bool stamp)
+    void enqueue(bool stamp, bool subset)
     {
-        if (precision_ == 64) enqueue_step<double>(stamp);
-        else enqueue_step<float>(stamp);
+        const int* act = subset ? act_sub_.as<int>() : act_full_.as<int>();
+        const int slots = subset ? sub_slots_ : nfit_;
+        if (precision_ == 64) enqueue_step<double>(stamp, act, slots);
+        else enqueue_step<float>(stamp, act, slots);
     }
 
-    void launch(bool stamp)
+    void drop_graphs()
+    {
+        for (auto& g : graphs_) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            if (g.graph) (void)hipGraphDestroy(g.graph);
+        }
+        graphs_.clear();
+    }
+
+    // Replays the graph recorded for (what, stamp, owner, slots), recording it
+    // on first use.  Device pointers inside stay valid until prepare() runs
+    // again, which drops every graph.
+    void run_graph(int what, bool stamp, const void* owner, int slots, const std::function<void()>& record)
+    {
+        for (auto& g : graphs_)
+            if (g.what == what && g.stamp == stamp && g.owner == owner && g.slots == slots) {
+                SP_HIP_CHECK(hipGraphLaunch(g.exec, stream_));
+                return;
+            }
+        if (graphs_.size() >= 8) drop_graphs();
+        CachedGraph g { what, stamp, owner, slots, nullptr, nullptr };
+        SP_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            record();
+        }
+        catch (...) {
+            hipGraph_t broken = nullptr;
+            (void)hipStreamEndCapture(stream_, &broken);
+            if (broken) (void)hipGraphDestroy(broken);
+            throw;
+        }
+        SP_HIP_CHECK(hipStreamEndCapture(stream_, &g.graph));
+        SP_HIP_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        graphs_.push_back(g);
+        SP_HIP_CHECK(hipGraphLaunch(g.exec, stream_));
+    }
+
+    void launch(bool stamp) { launch_mode(stamp, subset_); }
+
+    void launch_mode(bool stamp, bool subset)
     {
         prepare();
         use_device(ctx_);
         if (!desc_.capture_graph) {
-            enqueue(stamp);
+            enqueue(stamp, subset);
             SP_HIP_CHECK(hipGetLastError());
             return;
         }
-        if (!graph_exec_ || graph_stamp_ != stamp) {
-            if (graph_exec_) {
-                (void)hipGraphExecDestroy(graph_exec_);
-                graph_exec_ = nullptr;
-            }
-            if (graph_) {
-                (void)hipGraphDestroy(graph_);
-                graph_ = nullptr;
-            }
-            SP_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            enqueue(stamp);
-            SP_HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
-            SP_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-            graph_stamp_ = stamp;
+        run_graph(subset ? 1 : 0, stamp, nullptr, subset ? sub_slots_ : nfit_, [&] { enqueue(stamp, subset); });
+    }
+
+    // A proposal step: `propose` enqueues work on the session stream that
+    // writes the subset list (at most `slots` orientations) and the region
+    // mask on the device, then the subset search runs; both in one graph.
+    void launch_proposed(const void* owner, const std::function<void(hipStream_t)>& propose, int slots, bool stamp)
+    {
+        prepare();
+        use_device(ctx_);
+        if (slots <= 0 || slots > O_) throw std::invalid_argument("proposal slots");
+        auto record = [&] {
+            propose(stream_);
+            if (precision_ == 64) enqueue_step<double>(stamp, act_sub_.as<int>(), slots);
+            else enqueue_step<float>(stamp, act_sub_.as<int>(), slots);
+        };
+        if (!desc_.capture_graph) {
+            record();
+            SP_HIP_CHECK(hipGetLastError());
+            return;
         }
-        SP_HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+        run_graph(2, stamp, owner, slots, record);
+    }
+
+    SessionView view()
+    {
+        prepare();
+        SessionView v {};
+        v.env = env_.as<unsigned char>();
+        for (int a = 0; a != 3; ++a) v.ext[a] = ext_[a];
+        v.orientations = O_;
+        v.fitting = nfit_;
+        v.act = act_sub_.as<int>();
+        v.region = region_.as<unsigned char>();
+        v.max_regions_per_axis = kMaxRegionsPerAxis;
+        v.descs = d_descs_.as<OrientDesc>();
+        v.stream = stream_;
+        return v;
     }
 
     // One step without the graph, timing each kernel between events.
@@ -633,7 +758,7 @@ public:
         prof_events_.clear();
         prof_names_.clear();
         try {
-            enqueue(stamp);
+            enqueue(stamp, subset_);
         }
         catch (...) {
             prof_ = false;
@@ -722,18 +847,27 @@ private:
     std::array<int, 3> kmax_ {}, lmax_ {};
     FftPlan plans_[3] {};
     hipStream_t stream_ {};
-    DeviceBuffer env_, result_, kcells_, d_descs_, d_offsets_, d_fit_, fields_, sb_, h1_, h2_, t_, excl_;
+    DeviceBuffer env_, result_, kcells_, d_descs_, d_offsets_, fields_, sb_, h1_, h2_, t_, excl_;
+    DeviceBuffer act_full_, act_sub_, region_, orient_best_;
     std::vector<unsigned long long> excluded_;
     int nfit_ {};
+    bool subset_ {};
+    int sub_slots_ { 1 };
     StepDeviceResult* host_result_ {};
     long long fields_total_ {};
     bool keep_fields_ {}, keep_raw_ {}, prepared_ {};
     DeviceBuffer raw_;
     int precision_ { 32 };
     double bound_ {};
-    hipGraph_t graph_ {};
-    hipGraphExec_t graph_exec_ {};
-    bool graph_stamp_ {};
+    struct CachedGraph {
+        int what;
+        bool stamp;
+        const void* owner;
+        int slots;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<CachedGraph> graphs_;
     bool prof_ {};
     bool use_rf_ { true };
     std::vector<hipEvent_t> prof_events_;
@@ -748,6 +882,20 @@ struct sp_hip_session {
 };
 
 using sphip::DeviceError;
+
+namespace sphip {
+SessionView session_view(sp_hip_session* s) { return s->s->view(); }
+
+void session_launch_proposed(sp_hip_session* s, const void* owner, const std::function<void(hipStream_t)>& propose,
+                             int slots, bool stamp)
+{
+    s->s->launch_proposed(owner, propose, slots, stamp);
+}
+
+void session_launch_full(sp_hip_session* s, bool stamp) { s->s->launch_mode(stamp, false); }
+
+void session_read_result(sp_hip_session* s, sp_step_result& out) { s->s->read_result(out); }
+}  // namespace sphip
 
 namespace {
 int fail(sp_hip_ctx* ctx, int status, const std::string& msg)
@@ -835,6 +983,19 @@ int sp_hip_session_set_fast_path(sp_hip_session* s, int on)
 {
     if (!s) return SP_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] { s->s->set_fast_path(on != 0); });
+}
+
+int sp_hip_session_set_active(sp_hip_session* s, const uint32_t* orientations, uint32_t count,
+                              const uint8_t* region_mask, const uint32_t regions[3])
+{
+    if (!s || (count && !orientations)) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_active(orientations, count, region_mask, regions); });
+}
+
+int sp_hip_session_orientation_minima(sp_hip_session* s, uint64_t* keys, uint32_t count)
+{
+    if (!s || !keys) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->orientation_minima(keys, count); });
 }
 
 int sp_hip_session_exclude(sp_hip_session* s, uint32_t orientation, const int64_t translation[3])

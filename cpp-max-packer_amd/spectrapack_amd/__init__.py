@@ -308,6 +308,31 @@ class Session:
     def keep_raw(self, keep=True):
         self._check(lib().sp_hip_session_keep_raw(self._h, C.c_int(1 if keep else 0)))
 
+    def set_active(self, orientations=(), region_mask=None):
+        """Restrict the next steps to `orientations` (empty: all) and, with
+        region_mask (uint8 [rz, ry, rx]), to kernel corners in nonzero regions.
+        No arguments: the plain search again."""
+        ol = np.ascontiguousarray(np.asarray(orientations, dtype=np.uint32).ravel())
+        if region_mask is None:
+            m, regions = None, None
+        else:
+            rm = np.ascontiguousarray(np.asarray(region_mask, dtype=np.uint8))
+            if rm.ndim != 3:
+                raise ValueError("region_mask must be [rz, ry, rx]")
+            m = _p(rm, C.c_uint8)
+            regions = (C.c_uint32 * 3)(rm.shape[2], rm.shape[1], rm.shape[0])
+        self._check(lib().sp_hip_session_set_active(self._h, _p(ol, C.c_uint32) if ol.size else None,
+                                                    C.c_uint32(ol.size), m, regions))
+
+    def orientation_minima(self):
+        """Per orientation: None, or the lowest feasible (uz, uy, ux) corner of the last step."""
+        a = np.zeros(self.orientations, np.uint64)
+        self._check(lib().sp_hip_session_orientation_minima(self._h, _p(a, C.c_uint64), C.c_uint32(a.size)))
+        out = []
+        for v in a.tolist():
+            out.append(None if v == 0xFFFFFFFFFFFFFFFF else ((v >> 48) & 0xFFFF, (v >> 32) & 0xFFFF, (v >> 16) & 0xFFFF))
+        return out
+
     def raw_values(self, o):
         ex, ey, ez = self.env_shape
         kx, ky, kz = self.kernel_shapes[o]

@@ -194,6 +194,20 @@ int sp_hip_session_set_fast_path(sp_hip_session* s, int on);
  * deviation from FP64 on the same occupancy; requires keep_raw. */
 int sp_hip_session_keep_raw(sp_hip_session* s, int keep);
 int sp_hip_session_raw_values(sp_hip_session* s, uint32_t orientation, double* out, uint64_t count);
+/* Search restriction (no reference counterpart; used by the learned
+ * proposal).  The next steps consider only the listed orientations (count 0:
+ * every orientation) and, when region_mask is not null, only kernel-window
+ * corners u (relative to environment_first) whose region
+ * (u * regions / environment_shape per axis, X-fastest, regions[a] in 1..64)
+ * has a nonzero mask byte.  count 0 and a null mask restore the plain search.
+ * Orientations whose kernel does not fit are dropped from the list. */
+int sp_hip_session_set_active(sp_hip_session* s, const uint32_t* orientations, uint32_t count,
+                              const uint8_t* region_mask, const uint32_t regions[3]);
+/* Lowest feasible key of every orientation in the last step, packed as
+ * (uz << 48) | (uy << 32) | (ux << 16) | orientation with u the kernel-window
+ * corner relative to environment_first; all ones when the orientation had no
+ * feasible translation (or was not searched).  count >= orientations. */
+int sp_hip_session_orientation_minima(sp_hip_session* s, uint64_t* keys, uint32_t count);
 /* Stream the session launches on (hipStream_t), for events around its work. */
 void* sp_hip_session_stream(sp_hip_session* s);
 /* Device bytes held by the session. */
@@ -223,9 +237,65 @@ int sp_hip_voxelize_mesh_container(sp_hip_ctx* ctx, const double* vertices, uint
                                    const uint32_t shape[3], const double origin[3], double pitch, double clearance,
                                    uint8_t* cells);
 
+/* ---- learned proposal (no reference counterpart) ---------------------------
+ * A BF16 3D CNN (gfx950 MFMA, FP32 accumulate) ranks the orientations of a
+ * step and the 16^3 regions of the environment; the exact search then runs on
+ * the top orientations in the top regions only, and the full search is the
+ * fallback when that finds nothing.  Inputs: the occupancy downsampled to 64^3
+ * (uint8 fraction q = round(255 occupied / cells) per bin, bin b of an axis of
+ * extent e covering cells [b e / 64, (b + 1) e / 64)) and a 32-float part
+ * descriptor.  Outputs: 96 orientation logits and 4096 region logits
+ * (X-fastest 16^3).  Parameters are one flat FP32 array in this order:
+ *   w1[32][27] b1[32]  w2[32][27][32] b2[32]  w3[32][27][32] b3[32]
+ *   w4[32][27][32] b4[32]  wf[32][32]  wh[32] bh  wa[64][64] ba[64]
+ *   wo[96][64] bo[96]
+ * (conv weights [out][tap][in], tap = 9 dz + 3 dy + dx, source voxel
+ * stride * v + d - 1). */
+typedef struct sp_hip_proposal sp_hip_proposal;
+
+uint32_t sp_hip_proposal_parameter_count(void);
+/* max_batch: largest training or forward batch (1..1024). */
+int sp_hip_proposal_create(sp_hip_ctx* ctx, uint32_t max_batch, sp_hip_proposal** out);
+void sp_hip_proposal_destroy(sp_hip_proposal* p);
+/* Setting parameters also resets the Adam state. */
+int sp_hip_proposal_set_parameters(sp_hip_proposal* p, const float* params, uint64_t count);
+int sp_hip_proposal_get_parameters(sp_hip_proposal* p, float* params, uint64_t count);
+/* Device bank of 64^3 uint8 samples for training: reserve slots, fill them
+ * from the host or from a session's current occupancy, read one back. */
+int sp_hip_proposal_reserve(sp_hip_proposal* p, uint32_t samples);
+int sp_hip_proposal_load_samples(sp_hip_proposal* p, const uint8_t* samples, uint32_t first, uint32_t count);
+int sp_hip_proposal_record(sp_hip_proposal* p, sp_hip_session* s, uint32_t slot);
+int sp_hip_proposal_read_sample(sp_hip_proposal* p, uint32_t slot, uint8_t* sample);
+/* Forward over bank samples: logits [batch][96] and heat [batch][4096]
+ * (either may be null). */
+int sp_hip_proposal_forward(sp_hip_proposal* p, const uint32_t* slots, uint32_t batch, const float* descriptors,
+                            float* logits, float* heat);
+/* Loss and gradients over bank samples.  Orientation loss: cross entropy of
+ * the softmax over orientations with a nonzero mask byte against
+ * orientation_target [batch][96]; region loss: cross entropy of the softmax
+ * over the 4096 heat logits against region[b] (none when < 0), times
+ * heat_weight; both averaged over the batch.  losses receives the two means;
+ * gradients (nullable) the flat FP32 gradient; update != 0 applies one Adam
+ * step (beta 0.9 / 0.999, eps 1e-8) with learning_rate. */
+int sp_hip_proposal_train(sp_hip_proposal* p, const uint32_t* slots, uint32_t batch, const float* descriptors,
+                          const float* orientation_target, const uint8_t* orientation_mask, const int32_t* region,
+                          float heat_weight, float learning_rate, int update, float losses[2], float* gradients);
+/* One learned placement step on a session with at most 96 orientations and
+ * at least 64 cells per axis: the top_orientations fitting orientations by
+ * logit (ties: lower index) searched exactly in the top_regions regions by
+ * heat logit, all in one graph.  When nothing there is feasible and fallback
+ * != 0, the full search runs (*used_fallback = 1).  out->orientation is -1
+ * when no step was found. */
+int sp_hip_proposal_step(sp_hip_proposal* p, sp_hip_session* s, const float descriptor[32], uint32_t top_orientations,
+                         uint32_t top_regions, int stamp, int fallback, sp_step_result* out, int* used_fallback);
+/* Self-test of the MFMA tile layout: c[32][32] = a[32][16] * b[16][32], BF16
+ * bit patterns in, FP32 out. */
+int sp_hip_proposal_mfma_probe(sp_hip_proposal* p, const uint16_t* a, const uint16_t* b, float* c);
+
 /* A-priori FP32 rounding bound for a correlation of an environment of mass
- * env_mass with a kernel of mass kernel_mass on an FFT grid of fft_cells cells
- * (see DESIGN: probabilistic rounding model).  FP64 is chosen when > 0.4. */
+ * env_mass with a kernel of mass kernel_mass on an FFT grid of fft_cells cells:
+ * lambda * 2u * sqrt(3 log2 N) * rms, with rms^2 <= min(mB, mK) mB mK / N and
+ * lambda = sqrt(2 (ln 2N + 40 ln 2)).  FP64 is chosen when > 0.4. */
 double sp_hip_fp32_error_bound(uint64_t fft_cells, uint64_t env_mass, uint64_t kernel_mass);
 
 #ifdef __cplusplus

@@ -58,3 +58,36 @@ def direct_expected(oracle, spec, env_zyx, ker_zyx):
     # the oracle's translations are world translations (correlation_tests.cpp:62-66)
     first = (rng[0], rng[2], rng[4])
     return np.array(first), vals
+
+
+def restricted_choice(fields, env_shape, orientations=(), region_mask=None):
+    """Expected choice of a step restricted to `orientations` (empty: all) and
+    to kernel corners u whose region floor(u * regions / env_shape) has a
+    nonzero byte in region_mask [rz, ry, rx].  fields: the oracle's legal
+    overlap fields, one [lz, ly, lx] array (or None) per orientation; u is the
+    index into that field.  Returns (orientation, (uz, uy, ux)) or (-1, None),
+    and the per-orientation minima."""
+    ex, ey, ez = env_shape
+    allowed = list(orientations) if len(orientations) else list(range(len(fields)))
+    minima = [None] * len(fields)
+    best = None
+    for o in allowed:
+        f = fields[o]
+        if f is None:
+            continue
+        zs, ys, xs = np.nonzero(f == 0)
+        if region_mask is not None:
+            rz, ry, rx = region_mask.shape
+            keep = region_mask[zs * rz // ez, ys * ry // ey, xs * rx // ex] != 0
+            zs, ys, xs = zs[keep], ys[keep], xs[keep]
+        if zs.size == 0:
+            continue
+        order = np.lexsort((xs, ys, zs))
+        i = order[0]
+        minima[o] = (int(zs[i]), int(ys[i]), int(xs[i]))
+        cand = (minima[o], o)
+        if best is None or cand < best:
+            best = cand
+    if best is None:
+        return -1, None, minima
+    return best[1], best[0], minima
