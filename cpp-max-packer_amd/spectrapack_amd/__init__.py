@@ -392,3 +392,103 @@ class Session:
 
     def device_bytes(self):
         return lib().sp_hip_session_device_bytes(self._h)
+
+
+class Proposal:
+    """Learned orientation/region proposal (BF16 3D CNN on MFMA)."""
+
+    def __init__(self, ctx: Context, max_batch: int = 32):
+        self.ctx = ctx
+        self.max_batch = int(max_batch)
+        self._h = C.c_void_p()
+        rc = lib().sp_hip_proposal_create(ctx.handle, C.c_uint32(self.max_batch), C.byref(self._h))
+        if rc:
+            ctx._fail(rc)
+        self.parameter_count = int(lib().sp_hip_proposal_parameter_count())
+        self.samples = 0
+
+    def close(self):
+        if self._h:
+            lib().sp_hip_proposal_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            self.ctx._fail(rc)
+
+    def set_parameters(self, flat):
+        a = np.ascontiguousarray(np.asarray(flat, dtype=np.float32).ravel())
+        self._check(lib().sp_hip_proposal_set_parameters(self._h, _p(a, C.c_float), C.c_uint64(a.size)))
+
+    def parameters(self):
+        a = np.zeros(self.parameter_count, np.float32)
+        self._check(lib().sp_hip_proposal_get_parameters(self._h, _p(a, C.c_float), C.c_uint64(a.size)))
+        return a
+
+    def reserve(self, samples):
+        self._check(lib().sp_hip_proposal_reserve(self._h, C.c_uint32(samples)))
+        self.samples = int(samples)
+
+    def load_samples(self, samples, first=0):
+        a = np.ascontiguousarray(np.asarray(samples, dtype=np.uint8))
+        n = a.shape[0]
+        self._check(lib().sp_hip_proposal_load_samples(self._h, _p(a, C.c_uint8), C.c_uint32(first), C.c_uint32(n)))
+
+    def record(self, session: "Session", slot):
+        self._check(lib().sp_hip_proposal_record(self._h, session._h, C.c_uint32(slot)))
+
+    def sample(self, slot):
+        a = np.zeros(64 ** 3, np.uint8)
+        self._check(lib().sp_hip_proposal_read_sample(self._h, C.c_uint32(slot), _p(a, C.c_uint8)))
+        return a.reshape(64, 64, 64)
+
+    def forward(self, slots, descriptors):
+        sl = np.ascontiguousarray(np.asarray(slots, dtype=np.uint32).ravel())
+        d = np.ascontiguousarray(np.asarray(descriptors, dtype=np.float32).reshape(sl.size, 32))
+        logits = np.zeros((sl.size, 96), np.float32)
+        heat = np.zeros((sl.size, 4096), np.float32)
+        self._check(lib().sp_hip_proposal_forward(self._h, _p(sl, C.c_uint32), C.c_uint32(sl.size), _p(d, C.c_float),
+                                                  _p(logits, C.c_float), _p(heat, C.c_float)))
+        return logits, heat
+
+    def train(self, slots, descriptors, target, mask, region, heat_weight=1.0, lr=1e-3, update=True,
+              gradients=False):
+        sl = np.ascontiguousarray(np.asarray(slots, dtype=np.uint32).ravel())
+        n = sl.size
+        d = np.ascontiguousarray(np.asarray(descriptors, dtype=np.float32).reshape(n, 32))
+        t = np.ascontiguousarray(np.asarray(target, dtype=np.float32).reshape(n, 96))
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8).reshape(n, 96))
+        r = np.ascontiguousarray(np.asarray(region, dtype=np.int32).reshape(n))
+        losses = np.zeros(2, np.float32)
+        g = np.zeros(self.parameter_count, np.float32) if gradients else None
+        self._check(lib().sp_hip_proposal_train(self._h, _p(sl, C.c_uint32), C.c_uint32(n), _p(d, C.c_float),
+                                                _p(t, C.c_float), _p(m, C.c_uint8), _p(r, C.c_int32),
+                                                C.c_float(heat_weight), C.c_float(lr), C.c_int(1 if update else 0),
+                                                _p(losses, C.c_float), _p(g, C.c_float) if gradients else None))
+        return (float(losses[0]), float(losses[1])), g
+
+    def step(self, session: "Session", descriptor, top_orientations=8, top_regions=512, stamp=True, fallback=True):
+        d = np.ascontiguousarray(np.asarray(descriptor, dtype=np.float32).ravel())
+        if d.size != 32:
+            raise ValueError("descriptor must hold 32 floats")
+        r = StepResult()
+        used = C.c_int(0)
+        self._check(lib().sp_hip_proposal_step(self._h, session._h, _p(d, C.c_float), C.c_uint32(top_orientations),
+                                               C.c_uint32(top_regions), C.c_int(1 if stamp else 0),
+                                               C.c_int(1 if fallback else 0), C.byref(r), C.byref(used)))
+        out = Session._result(r)
+        out["fallback"] = bool(used.value)
+        return out
+
+    def mfma_probe(self, a_bits, b_bits):
+        a = np.ascontiguousarray(np.asarray(a_bits, dtype=np.uint16).reshape(32, 16))
+        b = np.ascontiguousarray(np.asarray(b_bits, dtype=np.uint16).reshape(16, 32))
+        c = np.zeros((32, 32), np.float32)
+        self._check(lib().sp_hip_proposal_mfma_probe(self._h, _p(a, C.c_uint16), _p(b, C.c_uint16), _p(c, C.c_float)))
+        return c
