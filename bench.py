@@ -111,6 +111,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-deviation", action="store_true", help="skip the FP32-vs-FP64 deviation step")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal packing comparison")
+    ap.add_argument("--learned-seeds", default="9000", help="held-out part seeds packed with and without proposals")
+    ap.add_argument("--top-o", type=int, default=8)
+    ap.add_argument("--top-r", type=int, default=512)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -231,11 +235,48 @@ def main():
         log("[bench] FP32 vs FP64 deviation ...")
         out["fp32_vs_fp64"] = fp_deviation(sp, ctx, w, env, kernels, r["error_bound"])
         log(f"[bench] max |fp32 - fp64| = {out['fp32_vs_fp64']['max_abs_dev']:.3g} (bound {r['error_bound']:.3g})")
+    if not args.no_learned:
+        out["learned_proposal"] = learned_comparison(ctx, args)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline sample ...")
         out["cpu_baseline"] = cpu_baseline(w, env, kernels, args.cpu_threads, max(1, args.cpu_threads))
         out["cpu_baseline"]["host"] = platform.processor() or platform.machine()
     print(json.dumps(out), flush=True)
+
+
+def learned_comparison(ctx, args):
+    """Packs held-out star parts at 256^3 with 96 orientations to exhaustion,
+    once with the plain search and once with the learned proposal (top
+    orientations in the top regions, full search as fallback); wall-clock
+    steps/s of both loops (one step and its result read-back per iteration)."""
+    import spectrapack_amd as sp
+    from spectrapack_amd import learned as L
+    from spectrapack_amd import workloads as W
+    if not os.path.exists(L.DEFAULT_WEIGHTS):
+        return {"skipped": "no trained weights at " + os.path.relpath(L.DEFAULT_WEIGHTS, ROOT)}
+    prop = sp.Proposal(ctx, max_batch=1)
+    prop.set_parameters(L.load_weights())
+    meta_path = os.path.splitext(L.DEFAULT_WEIGHTS)[0] + ".json"
+    meta = json.load(open(meta_path)) if os.path.exists(meta_path) else {}
+    rows = []
+    for seed in [int(v) for v in args.learned_seeds.split(",")]:
+        sc = L.Scene(ctx, W.star_so3_fine_box(seed))
+        plain = L.run_episode(sc)
+        learned = L.run_episode(sc, prop, args.top_o, args.top_r)
+        log(f"[bench] learned proposal seed {seed}: plain {plain['parts']} parts {plain['steps_per_s']:.1f} steps/s, "
+            f"learned {learned['parts']} parts {learned['steps_per_s']:.1f} steps/s ({learned['fallbacks']} fallbacks)")
+        rows.append(dict(seed=seed, plain=plain, learned=learned))
+        sc.close()
+    pp = sum(r["plain"]["parts"] for r in rows)
+    lp = sum(r["learned"]["parts"] for r in rows)
+    ps = sum(r["plain"]["steps"] for r in rows) / sum(r["plain"]["seconds"] for r in rows)
+    ls = sum(r["learned"]["steps"] for r in rows) / sum(r["learned"]["seconds"] for r in rows)
+    return {"workload": W.star_so3_fine_box(rows[0]["seed"]).name, "top_orientations": args.top_o,
+            "top_regions": args.top_r, "plain_parts": pp, "learned_parts": lp, "parts_ratio": lp / max(pp, 1),
+            "plain_steps_per_s": ps, "learned_steps_per_s": ls, "speedup": ls / ps,
+            "fallbacks": sum(r["learned"]["fallbacks"] for r in rows), "episodes": rows,
+            "training": {k: meta.get(k) for k in ("episodes", "samples", "epochs", "scene", "hits_trained",
+                                                  "hits_untrained")}}
 
 
 def _fft_shape(env_shape):

@@ -10,7 +10,8 @@
 //   a4  16^3 x 32   ReLU(conv stride 1 (a3) + b4)
 //   heat[v]   = a4[v] . wh + bh                       (16^3 region logits)
 //   g         = mean over v of a4[v]                  (32)
-//   hid       = ReLU(Wa [g, descriptor] + ba)         (64)
+//   f         = sum over v of softmax(heat)[v] a4[v]  (32, pooled where the heat points)
+//   hid       = ReLU(Wa [g, f, descriptor] + ba)      (64)
 //   logits    = Wo hid + bo                           (96 orientation logits)
 // Convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16: rows are
 // output voxels (forward) or input voxels (input gradient), columns the 32
@@ -44,6 +45,9 @@ constexpr int kK = kTaps * kCh;          // 864
 constexpr int kDesc = 32;
 constexpr int kHidden = 64;
 constexpr int kOrient = 96;
+constexpr int kHeadIn = 2 * 32 + 32;     // [g, f, descriptor]
+constexpr int kPoolBlocks = 16;          // pooling blocks per sample (256 voxels each)
+constexpr int kPoolPart = 2 + 2 * 32;    // per block: max, sum of exp, exp-weighted and plain channel sums
 
 // Flat FP32 parameter layout.
 constexpr int P_W1 = 0, P_B1 = P_W1 + kCh * kTaps;
@@ -52,7 +56,7 @@ constexpr int P_W3 = P_B2 + kCh, P_B3 = P_W3 + kCh * kK;
 constexpr int P_W4 = P_B3 + kCh, P_B4 = P_W4 + kCh * kK;
 constexpr int P_WF = P_B4 + kCh;
 constexpr int P_WH = P_WF + kCh * kDesc, P_BH = P_WH + kCh;
-constexpr int P_WA = P_BH + 1, P_BA = P_WA + kHidden * (kCh + kDesc);
+constexpr int P_WA = P_BH + 1, P_BA = P_WA + kHidden * kHeadIn;
 constexpr int P_WO = P_BA + kHidden, P_BO = P_WO + kOrient * kHidden;
 constexpr int P_TOTAL = P_BO + kOrient;
 constexpr int kHeadParams = P_TOTAL - P_WH;  // wh .. bo, contiguous
@@ -115,7 +119,31 @@ __global__ void k_downsample(const u8* __restrict__ env, int ex, int ey, int ez,
 // MODE 1 (input gradient, CIN = 32): rows = input voxels at RES, source = dz at
 //   RES / S; A[p][t, co] = dz[v][co] where S v = p + 1 - d, B = W[co][t][ci]
 //   read from the transposed copy Wt[ci][t][co]; output gated by act[p] > 0.
-// Layer 1 (CIN = 1) reads uint8 fractions from `bank` at slot sample[b].
+// Layer 1 (CIN = 1, 2 MFMA steps per tile): one 32-row tile per wave, 4 per
+// block.  CIN = 32: one 32-row tile per block, its 27 taps split over the 4
+// waves (tap t on wave t % 4) and the partial tiles summed through LDS, which
+// keeps the batch-1 inference path spread over 4x more CUs.
+template <int MODE, int S, int RES>
+__device__ __forceinline__ bool conv_source(int rx, int ry, int rz, int t, int& sx, int& sy, int& sz)
+{
+    constexpr int SRES = MODE == 0 ? RES * S : RES / S;
+    const int dx = t % 3, dy = (t / 3) % 3, dz = t / 9;
+    if constexpr (MODE == 0) {
+        sx = S * rx + dx - 1;
+        sy = S * ry + dy - 1;
+        sz = S * rz + dz - 1;
+        return sx >= 0 && sy >= 0 && sz >= 0 && sx < SRES && sy < SRES && sz < SRES;
+    }
+    else {
+        const int qx = rx + 1 - dx, qy = ry + 1 - dy, qz = rz + 1 - dz;
+        bool ok = qx >= 0 && qy >= 0 && qz >= 0 && (qx % S) == 0 && (qy % S) == 0 && (qz % S) == 0;
+        sx = qx / S;
+        sy = qy / S;
+        sz = qz / S;
+        return ok && sx < SRES && sy < SRES && sz < SRES;
+    }
+}
+
 template <int MODE, int CIN, int S, int RES>
 __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ src_v, const unsigned* __restrict__ sample,
                                               const u16* __restrict__ w, const float* __restrict__ bias,
@@ -127,13 +155,14 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ src_v, co
     constexpr int SVOX = SRES * SRES * SRES;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, hi = lane >> 5, c32 = lane & 31;
     const int b = blockIdx.y;
-    const int row0 = (blockIdx.x * 4 + wave) * 32;
-    const int r = row0 + c32;
-    const int rx = r % RES, ry = (r / RES) % RES, rz = r / (RES * RES);
+    const long long base = (long long)b * ROWS;
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
     if constexpr (CIN == 1) {
+        const int row0 = (blockIdx.x * 4 + wave) * 32;
+        const int r = row0 + c32;
+        const int rx = r % RES, ry = (r / RES) % RES, rz = r / (RES * RES);
         const u8* x = static_cast<const u8*>(src_v) + (long long)sample[b] * kV0;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
@@ -142,37 +171,32 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ src_v, co
             for (int j = 0; j < 8; ++j) {
                 const int t = c * 16 + hi * 8 + j;
                 h[j] = 0;
-                if (t < kTaps) {
-                    const int px = S * rx + t % 3 - 1, py = S * ry + (t / 3) % 3 - 1, pz = S * rz + t / 9 - 1;
-                    if (px >= 0 && py >= 0 && pz >= 0 && px < SRES && py < SRES && pz < SRES)
-                        h[j] = input_value(x[px + SRES * (py + SRES * pz)]);
-                }
+                int sx, sy, sz;
+                if (t < kTaps && conv_source<0, S, RES>(rx, ry, rz, t, sx, sy, sz))
+                    h[j] = input_value(x[sx + SRES * (sy + SRES * sz)]);
             }
             const bf16x8 a = pack8(h);
             const bf16x8 bv = load8(w + c32 * kK1 + c * 16 + hi * 8);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
         }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int row = row0 + acc_row(i, lane);
+            const long long at = (base + row) * kCh + c32;
+            float v = acc[i] + bias[c32];
+            v = v > 0.f ? v : 0.f;
+            out[at] = f2bf(v);
+        }
     }
     else {
+        __shared__ float red[4][32 * 32];
+        const int row0 = blockIdx.x * 32;
+        const int r = row0 + c32;
+        const int rx = r % RES, ry = (r / RES) % RES, rz = r / (RES * RES);
         const u16* x = static_cast<const u16*>(src_v) + (long long)b * SVOX * kCh;
-        for (int t = 0; t < kTaps; ++t) {
-            const int dx = t % 3, dy = (t / 3) % 3, dz = t / 9;
+        for (int t = wave; t < kTaps; t += 4) {
             int sx, sy, sz;
-            bool ok;
-            if constexpr (MODE == 0) {
-                sx = S * rx + dx - 1;
-                sy = S * ry + dy - 1;
-                sz = S * rz + dz - 1;
-                ok = sx >= 0 && sy >= 0 && sz >= 0 && sx < SRES && sy < SRES && sz < SRES;
-            }
-            else {
-                const int qx = rx + 1 - dx, qy = ry + 1 - dy, qz = rz + 1 - dz;
-                ok = qx >= 0 && qy >= 0 && qz >= 0 && (qx % S) == 0 && (qy % S) == 0 && (qz % S) == 0;
-                sx = qx / S;
-                sy = qy / S;
-                sz = qz / S;
-                ok = ok && sx < SRES && sy < SRES && sz < SRES;
-            }
+            const bool ok = conv_source<MODE, S, RES>(rx, ry, rz, t, sx, sy, sz);
             const u16* srow = x + (long long)(ok ? sx + SRES * (sy + SRES * sz) : 0) * kCh;
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
@@ -182,23 +206,23 @@ __global__ __launch_bounds__(256) void k_conv(const void* __restrict__ src_v, co
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
             }
         }
-    }
-    // epilogue
-    const long long base = (long long)b * ROWS;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = row0 + acc_row(i, lane);
-        const long long at = (base + row) * kCh + c32;
-        float v = acc[i];
-        if constexpr (MODE == 0) {
-            v += bias[c32];
-            if (film) v += film[b * kCh + c32];
-            v = v > 0.f ? v : 0.f;
+        for (int i = 0; i < 16; ++i) red[wave][acc_row(i, lane) * 32 + c32] = acc[i];
+        __syncthreads();
+        for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+            const int row = row0 + e / 32, col = e % 32;
+            const long long at = (base + row) * kCh + col;
+            float v = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+            if constexpr (MODE == 0) {
+                v += bias[col];
+                if (film) v += film[b * kCh + col];
+                v = v > 0.f ? v : 0.f;
+            }
+            else {
+                v = bf2f(gate[at]) > 0.f ? v : 0.f;
+            }
+            out[at] = f2bf(v);
         }
-        else {
-            v = bf2f(gate[at]) > 0.f ? v : 0.f;
-        }
-        out[at] = f2bf(v);
     }
 }
 
@@ -338,39 +362,96 @@ __device__ float block_max(float v, float* sh)
     return t;
 }
 
-// Heads, one block (256 threads) per sample.
-__global__ __launch_bounds__(256) void k_heads_fwd(const u16* __restrict__ a4, const float* __restrict__ params,
-                                                   const float* __restrict__ desc, float* __restrict__ heat,
-                                                   float* __restrict__ logits, float* __restrict__ gsave,
-                                                   float* __restrict__ hsave)
+// Heads, first pass: grid (kPoolBlocks, batch), one voxel per thread.  Writes
+// the heat logits and per block: the block max m of the heat, sum of
+// exp(heat - m), the exp-weighted channel sums and the plain channel sums.
+__global__ __launch_bounds__(256) void k_heads_pool(const u16* __restrict__ a4, const float* __restrict__ params,
+                                                    float* __restrict__ heat, float* __restrict__ part)
 {
-    __shared__ float part[8][kCh];
-    __shared__ float u[kCh + kDesc];
-    __shared__ float hid[kHidden];
-    const int b = blockIdx.x;
-    const u16* a = a4 + (long long)b * kV2 * kCh;
-    for (int v = threadIdx.x; v < kV2; v += 256) {
-        float s = 0.f;
-        for (int c = 0; c < kCh; ++c) s += bf2f(a[v * kCh + c]) * params[P_WH + c];
-        heat[b * kV2 + v] = s + params[P_BH];
+    __shared__ float tile[kCh][257];
+    __shared__ float red[8][kCh];
+    __shared__ float sh[8];
+    const int b = blockIdx.y, v = blockIdx.x * 256 + threadIdx.x;
+    const u16* a = a4 + ((long long)b * kV2 + v) * kCh;
+    float av[kCh];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 w = *reinterpret_cast<const uint4*>(a + 8 * q);
+        const u16* h = reinterpret_cast<const u16*>(&w);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) av[8 * q + j] = bf2f(h[j]);
     }
+    float hv = 0.f;
+#pragma unroll
+    for (int c = 0; c < kCh; ++c) hv += av[c] * params[P_WH + c];
+    hv += params[P_BH];
+    heat[(long long)b * kV2 + v] = hv;
+    const float m = block_max(hv, sh);
+    const float e = expf(hv - m);
+    const float se = block_sum(e, sh);
+    float* out = part + ((long long)b * kPoolBlocks + blockIdx.x) * kPoolPart;
     const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
-    float s = 0.f;
-    for (int v = g; v < kV2; v += 8) s += bf2f(a[v * kCh + c]);
-    part[g][c] = s;
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+        for (int k = 0; k < kCh; ++k) tile[k][threadIdx.x] = pass == 0 ? e * av[k] : av[k];
+        __syncthreads();
+        float t = 0.f;
+        for (int r = 0; r < 32; ++r) t += tile[c][g * 32 + r];
+        red[g][c] = t;
+        __syncthreads();
+        if (threadIdx.x < kCh) {
+            float u = 0.f;
+            for (int i = 0; i < 8; ++i) u += red[i][threadIdx.x];
+            out[2 + pass * kCh + threadIdx.x] = u;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = m;
+        out[1] = se;
+    }
+}
+
+// Heads, second pass (one block per sample): merges the pooling blocks into
+// g, f and the softmax constants (max M, normaliser Z), then the MLP.
+__global__ __launch_bounds__(256) void k_heads_mlp(const float* __restrict__ part, const float* __restrict__ params,
+                                                   const float* __restrict__ desc, float* __restrict__ logits,
+                                                   float* __restrict__ usave, float* __restrict__ hsave,
+                                                   float* __restrict__ msave)
+{
+    __shared__ float u[kHeadIn];
+    __shared__ float hid[kHidden];
+    __shared__ float mz[2];
+    const int b = blockIdx.x;
+    const float* pp = part + (long long)b * kPoolBlocks * kPoolPart;
+    if (threadIdx.x == 0) {
+        float M = pp[0];
+        for (int k = 1; k < kPoolBlocks; ++k) M = fmaxf(M, pp[k * kPoolPart]);
+        float Z = 0.f;
+        for (int k = 0; k < kPoolBlocks; ++k) Z += pp[k * kPoolPart + 1] * expf(pp[k * kPoolPart] - M);
+        mz[0] = M;
+        mz[1] = Z;
+        msave[2 * b] = M;
+        msave[2 * b + 1] = Z;
+    }
     __syncthreads();
     if (threadIdx.x < kCh) {
-        float t = 0.f;
-        for (int i = 0; i < 8; ++i) t += part[i][threadIdx.x];
-        u[threadIdx.x] = t * (1.0f / kV2);
-        gsave[b * kCh + threadIdx.x] = u[threadIdx.x];
+        const int c = threadIdx.x;
+        float gsum = 0.f, fsum = 0.f;
+        for (int k = 0; k < kPoolBlocks; ++k) {
+            gsum += pp[k * kPoolPart + 2 + kCh + c];
+            fsum += pp[k * kPoolPart + 2 + c] * expf(pp[k * kPoolPart] - mz[0]);
+        }
+        u[c] = gsum * (1.0f / kV2);
+        u[kCh + c] = fsum / mz[1];
     }
-    else if (threadIdx.x < kCh + kDesc)
-        u[threadIdx.x] = desc[b * kDesc + threadIdx.x - kCh];
+    else if (threadIdx.x >= 2 * kCh && threadIdx.x < kHeadIn)
+        u[threadIdx.x] = desc[b * kDesc + threadIdx.x - 2 * kCh];
     __syncthreads();
+    if (threadIdx.x < 2 * kCh) usave[b * 2 * kCh + threadIdx.x] = u[threadIdx.x];
     if (threadIdx.x < kHidden) {
         float t = params[P_BA + threadIdx.x];
-        for (int i = 0; i < kCh + kDesc; ++i) t += params[P_WA + threadIdx.x * (kCh + kDesc) + i] * u[i];
+        for (int i = 0; i < kHeadIn; ++i) t += params[P_WA + threadIdx.x * kHeadIn + i] * u[i];
         t = t > 0.f ? t : 0.f;
         hid[threadIdx.x] = t;
         hsave[b * kHidden + threadIdx.x] = t;
@@ -385,23 +466,27 @@ __global__ __launch_bounds__(256) void k_heads_fwd(const u16* __restrict__ a4, c
 
 // Losses and head gradients, one block per sample.  Orientation loss: cross
 // entropy of the softmax over the sample's valid orientations against a target
-// distribution; region loss: cross entropy of the softmax over the 16^3 heat
-// logits against the region index (none when < 0), weighted by heat_weight.
-// Both are averaged over the batch.  Writes the per-sample head-parameter
-// gradients (layout wh .. bo) and dz of layer 4 (BF16).
+// distribution; region loss: cross entropy of the heat softmax (the one the
+// attention pooling uses, constants M and Z from the forward) against the
+// region index (none when < 0), weighted by heat_weight.  Both are averaged
+// over the batch.  Writes the per-sample head-parameter gradients (layout
+// wh .. bo) and dz of layer 4 (BF16), including the gradient through the
+// attention weights.
 __global__ __launch_bounds__(256) void k_heads_bwd(const u16* __restrict__ a4, const float* __restrict__ params,
                                                    const float* __restrict__ desc, const float* __restrict__ heat,
-                                                   const float* __restrict__ logits, const float* __restrict__ gsave,
-                                                   const float* __restrict__ hsave, const float* __restrict__ target,
-                                                   const u8* __restrict__ omask, const int* __restrict__ region,
-                                                   int batch, float heat_weight, float* __restrict__ losses,
-                                                   float* __restrict__ head_partial, u16* __restrict__ dz4)
+                                                   const float* __restrict__ logits, const float* __restrict__ usave,
+                                                   const float* __restrict__ hsave, const float* __restrict__ msave,
+                                                   const float* __restrict__ target, const u8* __restrict__ omask,
+                                                   const int* __restrict__ region, int batch, float heat_weight,
+                                                   float* __restrict__ losses, float* __restrict__ head_partial,
+                                                   u16* __restrict__ dz4)
 {
     __shared__ float dheat[kV2];
+    __shared__ float pis[kV2];
     __shared__ float dlog[kOrient];
     __shared__ float dpre[kHidden];
-    __shared__ float u[kCh + kDesc];
-    __shared__ float dg[kCh];
+    __shared__ float u[kHeadIn];
+    __shared__ float dg[kCh], df[kCh];
     __shared__ float sh[8];
     __shared__ float part[8][kCh];
     const int b = blockIdx.x;
@@ -423,26 +508,16 @@ __global__ __launch_bounds__(256) void k_heads_bwd(const u16* __restrict__ a4, c
         dlog[o] = valid ? (p - tg) * inv_b : 0.f;
     }
     const float loss_o = block_sum(lossp, sh);
-    // heat softmax
+    // heat softmax from the forward's constants
     const int reg = region[b];
-    float mh = -INFINITY;
-    for (int v = threadIdx.x; v < kV2; v += 256) mh = fmaxf(mh, heat[b * kV2 + v]);
-    mh = block_max(mh, sh);
-    float sum_h = 0.f;
-    for (int v = threadIdx.x; v < kV2; v += 256) {
-        const float e = expf(heat[b * kV2 + v] - mh);
-        dheat[v] = e;
-        sum_h += e;
-    }
-    sum_h = block_sum(sum_h, sh);
-    const float hw = reg >= 0 ? heat_weight * inv_b : 0.f;
-    for (int v = threadIdx.x; v < kV2; v += 256) dheat[v] = hw * (dheat[v] / sum_h - (v == reg ? 1.f : 0.f));
+    const float M = msave[2 * b], Z = msave[2 * b + 1];
+    for (int v = threadIdx.x; v < kV2; v += 256) pis[v] = expf(heat[b * kV2 + v] - M) / Z;
     if (threadIdx.x == 0) {
         losses[2 * b] = loss_o;
-        losses[2 * b + 1] = reg >= 0 ? -(heat[b * kV2 + reg] - mh - logf(sum_h)) : 0.f;
+        losses[2 * b + 1] = reg >= 0 ? -(heat[b * kV2 + reg] - M - logf(Z)) : 0.f;
     }
-    if (threadIdx.x < kCh) u[threadIdx.x] = gsave[b * kCh + threadIdx.x];
-    else if (threadIdx.x < kCh + kDesc) u[threadIdx.x] = desc[b * kDesc + threadIdx.x - kCh];
+    if (threadIdx.x < 2 * kCh) u[threadIdx.x] = usave[b * 2 * kCh + threadIdx.x];
+    else if (threadIdx.x < kHeadIn) u[threadIdx.x] = desc[b * kDesc + threadIdx.x - 2 * kCh];
     __syncthreads();
     // orientation head
     constexpr int oWO = P_WO - P_WH, oBO = P_BO - P_WH, oWA = P_WA - P_WH, oBA = P_BA - P_WH;
@@ -456,14 +531,28 @@ __global__ __launch_bounds__(256) void k_heads_bwd(const u16* __restrict__ a4, c
         dpre[j] = hsave[b * kHidden + j] > 0.f ? d : 0.f;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < kHidden * (kCh + kDesc); e += 256)
-        hp[oWA + e] = dpre[e / (kCh + kDesc)] * u[e % (kCh + kDesc)];
+    for (int e = threadIdx.x; e < kHidden * kHeadIn; e += 256) hp[oWA + e] = dpre[e / kHeadIn] * u[e % kHeadIn];
     if (threadIdx.x < kHidden) hp[oBA + threadIdx.x] = dpre[threadIdx.x];
-    if (threadIdx.x < kCh) {
+    if (threadIdx.x < 2 * kCh) {
         float d = 0.f;
-        for (int j = 0; j < kHidden; ++j) d += params[P_WA + j * (kCh + kDesc) + threadIdx.x] * dpre[j];
-        dg[threadIdx.x] = d * (1.0f / kV2);
+        for (int j = 0; j < kHidden; ++j) d += params[P_WA + j * kHeadIn + threadIdx.x] * dpre[j];
+        if (threadIdx.x < kCh) dg[threadIdx.x] = d * (1.0f / kV2);
+        else df[threadIdx.x - kCh] = d;
     }
+    __syncthreads();
+    // attention: s_v = a_v . df; dheat_v = hw (pi_v - [v = reg]) + pi_v (s_v - sum_u pi_u s_u)
+    float ps = 0.f;
+    for (int v = threadIdx.x; v < kV2; v += 256) {
+        float sv = 0.f;
+        for (int c = 0; c < kCh; ++c) sv += bf2f(a[v * kCh + c]) * df[c];
+        dheat[v] = sv;
+        ps += pis[v] * sv;
+    }
+    const float sbar = block_sum(ps, sh);
+    const float hw = reg >= 0 ? heat_weight * inv_b : 0.f;
+    for (int v = threadIdx.x; v < kV2; v += 256)
+        dheat[v] = hw * (pis[v] - (v == reg ? 1.f : 0.f)) + pis[v] * (dheat[v] - sbar);
+    __syncthreads();
     // heat head: dwh[c] = sum_v dheat[v] a4[v][c]; dbh = sum_v dheat[v]
     const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
     float s = 0.f;
@@ -479,11 +568,11 @@ __global__ __launch_bounds__(256) void k_heads_bwd(const u16* __restrict__ a4, c
     }
     if (threadIdx.x == 0) hp[P_BH - P_WH] = sb;
     __syncthreads();
-    // dz4 = (a4 > 0) * (dheat[v] wh[c] + dg[c])
+    // dz4 = (a4 > 0) * (dheat[v] wh[c] + dg[c] + pi[v] df[c])
     u16* out = dz4 + (long long)b * kV2 * kCh;
     for (int e = threadIdx.x; e < kV2 * kCh; e += 256) {
         const int v = e / kCh, cc = e % kCh;
-        const float d = dheat[v] * params[P_WH + cc] + dg[cc];
+        const float d = dheat[v] * params[P_WH + cc] + dg[cc] + pis[v] * df[cc];
         out[e] = f2bf(bf2f(a[e]) > 0.f ? d : 0.f);
     }
 }
@@ -564,16 +653,25 @@ __global__ void k_cast_weights(const float* __restrict__ p, u16* __restrict__ w1
 
 // Proposal selection for one sample: the top_o valid orientations by logit
 // (ties: lower index) into the active list, and the top_r regions by heat
-// logit (ties: lower index) into the region mask (16 per axis).
+// logit (ties: lower index) into the region mask (16 per axis).  Regions are
+// ordered by a bitonic sort of 64-bit keys (descending heat, then index).
+__device__ __forceinline__ unsigned descending_float_key(float f)
+{
+    const unsigned u = __float_as_uint(f);
+    const unsigned ascending = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return ~ascending;
+}
+
 __global__ __launch_bounds__(1024) void k_select(const float* __restrict__ logits, const float* __restrict__ heat,
                                                  const OrientDesc* __restrict__ descs, int orientations,
                                                  const int* __restrict__ cfg, int* __restrict__ act,
                                                  u8* __restrict__ region)
 {
-    __shared__ float hs[kV2];
+    __shared__ unsigned long long keys[kV2];
     __shared__ int nvalid;
     const int top_o = cfg[0], top_r = cfg[1];
-    for (int v = threadIdx.x; v < kV2; v += blockDim.x) hs[v] = heat[v];
+    for (int v = threadIdx.x; v < kV2; v += blockDim.x)
+        keys[v] = ((unsigned long long)descending_float_key(heat[v]) << 32) | unsigned(v);
     if (threadIdx.x == 0) nvalid = 0;
     __syncthreads();
     const int o = threadIdx.x;
@@ -589,15 +687,23 @@ __global__ __launch_bounds__(1024) void k_select(const float* __restrict__ logit
         }
         if (rank < top_o) act[kActHeader + rank] = o;
     }
-    for (int v = threadIdx.x; v < kV2; v += blockDim.x) {
-        const float hv = hs[v];
-        int rank = 0;
-        for (int q = 0; q < kV2; ++q) {
-            const float hq = hs[q];
-            rank += (hq > hv || (hq == hv && q < v)) ? 1 : 0;
+    // bitonic sort, ascending
+    for (int size = 2; size <= kV2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < kV2 / 2; i += blockDim.x) {
+                const int lo_i = 2 * i - (i & (stride - 1));
+                const int hi_i = lo_i + stride;
+                const bool up = (lo_i & size) == 0;
+                const unsigned long long a = keys[lo_i], b = keys[hi_i];
+                if ((a > b) == up) {
+                    keys[lo_i] = b;
+                    keys[hi_i] = a;
+                }
+            }
+            __syncthreads();
         }
-        region[v] = rank < top_r ? 1 : 0;
     }
+    for (int r = threadIdx.x; r < kV2; r += blockDim.x) region[unsigned(keys[r] & 0xffffffffu)] = r < top_r ? 1 : 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         act[0] = nvalid < top_o ? nvalid : top_o;
@@ -650,8 +756,10 @@ public:
         a4_.allocate(sizeof(u16) * B * kV2 * kCh);
         heat_.allocate(sizeof(float) * B * kV2);
         logits_.allocate(sizeof(float) * B * kOrient);
-        gsave_.allocate(sizeof(float) * B * kCh);
+        usave_.allocate(sizeof(float) * B * 2 * kCh);
         hsave_.allocate(sizeof(float) * B * kHidden);
+        msave_.allocate(sizeof(float) * B * 2);
+        pool_.allocate(sizeof(float) * B * kPoolBlocks * kPoolPart);
         infer_x_.allocate(kV0);
         infer_slot_.allocate(sizeof(unsigned));
         infer_desc_.allocate(sizeof(float) * kDesc);
@@ -851,14 +959,16 @@ private:
         hipLaunchKernelGGL(k_film, dim3((batch * kCh + 255) / 256), dim3(256), 0, st, p, desc, batch, film_.as<float>());
         hipLaunchKernelGGL((k_conv<0, 1, 2, 32>), dim3(kV1 / 128, batch), dim3(256), 0, st, bank, slots, w1_.as<u16>(),
                            p + P_B1, nullptr, nullptr, a1_.as<u16>());
-        hipLaunchKernelGGL((k_conv<0, 32, 2, 16>), dim3(kV2 / 128, batch), dim3(256), 0, st, a1_.get(), nullptr,
+        hipLaunchKernelGGL((k_conv<0, 32, 2, 16>), dim3(kV2 / 32, batch), dim3(256), 0, st, a1_.get(), nullptr,
                            w_[0].as<u16>(), p + P_B2, film_.as<float>(), nullptr, a2_.as<u16>());
-        hipLaunchKernelGGL((k_conv<0, 32, 1, 16>), dim3(kV2 / 128, batch), dim3(256), 0, st, a2_.get(), nullptr,
+        hipLaunchKernelGGL((k_conv<0, 32, 1, 16>), dim3(kV2 / 32, batch), dim3(256), 0, st, a2_.get(), nullptr,
                            w_[1].as<u16>(), p + P_B3, nullptr, nullptr, a3_.as<u16>());
-        hipLaunchKernelGGL((k_conv<0, 32, 1, 16>), dim3(kV2 / 128, batch), dim3(256), 0, st, a3_.get(), nullptr,
+        hipLaunchKernelGGL((k_conv<0, 32, 1, 16>), dim3(kV2 / 32, batch), dim3(256), 0, st, a3_.get(), nullptr,
                            w_[2].as<u16>(), p + P_B4, nullptr, nullptr, a4_.as<u16>());
-        hipLaunchKernelGGL(k_heads_fwd, dim3(batch), dim3(256), 0, st, a4_.as<u16>(), p, desc, heat_.as<float>(),
-                           logits_.as<float>(), gsave_.as<float>(), hsave_.as<float>());
+        hipLaunchKernelGGL(k_heads_pool, dim3(kPoolBlocks, batch), dim3(256), 0, st, a4_.as<u16>(), p,
+                           heat_.as<float>(), pool_.as<float>());
+        hipLaunchKernelGGL(k_heads_mlp, dim3(batch), dim3(256), 0, st, pool_.as<float>(), p, desc, logits_.as<float>(),
+                           usave_.as<float>(), hsave_.as<float>(), msave_.as<float>());
     }
 
     void ensure_training(int batch)
@@ -885,9 +995,9 @@ private:
         const float* p = params_.as<float>();
         float* g = grads_.as<float>();
         hipLaunchKernelGGL(k_heads_bwd, dim3(batch), dim3(256), 0, st, a4_.as<u16>(), p, desc_.as<float>(),
-                           heat_.as<float>(), logits_.as<float>(), gsave_.as<float>(), hsave_.as<float>(),
-                           target_.as<float>(), omask_.as<u8>(), region_.as<int>(), batch, heat_weight,
-                           losses_.as<float>(), head_partial_.as<float>(), dz4_.as<u16>());
+                           heat_.as<float>(), logits_.as<float>(), usave_.as<float>(), hsave_.as<float>(),
+                           msave_.as<float>(), target_.as<float>(), omask_.as<u8>(), region_.as<int>(), batch,
+                           heat_weight, losses_.as<float>(), head_partial_.as<float>(), dz4_.as<u16>());
         const int chunks2 = batch * kV2 / kRowsPerChunk, chunks1 = batch * kV1 / kRowsPerChunk;
         const int red2 = (kTaps * kCh * kCh + 255) / 256;
         // layer 4: weights (input a3), input gradient gated by a3
@@ -897,7 +1007,7 @@ private:
                            g + P_W4);
         hipLaunchKernelGGL(k_channel_sums, dim3(batch, kSumSegments), dim3(256), 0, st, dz4_.as<u16>(), kV2,
                            sums_[3].as<float>());
-        hipLaunchKernelGGL((k_conv<1, 32, 1, 16>), dim3(kV2 / 128, batch), dim3(256), 0, st, dz4_.get(), nullptr,
+        hipLaunchKernelGGL((k_conv<1, 32, 1, 16>), dim3(kV2 / 32, batch), dim3(256), 0, st, dz4_.get(), nullptr,
                            wt_[2].as<u16>(), nullptr, nullptr, a3_.as<u16>(), dz3_.as<u16>());
         // layer 3
         hipLaunchKernelGGL((k_conv_wgrad<32, 1, 16>), dim3(kTaps, chunks2), dim3(256), 0, st, a2_.get(), nullptr,
@@ -906,7 +1016,7 @@ private:
                            g + P_W3);
         hipLaunchKernelGGL(k_channel_sums, dim3(batch, kSumSegments), dim3(256), 0, st, dz3_.as<u16>(), kV2,
                            sums_[2].as<float>());
-        hipLaunchKernelGGL((k_conv<1, 32, 1, 16>), dim3(kV2 / 128, batch), dim3(256), 0, st, dz3_.get(), nullptr,
+        hipLaunchKernelGGL((k_conv<1, 32, 1, 16>), dim3(kV2 / 32, batch), dim3(256), 0, st, dz3_.get(), nullptr,
                            wt_[1].as<u16>(), nullptr, nullptr, a2_.as<u16>(), dz2_.as<u16>());
         // layer 2 (stride 2): weights from a1, input gradient at 32^3 gated by a1
         hipLaunchKernelGGL((k_conv_wgrad<32, 2, 16>), dim3(kTaps, chunks2), dim3(256), 0, st, a1_.get(), nullptr,
@@ -915,7 +1025,7 @@ private:
                            g + P_W2);
         hipLaunchKernelGGL(k_channel_sums, dim3(batch, kSumSegments), dim3(256), 0, st, dz2_.as<u16>(), kV2,
                            sums_[1].as<float>());
-        hipLaunchKernelGGL((k_conv<1, 32, 2, 32>), dim3(kV1 / 128, batch), dim3(256), 0, st, dz2_.get(), nullptr,
+        hipLaunchKernelGGL((k_conv<1, 32, 2, 32>), dim3(kV1 / 32, batch), dim3(256), 0, st, dz2_.get(), nullptr,
                            wt_[0].as<u16>(), nullptr, nullptr, a1_.as<u16>(), dz1_.as<u16>());
         // layer 1: weights only
         hipLaunchKernelGGL((k_conv_wgrad<1, 2, 32>), dim3(1, chunks1), dim3(256), 0, st, bank_.get(), slots_.as<unsigned>(),
@@ -934,7 +1044,7 @@ private:
     int max_batch_;
     hipStream_t stream_ {};
     DeviceBuffer params_, grads_, m_, v_, w1_, w_[3], wt_[3];
-    DeviceBuffer slots_, desc_, film_, a1_, a2_, a3_, a4_, heat_, logits_, gsave_, hsave_;
+    DeviceBuffer slots_, desc_, film_, a1_, a2_, a3_, a4_, heat_, logits_, usave_, hsave_, msave_, pool_;
     DeviceBuffer infer_x_, infer_slot_, infer_desc_, cfg_;
     DeviceBuffer dz4_, dz3_, dz2_, dz1_, partial_, sums_[4], head_partial_, target_, omask_, region_, losses_;
     DeviceBuffer bank_;

@@ -269,6 +269,7 @@ public:
     {
         if (on != use_rf_) {
             use_rf_ = on;
+            spectra_ready_ = false;
             drop_graphs();
         }
     }
@@ -501,6 +502,7 @@ public:
             else (void)twiddles<float>(n_[a]);
         }
         drop_graphs();
+        spectra_ready_ = false;
         prepared_ = true;
     }
 
@@ -516,6 +518,50 @@ public:
         prof_names_.push_back(name);
     }
 
+    // G1 + G2: each listed orientation's kernel spectrum up to its y pass (H2).
+    // It does not depend on the occupancy; with resident kernel spectra it
+    // runs once per preparation instead of in every step.
+    template <typename T> void enqueue_kernel_spectra(const int* act, int slots)
+    {
+        using T2 = cx<T>;
+        const int ny = n_[1], nz = n_[2];
+        const int nh = nz / 2 + 1;
+        const T2* twy = twiddles<T>(ny);
+        const T2* twz = twiddles<T>(nz);
+        T2* H1 = h1_.as<T2>();
+        T2* H2 = h2_.as<T2>();
+        const OrientDesc* dd = d_descs_.as<OrientDesc>();
+        const std::size_t e = sizeof(T2);
+        const long long h1o = (long long)nh * kmax_[1] * kmax_[0];
+        const long long h2o = (long long)nh * ny * kmax_[0];
+        // G1: kernel R2C along z over its own columns (x < kx, y < ky)
+        if (rf_supported(nz) && use_rf_) {
+            rf_r2c_launch<T, unsigned char>(nz, stream_, kmax_[0], kmax_[1], slots, kcells_.as<unsigned char>(), 0, 0, 0,
+                                            0, 0, H1, h1o, kmax_[0], kmax_[1], 0, 0, dd, act, twz);
+        }
+        else {
+            const int W = choose_w(nz, e, 2, true, 32, 2);
+            hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((kmax_[0] + W - 1) / W, kmax_[1], slots),
+                               dim3(kThreads), lds_bytes<T>(nz, 2, W / 2), stream_, kcells_.as<unsigned char>(), 0LL, 0,
+                               0, 0, 0, 0, H1, h1o, kmax_[0], kmax_[1], 0, 0, dd, act, twz, plans_[2], W);
+        }
+        mark("kernel_r2c_z");
+        // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
+        if (rf_supported(ny) && use_rf_) {
+            rf_cols_launch<T>(ny, stream_, kmax_[0], nh, slots, H1, h1o, (long long)kmax_[0], 1LL,
+                              (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)kmax_[0], 1LL,
+                              (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, 1);
+        }
+        else {
+            const int W = choose_w(ny, e, 2, false, 16, 1);
+            hipLaunchKernelGGL((k_c2c<T>), dim3((kmax_[0] + W - 1) / W, nh, slots), dim3(kThreads),
+                               lds_bytes<T>(ny, 2, W), stream_, H1, h1o, (long long)kmax_[0], 1LL,
+                               (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)kmax_[0], 1LL,
+                               (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, plans_[1], W, 1);
+        }
+        mark("kernel_c2c_y");
+    }
+
     template <typename T> void enqueue_step(bool stamp, const int* act, int slots)
     {
         using T2 = cx<T>;
@@ -525,7 +571,6 @@ public:
         const T2* twy = twiddles<T>(ny);
         const T2* twz = twiddles<T>(nz);
         T2* SB = sb_.as<T2>();
-        T2* H1 = h1_.as<T2>();
         T2* H2 = h2_.as<T2>();
         T2* TT = t_.as<T2>();
         const OrientDesc* dd = d_descs_.as<OrientDesc>();
@@ -572,36 +617,10 @@ public:
                                (long long)nx * ny, ny, nx, nx, nullptr, 0, nullptr, twx, plans_[0], W, 1);
             mark("env_c2c_x");
         }
-        const long long h1o = (long long)nh * kmax_[1] * kmax_[0];
         const long long h2o = (long long)nh * ny * kmax_[0];
         const long long to = (long long)nh * ny * nx;
+        if (slots > 0 && !resident_) enqueue_kernel_spectra<T>(act, slots);
         if (slots > 0) {
-            // G1: kernel R2C along z over its own columns (x < kx, y < ky)
-            if (rf_supported(nz) && use_rf_) {
-                rf_r2c_launch<T, unsigned char>(nz, stream_, kmax_[0], kmax_[1], slots, kcells_.as<unsigned char>(), 0, 0, 0,
-                                                0, 0, H1, h1o, kmax_[0], kmax_[1], 0, 0, dd, act, twz);
-            }
-            else {
-                const int W = choose_w(nz, e, 2, true, 32, 2);
-                hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((kmax_[0] + W - 1) / W, kmax_[1], slots),
-                                   dim3(kThreads), lds_bytes<T>(nz, 2, W / 2), stream_, kcells_.as<unsigned char>(), 0LL, 0,
-                                   0, 0, 0, 0, H1, h1o, kmax_[0], kmax_[1], 0, 0, dd, act, twz, plans_[2], W);
-            }
-            mark("kernel_r2c_z");
-            // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
-            if (rf_supported(ny) && use_rf_) {
-                rf_cols_launch<T>(ny, stream_, kmax_[0], nh, slots, H1, h1o, (long long)kmax_[0], 1LL,
-                                  (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)kmax_[0], 1LL,
-                                  (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, 1);
-            }
-            else {
-                const int W = choose_w(ny, e, 2, false, 16, 1);
-                hipLaunchKernelGGL((k_c2c<T>), dim3((kmax_[0] + W - 1) / W, nh, slots), dim3(kThreads),
-                                   lds_bytes<T>(ny, 2, W), stream_, H1, h1o, (long long)kmax_[0], 1LL,
-                                   (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)kmax_[0], 1LL,
-                                   (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, plans_[1], W, 1);
-            }
-            mark("kernel_c2c_y");
             // P1: finish kernel x transform, product with conj, inverse x
             if (rf_supported(nx) && use_rf_) {
                 rf_mul_launch<T>(nx, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twx);
@@ -650,6 +669,29 @@ public:
             hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
                                ext_[2], kcells_.as<unsigned char>(), dd, res);
             mark("stamp");
+        }
+    }
+
+    // Resident kernel spectra: H2 of every fitting orientation, computed once
+    // after preparation (outside any graph) and reused by every step.
+    void ensure_kernel_spectra()
+    {
+        if (!resident_ || spectra_ready_ || nfit_ == 0) return;
+        const bool prof = prof_;
+        prof_ = false;
+        if (precision_ == 64) enqueue_kernel_spectra<double>(act_full_.as<int>(), nfit_);
+        else enqueue_kernel_spectra<float>(act_full_.as<int>(), nfit_);
+        prof_ = prof;
+        SP_HIP_CHECK(hipGetLastError());
+        spectra_ready_ = true;
+    }
+
+    void set_resident_kernels(bool on)
+    {
+        if (on != resident_) {
+            resident_ = on;
+            spectra_ready_ = false;
+            drop_graphs();
         }
     }
 
@@ -704,6 +746,7 @@ public:
     {
         prepare();
         use_device(ctx_);
+        ensure_kernel_spectra();
         if (!desc_.capture_graph) {
             enqueue(stamp, subset);
             SP_HIP_CHECK(hipGetLastError());
@@ -719,6 +762,7 @@ public:
     {
         prepare();
         use_device(ctx_);
+        ensure_kernel_spectra();
         if (slots <= 0 || slots > O_) throw std::invalid_argument("proposal slots");
         auto record = [&] {
             propose(stream_);
@@ -754,6 +798,7 @@ public:
     {
         prepare();
         use_device(ctx_);
+        ensure_kernel_spectra();
         prof_ = true;
         prof_events_.clear();
         prof_names_.clear();
@@ -870,6 +915,8 @@ private:
     std::vector<CachedGraph> graphs_;
     bool prof_ {};
     bool use_rf_ { true };
+    bool resident_ { true };
+    bool spectra_ready_ {};
     std::vector<hipEvent_t> prof_events_;
     std::vector<const char*> prof_names_;
 };
@@ -990,6 +1037,12 @@ int sp_hip_session_set_active(sp_hip_session* s, const uint32_t* orientations, u
 {
     if (!s || (count && !orientations)) return SP_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] { s->s->set_active(orientations, count, region_mask, regions); });
+}
+
+int sp_hip_session_set_resident_kernels(sp_hip_session* s, int on)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_resident_kernels(on != 0); });
 }
 
 int sp_hip_session_orientation_minima(sp_hip_session* s, uint64_t* keys, uint32_t count)

@@ -294,6 +294,9 @@ class Session:
         self._check(lib().sp_hip_session_set_kernel(self._h, C.c_uint32(o), _p(a, C.c_uint8), sh, fi))
         self.kernel_shapes[o] = tuple(int(v) for v in shape)
 
+    def set_resident_kernels(self, on=True):
+        self._check(lib().sp_hip_session_set_resident_kernels(self._h, C.c_int(1 if on else 0)))
+
     def set_fast_path(self, on=True):
         self._check(lib().sp_hip_session_set_fast_path(self._h, C.c_int(1 if on else 0)))
 

@@ -1,0 +1,315 @@
+"""Learned proposal pipeline: plain-search episodes -> training samples ->
+training on the GPU -> packing with proposals, compared with plain search.
+
+Everything that computes runs on the device through the C ABI
+(sp_hip_session_*, sp_hip_proposal_*); this module holds the bookkeeping:
+the part descriptor, the labels taken from plain-search traces, the training
+schedule and the episode loops.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import numpy as np
+
+from . import Context, Proposal, Session
+from . import workloads as W
+
+REGIONS = 16
+DESCRIPTOR = 32
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
+DEFAULT_WEIGHTS = os.path.join(WEIGHTS_DIR, "proposal_star96.npy")
+
+
+SH_DEGREE = 4  # (SH_DEGREE + 1)^2 = 25 coefficients
+
+
+def real_spherical_harmonics(dirs, degree=SH_DEGREE):
+    """Orthonormal real spherical harmonics Y_lm (l <= degree) at unit
+    vectors dirs [n, 3], columns ordered by l then m = -l..l."""
+    from scipy.special import sph_harm_y
+    d = np.asarray(dirs, np.float64)
+    theta = np.arccos(np.clip(d[:, 2], -1.0, 1.0))
+    phi = np.arctan2(d[:, 1], d[:, 0])
+    cols = []
+    for l in range(degree + 1):
+        for m in range(-l, l + 1):
+            y = sph_harm_y(l, abs(m), theta, phi)
+            cols.append(np.sqrt(2.0) * (y.imag if m < 0 else y.real) if m != 0 else y.real)
+    return np.stack(cols, axis=1)
+
+
+def down_directions(catalog):
+    """Per orientation, the part-frame direction that the rotation maps to
+    world +z (the third row of its rotation matrix)."""
+    return np.array([W.rotation_matrix(q)[2] for q in catalog])
+
+
+def orientation_heights(kernels):
+    """Per oriented kernel: mean height of its occupied cells above its lowest
+    occupied layer, in cells (high for orientations resting on a point)."""
+    out = []
+    for k in kernels:
+        z = np.nonzero(np.asarray(k["cells"]) != 0)[0]
+        out.append(float((z - z.min()).mean()) if z.size else 0.0)
+    return np.array(out)
+
+
+def descriptor(kernels, catalog, env_shape):
+    """32 floats describing a part through its oriented kernels (cells
+    [z, y, x], shape (x, y, z)) and the catalog that produced them:
+    [0] rms radius of the first kernel's occupied cells about their centroid,
+        in units of the 64^3 network input, / 8;
+    [1:26] least-squares coefficients, over the catalog's down directions, of
+        the real spherical harmonics of degree <= 4 fitted to the standardised
+        per-orientation mean heights (orientation_heights);
+    [26:29] eigenvalues of the first kernel's cell covariance / rms^2 (sorted);
+    [29] mean height / rms; [30] height spread / rms; [31] fill fraction of
+    the first kernel's box.  Each orientation's fitted height is linear in
+    [1:26]."""
+    d = np.zeros(DESCRIPTOR, np.float64)
+    k0 = kernels[0]
+    c = np.asarray(k0["cells"]) != 0
+    z, y, x = np.nonzero(c)
+    if len(x) == 0:
+        return d.astype(np.float32)
+    scale = 64.0 / np.asarray(env_shape, np.float64)
+    p = np.stack([x, y, z], axis=1).astype(np.float64) + 0.5
+    q = p - p.mean(axis=0)
+    rms_cells = float(np.sqrt((q * q).sum(axis=1).mean()))
+    d[0] = float(np.sqrt(((q * scale) ** 2).sum(axis=1).mean())) / 8.0
+    h = orientation_heights(kernels)
+    mu, sd = float(h.mean()), float(h.std())
+    hs = (h - mu) / sd if sd > 0 else np.zeros_like(h)
+    B = real_spherical_harmonics(down_directions(catalog[:len(kernels)]))
+    coef, *_ = np.linalg.lstsq(B, hs, rcond=None)
+    d[1:26] = coef
+    if rms_cells > 0:
+        ev = np.sort(np.linalg.eigvalsh(q.T @ q / len(q)))
+        d[26:29] = ev / rms_cells ** 2
+        d[29] = mu / rms_cells
+        d[30] = sd / rms_cells
+    d[31] = len(x) / float(np.prod(k0["shape"]))
+    return d.astype(np.float32)
+
+
+# Flat parameter layout of the network (sp_hip_proposal_set_parameters).
+LAYOUT = [("w1", (32, 27)), ("b1", (32,)), ("w2", (32, 27, 32)), ("b2", (32,)), ("w3", (32, 27, 32)), ("b3", (32,)),
+          ("w4", (32, 27, 32)), ("b4", (32,)), ("wf", (32, 32)), ("wh", (32,)), ("bh", (1,)), ("wa", (64, 96)),
+          ("ba", (64,)), ("wo", (96, 64)), ("bo", (96,))]
+FAN_IN = {"w1": 27, "w2": 864, "w3": 864, "w4": 864, "wf": 32, "wh": 32, "wa": 96, "wo": 64}
+
+
+def initial_parameters(seed):
+    """He-normal weights (descriptor weights scaled by 0.1), zero biases."""
+    rs = np.random.default_rng(seed)
+    parts = []
+    for name, shape in LAYOUT:
+        if name.startswith("b"):
+            parts.append(np.zeros(int(np.prod(shape))))
+            continue
+        w = rs.normal(0.0, np.sqrt(2.0 / FAN_IN[name]), int(np.prod(shape)))
+        parts.append(w * (0.1 if name == "wf" else 1.0))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def region_index(u_xyz, env_shape):
+    """Region (16 per axis, X-fastest) of kernel corner u."""
+    r = [int(u_xyz[a]) * REGIONS // int(env_shape[a]) for a in range(3)]
+    return r[0] + REGIONS * (r[1] + REGIONS * r[2])
+
+
+def orientation_target(minima, chosen):
+    """Target over orientations from one plain step: half on the chosen
+    orientation, half spread evenly over the orientations whose lowest
+    feasible corner reaches the lowest z of the step (the chosen one among
+    them)."""
+    t = np.zeros(96, np.float32)
+    zs = [(m[0], o) for o, m in enumerate(minima) if m is not None]
+    if not zs or chosen < 0:
+        return t
+    zmin = min(z for z, _ in zs)
+    tied = [o for z, o in zs if z == zmin]
+    assert chosen in tied
+    t[tied] = 0.5 / len(tied)
+    t[chosen] += 0.5
+    return t
+
+
+class Scene:
+    """A workload's container and kernels on the device, and a session over them."""
+
+    def __init__(self, ctx: Context, w: W.Workload, capture_graph=True):
+        self.w = w
+        if w.container is not None:
+            self.env0 = ctx.mesh_container(w.container.vertices, w.container.triangles, w.env_first, w.env_shape,
+                                           w.pitch)
+        else:
+            self.env0 = ctx.box_container(w.env_first, w.env_shape, w.pitch, w.box_dims)
+        self.kernels = ctx.voxelize_objects(w.part.vertices, w.part.triangles, w.catalog, w.pitch)
+        prec = 32 if w.precision == "fp32" else 0
+        self.session = Session(ctx, w.env_shape, w.env_first, len(self.kernels), precision=prec,
+                               capture_graph=capture_graph)
+        for i, k in enumerate(self.kernels):
+            self.session.set_kernel(i, k["cells"], k["shape"], k["first"])
+        self.desc = descriptor(self.kernels, w.catalog, w.env_shape)
+        es = w.env_shape
+        self.fits = np.array([all(k["shape"][a] <= es[a] for a in range(3)) for k in self.kernels] +
+                             [False] * (96 - len(self.kernels)), np.uint8)
+        self.reset()
+
+    def reset(self):
+        self.session.set_active()
+        self.session.set_environment(self.env0)
+
+    def close(self):
+        self.session.close()
+
+
+def collect(ctx, proposal: Proposal, seeds, max_steps=100000, log=print, first_slot=0):
+    """Plain-search episodes (stamping every choice) recorded as samples in
+    the proposal's bank.  Returns the labels and per-episode statistics."""
+    slot = first_slot
+    rec = dict(slot=[], desc=[], target=[], mask=[], region=[], orientation=[], episode=[], tied=[])
+    episodes = []
+    for e, seed in enumerate(seeds):
+        t0 = time.perf_counter()
+        sc = Scene(ctx, W.star_so3_box(int(seed)))
+        s = sc.session
+        steps = 0
+        while steps < max_steps:
+            if slot >= proposal.samples:
+                raise RuntimeError("sample bank full")
+            proposal.record(s, slot)
+            r = s.step(stamp=True)
+            if r["orientation"] < 0:
+                break
+            minima = s.orientation_minima()
+            u = tuple(r["key"][a] - sc.w.env_first[a] for a in range(3))
+            rec["slot"].append(slot)
+            rec["desc"].append(sc.desc)
+            rec["target"].append(orientation_target(minima, r["orientation"]))
+            rec["mask"].append(sc.fits)
+            rec["region"].append(region_index(u, sc.w.env_shape))
+            rec["orientation"].append(r["orientation"])
+            zmin = min(m[0] for m in minima if m is not None)
+            rec["tied"].append(np.array([m is not None and m[0] == zmin for m in minima] +
+                                        [False] * (96 - len(minima)), bool))
+            rec["episode"].append(e)
+            slot += 1
+            steps += 1
+        dt = time.perf_counter() - t0
+        episodes.append(dict(seed=int(seed), parts=steps, seconds=dt))
+        log(f"[collect] episode {e + 1}/{len(seeds)} seed={seed} parts={steps} {dt:.1f}s samples={slot - first_slot}")
+        sc.close()
+    out = {k: np.asarray(v) for k, v in rec.items()}
+    out["episodes"] = episodes
+    return out
+
+
+def ranking_hits(proposal: Proposal, data, idx, top_o=8, top_r=512, batch=32):
+    """Fractions of samples whose plain choice lies in the proposal: chosen
+    orientation among the top_o valid logits, chosen region among the top_r
+    heat logits, both, and some proposed orientation reaching the step's
+    lowest z."""
+    ho = hr = hb = hz = 0
+    for i in range(0, len(idx), batch):
+        j = idx[i:i + batch]
+        logits, heat = proposal.forward(data["slot"][j], data["desc"][j])
+        for k, s in enumerate(j):
+            lg = np.where(data["mask"][s] != 0, logits[k], -np.inf)
+            to = np.argsort(-lg, kind="stable")[:top_o]
+            tr = np.argsort(-heat[k], kind="stable")[:top_r]
+            a = data["orientation"][s] in to
+            b = data["region"][s] in tr
+            ho += a
+            hr += b
+            hb += a and b
+            hz += bool(data["tied"][s][to].any())
+    n = max(len(idx), 1)
+    return dict(orientation=ho / n, region=hr / n, both=hb / n, lowest_z=hz / n)
+
+
+def train(proposal: Proposal, data, train_idx, epochs=6, batch=32, lr=1e-3, heat_weight=1.0, seed=0, log=print):
+    rs = np.random.default_rng(seed)
+    n = len(train_idx)
+    iters = epochs * ((n + batch - 1) // batch)
+    it = 0
+    history = []
+    for ep in range(epochs):
+        order = rs.permutation(train_idx)
+        lo_sum = lh_sum = 0.0
+        nb = 0
+        t0 = time.perf_counter()
+        for i in range(0, n, batch):
+            j = order[i:i + batch]
+            cur = lr * 0.5 * (1.0 + np.cos(np.pi * it / iters))
+            (lo, lh), _ = proposal.train(data["slot"][j], data["desc"][j], data["target"][j], data["mask"][j],
+                                         data["region"][j], heat_weight=heat_weight, lr=cur, update=True)
+            lo_sum += lo
+            lh_sum += lh
+            nb += 1
+            it += 1
+        history.append(dict(epoch=ep, loss_orientation=lo_sum / nb, loss_region=lh_sum / nb,
+                            seconds=time.perf_counter() - t0))
+        log(f"[train] epoch {ep + 1}/{epochs} loss_o={lo_sum / nb:.3f} loss_r={lh_sum / nb:.3f} "
+            f"{time.perf_counter() - t0:.1f}s")
+    return history
+
+
+def run_episode(scene: Scene, proposal: Proposal | None = None, top_o=8, top_r=512, max_steps=100000):
+    """Packs until nothing fits: the plain search, or proposals with the full
+    search as fallback.  Every choice is stamped.  Returns parts, steps,
+    seconds and fallbacks; steps include the final one that finds nothing."""
+    scene.reset()
+    s = scene.session
+    parts = steps = fallbacks = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        if proposal is None:
+            r = s.step(stamp=True)
+        else:
+            r = proposal.step(s, scene.desc, top_orientations=top_o, top_regions=top_r, stamp=True, fallback=True)
+            fallbacks += r["fallback"]
+        steps += 1
+        if r["orientation"] < 0:
+            break
+        parts += 1
+    dt = time.perf_counter() - t0
+    return dict(parts=parts, steps=steps, seconds=dt, steps_per_s=steps / dt, fallbacks=fallbacks)
+
+
+def run_episode_static(scene: Scene, orientations, max_steps=100000):
+    """Packs with a fixed orientation subset (full search as fallback): the
+    proposal without a network."""
+    scene.reset()
+    s = scene.session
+    parts = steps = fallbacks = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        s.set_active(orientations)
+        r = s.step(stamp=True)
+        if r["orientation"] < 0:
+            s.set_active()
+            r = s.step(stamp=True)
+            fallbacks += 1
+        steps += 1
+        if r["orientation"] < 0:
+            break
+        parts += 1
+    s.set_active()
+    dt = time.perf_counter() - t0
+    return dict(parts=parts, steps=steps, seconds=dt, steps_per_s=steps / dt, fallbacks=fallbacks)
+
+
+def save_weights(path, flat, meta):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.save(path, np.asarray(flat, np.float32), allow_pickle=False)
+    with open(os.path.splitext(path)[0] + ".json", "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+def load_weights(path=DEFAULT_WEIGHTS):
+    return np.load(path, allow_pickle=False)

@@ -333,9 +333,31 @@ def knot_in_fine_box(seed: int = DEFAULT_SEED) -> Workload:
                     box_dims=(255.0, 255.0, 255.0), precision="auto")
 
 
+# Fixed orientation catalog of the learned proposal (its 96 logits index it).
+PROPOSAL_CATALOG_SEED = 20261203
+
+
+def star_so3_box(seed: int = DEFAULT_SEED) -> Workload:
+    """Learned-proposal episodes: a 512-face star part (radii U[4,12] mm, seeded)
+    in a 128 mm box at 1 mm (130^3 window) with the fixed 96-orientation SO(3)
+    catalog."""
+    first, shape = box_window((128.0, 128.0, 128.0), 1.0)
+    return Workload("star512-in-box128-h1-o96", star(seed), 1.0, first, shape, so3_catalog(96, PROPOSAL_CATALOG_SEED),
+                    box_dims=(128.0, 128.0, 128.0), precision="fp32")
+
+
+def star_so3_fine_box(seed: int = DEFAULT_SEED) -> Workload:
+    """The same scene at 0.5 mm: a 127 mm box gives a 256^3 window; 96 orientations."""
+    first, shape = box_window((127.0, 127.0, 127.0), 0.5)
+    return Workload("star512-in-box127-h0.5-256cubed-o96", star(seed), 0.5, first, shape,
+                    so3_catalog(96, PROPOSAL_CATALOG_SEED), box_dims=(127.0, 127.0, 127.0), precision="fp32")
+
+
 WORKLOADS = {
     "box": box_in_box,
     "star": star_in_box,
     "bracket": bracket_in_sphere,
     "knot": knot_in_fine_box,
+    "star96": star_so3_box,
+    "star96_fine": star_so3_fine_box,
 }

@@ -194,6 +194,11 @@ int sp_hip_session_set_fast_path(sp_hip_session* s, int on);
  * deviation from FP64 on the same occupancy; requires keep_raw. */
 int sp_hip_session_keep_raw(sp_hip_session* s, int keep);
 int sp_hip_session_raw_values(sp_hip_session* s, uint32_t orientation, double* out, uint64_t count);
+/* 1 (default): every orientation's kernel spectrum (forward transform up to
+ * its y pass; it does not depend on the occupancy) is computed once after the
+ * kernels are set and stays resident in HBM; steps start from it.  0: each
+ * step recomputes it.  Results are identical. */
+int sp_hip_session_set_resident_kernels(sp_hip_session* s, int on);
 /* Search restriction (no reference counterpart; used by the learned
  * proposal).  The next steps consider only the listed orientations (count 0:
  * every orientation) and, when region_mask is not null, only kernel-window
@@ -247,10 +252,11 @@ int sp_hip_voxelize_mesh_container(sp_hip_ctx* ctx, const double* vertices, uint
  * descriptor.  Outputs: 96 orientation logits and 4096 region logits
  * (X-fastest 16^3).  Parameters are one flat FP32 array in this order:
  *   w1[32][27] b1[32]  w2[32][27][32] b2[32]  w3[32][27][32] b3[32]
- *   w4[32][27][32] b4[32]  wf[32][32]  wh[32] bh  wa[64][64] ba[64]
+ *   w4[32][27][32] b4[32]  wf[32][32]  wh[32] bh  wa[64][96] ba[64]
  *   wo[96][64] bo[96]
  * (conv weights [out][tap][in], tap = 9 dz + 3 dy + dx, source voxel
- * stride * v + d - 1). */
+ * stride * v + d - 1; wa reads [mean of layer 4, layer 4 pooled by the
+ * softmax of the region logits, descriptor]). */
 typedef struct sp_hip_proposal sp_hip_proposal;
 
 uint32_t sp_hip_proposal_parameter_count(void);

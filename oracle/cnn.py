@@ -32,7 +32,7 @@ LAYOUT = [
     ("w4", (CH, 27, CH)), ("b4", (CH,)),
     ("wf", (CH, DESC)),
     ("wh", (CH,)), ("bh", (1,)),
-    ("wa", (HIDDEN, CH + DESC)), ("ba", (HIDDEN,)),
+    ("wa", (HIDDEN, 2 * CH + DESC)), ("ba", (HIDDEN,)),
     ("wo", (ORIENT, HIDDEN)), ("bo", (ORIENT,)),
 ]
 PARAMETER_COUNT = sum(int(np.prod(s)) for _, s in LAYOUT)
@@ -140,11 +140,14 @@ def forward(flat, q, desc, emulate=True):
     a4f = a4.reshape(B, -1, CH)                               # [B, 4096, 32]
     heat = a4f @ P["wh"] + P["bh"][0]
     g = a4f.mean(axis=1)
-    u = np.concatenate([g, desc], axis=1)
+    pi = np.exp(heat - heat.max(axis=1, keepdims=True))
+    pi /= pi.sum(axis=1, keepdims=True)                       # softmax of the heat logits
+    fatt = np.einsum("bv,bvc->bc", pi, a4f)                   # heat-weighted pooling
+    u = np.concatenate([g, fatt, desc], axis=1)
     hid = np.maximum(u @ P["wa"].T + P["ba"], 0)
     logits = hid @ P["wo"].T + P["bo"]
     return dict(P=P, w=(w1, w2, w3, w4), x0=x0, cols=(c1, c2, c3, c4), a=(a1, a2, a3, a4), a4f=a4f, heat=heat, g=g,
-                u=u, hid=hid, logits=logits, desc=desc, emulate=emulate)
+                pi=pi, fatt=fatt, u=u, hid=hid, logits=logits, desc=desc, emulate=emulate)
 
 
 def _log_softmax(z, valid):
@@ -182,11 +185,16 @@ def loss_and_grads(flat, q, desc, target, mask, region, heat_weight=1.0, emulate
     dpre = (dlog @ P["wo"]) * (f["hid"] > 0)
     G["wa"] = dpre.T @ f["u"]
     G["ba"] = dpre.sum(0)
-    dg = (dpre @ P["wa"])[:, :CH] / f["a4f"].shape[1]
+    du = dpre @ P["wa"]
+    dg = du[:, :CH] / f["a4f"].shape[1]
+    df = du[:, CH:2 * CH]
+    # attention pooling: f = sum_v pi_v a_v with pi = softmax(heat)
+    sv = np.einsum("bvc,bc->bv", f["a4f"], df)
+    dheat = dheat + f["pi"] * (sv - (f["pi"] * sv).sum(axis=1, keepdims=True))
     G["wh"] = np.einsum("bv,bvc->c", dheat, f["a4f"])
     G["bh"] = np.array([dheat.sum()])
     a1, a2, a3, a4 = f["a"]
-    da4 = dheat[:, :, None] * P["wh"][None, None, :] + dg[:, None, :]
+    da4 = dheat[:, :, None] * P["wh"][None, None, :] + dg[:, None, :] + f["pi"][:, :, None] * df[:, None, :]
     dz4 = rw(np.where(a4.reshape(da4.shape) > 0, da4, 0.0)).reshape(a4.shape)
     w1, w2, w3, w4 = f["w"]
     c1, c2, c3, c4 = f["cols"]
@@ -219,7 +227,7 @@ def init_parameters(seed):
         if name.startswith("b"):
             d[name] = np.zeros(shape)
             continue
-        fan_in = {"w1": 27, "w2": 864, "w3": 864, "w4": 864, "wf": DESC, "wh": CH, "wa": CH + DESC,
+        fan_in = {"w1": 27, "w2": 864, "w3": 864, "w4": 864, "wf": DESC, "wh": CH, "wa": 2 * CH + DESC,
                   "wo": HIDDEN}[name]
         d[name] = rs.normal(0.0, np.sqrt(2.0 / fan_in), shape)
     d["wf"] *= 0.1

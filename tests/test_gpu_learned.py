@@ -142,7 +142,7 @@ def test_training_fits_a_small_batch(proposal):
     proposal.reserve(3)
     proposal.load_samples(q)
     first = None
-    for it in range(60):
+    for it in range(150):
         (lo, lh), _ = proposal.train([0, 1, 2], desc, target, mask, region, lr=3e-3, update=True)
         first = first if first is not None else lo + lh
     print(f"[fit] loss {first:.3f} -> {lo + lh:.3f}", flush=True)

@@ -91,3 +91,31 @@ def test_exhausted_returns_minus_one(ctx, oracle):
     k = dict(first=(0, 0, 0), shape=(2, 2, 2), cells=np.ones((2, 2, 2), np.uint8))
     s = make_session(ctx, env, (0, 0, 0), [k])
     assert s.step()["orientation"] == -1
+
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_kernel_spectra_resident_or_recomputed(ctx, oracle, resident):
+    rs = np.random.default_rng(11)
+    shape = (33, 27, 30)
+    first = (1, -2, 0)
+    env = (rs.uniform(0, 1, shape[::-1]) < 0.06).astype(np.uint8)
+    kernels = []
+    for _ in range(5):
+        s = tuple(int(v) for v in rs.integers(2, 9, 3))
+        kernels.append(dict(first=tuple(int(v) for v in rs.integers(-4, 4, 3)), shape=s,
+                            cells=(rs.uniform(0, 1, s[::-1]) < 0.8).astype(np.uint8)))
+    sess = make_session(ctx, env, first, kernels, precision=32)
+    sess.set_resident_kernels(resident)
+    sess.keep_fields(True)
+    for step in range(4):
+        print(f"[step] resident={resident} step={step}", flush=True)
+        o, t, fields = oracle.place_step(env, first, kernels, direct=True, fields=True)
+        r = sess.step(stamp=True)
+        for i in range(len(kernels)):
+            g = sess.overlap_field(i)
+            assert (g is None and fields[i] is None) or np.array_equal(g, fields[i])
+        assert r["orientation"] == o
+        if o < 0:
+            break
+        assert tuple(r["translation"]) == tuple(t)
+        env = oracle.stamp(env, first, kernels[o], t)

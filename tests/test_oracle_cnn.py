@@ -121,4 +121,4 @@ def test_planted_sign_error_is_caught():
 
 
 def test_layout_size():
-    assert cnn.PARAMETER_COUNT == 95393
+    assert cnn.PARAMETER_COUNT == 95393 + 64 * 32
