@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--learned-seeds", default="9000", help="held-out part seeds packed with and without proposals")
     ap.add_argument("--top-o", type=int, default=8)
     ap.add_argument("--top-r", type=int, default=512)
+    ap.add_argument("--only-learned", action="store_true",
+                    help="run only learned-proposal episodes (no plain step timing, no plain episodes); for profiling")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -138,6 +140,10 @@ def main():
     import spectrapack_amd as sp
     from spectrapack_amd import workloads as W
 
+    if args.only_learned:
+        ctx = sp.Context(0)
+        print(json.dumps({"learned_proposal": learned_comparison(ctx, args, plain=False)}), flush=True)
+        return
     log(f"[bench] workload={args.workload} steps={args.steps} warmup={args.warmup}")
     w = W.WORKLOADS[args.workload]()
     ctx = sp.Context(0)
@@ -161,10 +167,14 @@ def main():
     r = s.step(stamp=True)  # prepares buffers and captures the graph
     prec = r["precision"]
     log(f"[bench] precision fp{prec} bound={r['error_bound']:.3g} device bytes={s.device_bytes() / 2**30:.2f} GiB")
-    placed = 1 if r["orientation"] >= 0 else 0
+    tiles = {}
+    for p, name in enumerate(("mul_x", "inv_c2c_y", "c2r_z_argmin")):
+        radix, width, tms = s.tile(p)
+        tiles[name] = {"radix": radix, "width": width, "tuned_ms": tms}
+    log("[bench] autotuned tiles: " + ", ".join(f"{k} R{v['radix']} W{v['width']} ({v['tuned_ms']:.3f} ms)"
+                                                for k, v in tiles.items()))
     for _ in range(args.warmup):
         r = s.step(stamp=True)
-        placed += r["orientation"] >= 0
     stream = torch.cuda.ExternalStream(s.stream())
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -180,6 +190,22 @@ def main():
     ms_step = e0.elapsed_time(e1) / args.steps
     log(f"[bench] {args.steps} steps: {ms_step:.3f} ms/step (events), wall {t_wall / args.steps * 1e3:.3f} ms/step; "
         f"last choice {last['orientation']} residual {last['max_residual']:.3g}")
+
+    # the same steps with every orientation's kernel spectrum recomputed in
+    # each step (the spectra do not depend on the occupancy; by default they
+    # stay resident in HBM)
+    s.set_resident_kernels(False)
+    s.step(stamp=True)
+    f0 = torch.cuda.Event(enable_timing=True)
+    f1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    f0.record(stream)
+    s.run(args.steps, stamp=True)
+    f1.record(stream)
+    f1.synchronize()
+    ms_step_recompute = f0.elapsed_time(f1) / args.steps
+    s.set_resident_kernels(True)
+    log(f"[bench] kernel spectra recomputed every step: {ms_step_recompute:.3f} ms/step")
 
     # per-kernel breakdown (no graph, events between kernels)
     prof = {}
@@ -219,7 +245,9 @@ def main():
         "data": "synthetic (seeded meshes and catalog, seed 20261203)",
         "config": {"workload": w.name, "grid": list(w.env_shape), "fft_grid": list(n), "orientations": len(kernels),
                    "pitch_mm": w.pitch, "precision": f"fp{prec}", "error_bound": r["error_bound"],
-                   "graph": not args.no_graph, "parallelism": "single GPU"},
+                   "graph": not args.no_graph, "parallelism": "single GPU", "kernel_spectra": "resident",
+                   "autotuned_tiles": tiles},
+        "ms_per_step_kernel_spectra_recomputed": ms_step_recompute,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
@@ -228,7 +256,8 @@ def main():
         "kernel_ms": prof,
         "raster_s": t_raster,
         "container_s": t_container,
-        "parts_placed": int(placed + args.steps if last["orientation"] >= 0 else placed),
+        "steps_run": 2 + args.warmup + 2 * args.steps,
+        "last_step_found_a_placement": last["orientation"] >= 0,
         "max_residual_last_step": last["max_residual"],
     }
     if not args.no_deviation:
@@ -244,7 +273,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def learned_comparison(ctx, args):
+def learned_comparison(ctx, args, plain=True):
     """Packs held-out star parts at 256^3 with 96 orientations to exhaustion,
     once with the plain search and once with the learned proposal (top
     orientations in the top regions, full search as fallback); wall-clock
@@ -261,12 +290,19 @@ def learned_comparison(ctx, args):
     rows = []
     for seed in [int(v) for v in args.learned_seeds.split(",")]:
         sc = L.Scene(ctx, W.star_so3_fine_box(seed))
-        plain = L.run_episode(sc)
         learned = L.run_episode(sc, prop, args.top_o, args.top_r)
-        log(f"[bench] learned proposal seed {seed}: plain {plain['parts']} parts {plain['steps_per_s']:.1f} steps/s, "
+        if not plain:
+            log(f"[bench] learned proposal seed {seed}: {learned['parts']} parts {learned['steps_per_s']:.1f} steps/s")
+            rows.append(dict(seed=seed, learned=learned))
+            sc.close()
+            continue
+        pl = L.run_episode(sc)
+        log(f"[bench] learned proposal seed {seed}: plain {pl['parts']} parts {pl['steps_per_s']:.1f} steps/s, "
             f"learned {learned['parts']} parts {learned['steps_per_s']:.1f} steps/s ({learned['fallbacks']} fallbacks)")
-        rows.append(dict(seed=seed, plain=plain, learned=learned))
+        rows.append(dict(seed=seed, plain=pl, learned=learned))
         sc.close()
+    if not plain:
+        return {"episodes": rows, "top_orientations": args.top_o, "top_regions": args.top_r}
     pp = sum(r["plain"]["parts"] for r in rows)
     lp = sum(r["learned"]["parts"] for r in rows)
     ps = sum(r["plain"]["steps"] for r in rows) / sum(r["plain"]["seconds"] for r in rows)

@@ -94,6 +94,8 @@ struct Context {
     // scratch reused across correlate calls
     DeviceBuffer scratch[6];
     DeviceBuffer pinned_flag;
+    // autotuned (radix, tile width) per pass and shape, shared by sessions
+    std::map<std::string, std::pair<int, int>> tiles;
 };
 
 // Apply the context's device for the current host thread.

@@ -292,6 +292,8 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
     float my_res = 0.f;
     unsigned int my_flags = 0;
     int* fo = fields ? fields + field_offsets[o] : nullptr;
+    const bool any_excluded = excl[0] != 0;
+    const bool regions_on = filt.act && filt.act[1];
 #pragma unroll
     for (int s = 0; s < R; ++s) {
         const int z = t + P * s;
@@ -311,7 +313,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
             if (rv == T(0)) {
                 const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                                ((unsigned long long)x << 16) | (unsigned long long)o;
-                if (!excluded(excl, key) && region_allowed(filt, x, y, z))
+                if ((!any_excluded || !excluded(excl, key)) && (!regions_on || region_allowed(filt, x, y, z)))
                     mine = key < mine ? key : mine;
             }
         }

@@ -6,6 +6,7 @@
 // step runs on one stream; with capture_graph the sequence is recorded once as
 // a hipGraph and replayed per step.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <numeric>
@@ -91,30 +92,86 @@ template <typename T, int N, int R> constexpr std::size_t rf_lds(int pencils)
 
 inline bool rf_supported(int n) { return n == 256 || n == 512; }
 
-template <typename T>
-void rf_cols_launch(int n, hipStream_t st, int count_w_max, int batches, int orients, const cx<T>* in, long long in_o,
-                    long long in_sn, long long in_sw, long long in_sb, cx<T>* out, long long out_o, long long out_sn,
-                    long long out_sw, long long out_sb, int count_w, int n_in, int n_out, const OrientDesc* descs,
-                    int mode, const int* act, const cx<T>* tw, int fwd)
+// ---- tile candidates (radix R, tile width W) for the register-FFT passes ---
+// Every candidate of a list is compiled; the autotuner times the ones whose
+// LDS fits the device and keeps the fastest per shape.
+struct Tile {
+    int r = 0, w = 0;
+};
+template <int N, int R, int W> struct TileT {
+    static constexpr int n = N, r = R, w = W;
+};
+template <typename... Ts> struct TileList {};
+
+template <typename F, typename... Ts> bool dispatch_tile(TileList<Ts...>, int n, Tile t, F&& f)
 {
-    if (n == 256) {
-        constexpr int N = 256, R = 16, W = 32;
+    return ((n == Ts::n && t.r == Ts::r && t.w == Ts::w ? (f(Ts {}), true) : false) || ...);
+}
+
+template <typename... Ts> std::vector<Tile> tiles_for(TileList<Ts...>, int n)
+{
+    std::vector<Tile> v;
+    ((n == Ts::n ? v.push_back(Tile { Ts::r, Ts::w }) : void()), ...);
+    return v;
+}
+
+// P1 (kernel x pass + product + inverse x), P2 (inverse y), P3 (C2R z + argmin)
+using MulTiles = TileList<TileT<256, 16, 8>, TileT<256, 16, 16>, TileT<256, 16, 32>, TileT<256, 4, 4>, TileT<256, 4, 8>,
+                          TileT<256, 4, 16>, TileT<512, 8, 2>, TileT<512, 8, 4>, TileT<512, 8, 8>, TileT<512, 8, 16>>;
+using ColTiles = TileList<TileT<256, 16, 16>, TileT<256, 16, 32>, TileT<256, 16, 64>, TileT<256, 4, 8>, TileT<256, 4, 16>,
+                          TileT<512, 8, 4>, TileT<512, 8, 8>, TileT<512, 8, 16>>;
+using C2rTiles = TileList<TileT<256, 16, 32>, TileT<256, 16, 64>, TileT<256, 16, 128>, TileT<256, 4, 16>, TileT<256, 4, 32>,
+                          TileT<512, 8, 8>, TileT<512, 8, 16>, TileT<512, 8, 32>>;
+
+enum { kPassMul = 0, kPassInvY = 1, kPassC2r = 2, kPasses = 3 };
+
+inline Tile default_tile(int pass, int n)
+{
+    if (pass == kPassMul) return n == 256 ? Tile { 16, 16 } : Tile { 8, 4 };
+    if (pass == kPassInvY) return n == 256 ? Tile { 16, 32 } : Tile { 8, 16 };
+    return n == 256 ? Tile { 16, 64 } : Tile { 8, 16 };
+}
+
+inline std::vector<Tile> tile_candidates(int pass, int n)
+{
+    if (pass == kPassMul) return tiles_for(MulTiles {}, n);
+    if (pass == kPassInvY) return tiles_for(ColTiles {}, n);
+    return tiles_for(C2rTiles {}, n);
+}
+
+// Dynamic LDS bytes of a pass at a tile (elem = bytes per complex value).
+inline std::size_t tile_lds(int pass, int n, Tile t, std::size_t elem)
+{
+    const std::size_t stride = std::size_t(n) + n / 16 + 1;
+    if (pass == kPassC2r) {
+        const std::size_t fft = elem * (std::size_t(n) + std::size_t(t.w / 2) * stride);
+        const std::size_t stage = elem * (std::size_t(n) + std::size_t(n / 2 + 1) * (t.w + 1));
+        return std::max(fft, stage);
+    }
+    return elem * (std::size_t(n) + std::size_t(t.w) * stride);
+}
+
+template <typename T>
+void rf_cols_launch(int n, Tile tile, hipStream_t st, int count_w_max, int batches, int orients, const cx<T>* in,
+                    long long in_o, long long in_sn, long long in_sw, long long in_sb, cx<T>* out, long long out_o,
+                    long long out_sn, long long out_sw, long long out_sb, int count_w, int n_in, int n_out,
+                    const OrientDesc* descs, int mode, const int* act, const cx<T>* tw, int fwd)
+{
+    const bool ok = dispatch_tile(ColTiles {}, n, tile, [&](auto tt) {
+        using TT = decltype(tt);
+        constexpr int N = TT::n, R = TT::r, W = TT::w;
         hipLaunchKernelGGL((rf_cols<T, N, R, W>), dim3((count_w_max + W - 1) / W, batches, orients), dim3(W * (N / R)),
                            (rf_lds<T, N, R>(W)), st, in, in_o, in_sn, in_sw, in_sb, out, out_o, out_sn, out_sw, out_sb,
                            count_w, n_in, n_out, descs, mode, act, tw, fwd);
-    }
-    else {
-        constexpr int N = 512, R = 8, W = 16;
-        hipLaunchKernelGGL((rf_cols<T, N, R, W>), dim3((count_w_max + W - 1) / W, batches, orients), dim3(W * (N / R)),
-                           (rf_lds<T, N, R>(W)), st, in, in_o, in_sn, in_sw, in_sb, out, out_o, out_sn, out_sw, out_sb,
-                           count_w, n_in, n_out, descs, mode, act, tw, fwd);
-    }
+    });
+    if (!ok) throw DeviceError { "no such FFT tile" };
 }
 
 template <typename T>
 void rf_rows_launch(int n, hipStream_t st, cx<T>* data, int rows, long long row_pitch, int batches, long long batch_pitch,
                     const cx<T>* tw, int fwd)
 {
+    if (!rf_supported(n)) throw DeviceError { "register FFT length" };
     if (n == 256) {
         constexpr int N = 256, R = 16, W = 16;
         hipLaunchKernelGGL((rf_rows<T, N, R, W>), dim3((rows + W - 1) / W, batches), dim3(W * (N / R)), (rf_lds<T, N, R>(W)),
@@ -128,19 +185,16 @@ void rf_rows_launch(int n, hipStream_t st, cx<T>* data, int rows, long long row_
 }
 
 template <typename T>
-void rf_mul_launch(int n, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px, const cx<T>* SB,
-                   cx<T>* out, long long out_o, const OrientDesc* descs, const int* act, const cx<T>* tw)
+void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px,
+                   const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act, const cx<T>* tw)
 {
-    if (n == 256) {
-        constexpr int N = 256, R = 16, W = 16;
+    const bool ok = dispatch_tile(MulTiles {}, n, tile, [&](auto tt) {
+        using TT = decltype(tt);
+        constexpr int N = TT::n, R = TT::r, W = TT::w;
         hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st,
                            H, h_o, h_px, SB, ny, out, out_o, descs, act, tw);
-    }
-    else {
-        constexpr int N = 512, R = 8, W = 4;
-        hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st,
-                           H, h_o, h_px, SB, ny, out, out_o, descs, act, tw);
-    }
+    });
+    if (!ok) throw DeviceError { "no such FFT tile" };
 }
 
 template <typename T, typename In>
@@ -148,6 +202,7 @@ void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batche
                    int ex, int ey, int ez, cx<T>* out, long long out_bs, int out_px, int out_py, int cx_, int cy_,
                    const OrientDesc* descs, const int* act, const cx<T>* tw)
 {
+    if (!rf_supported(n)) throw DeviceError { "register FFT length" };
     if (n == 256) {
         constexpr int N = 256, R = 16, W = 32;
         hipLaunchKernelGGL((rf_r2c_z<T, N, R, W, In>), dim3((cols_max + W - 1) / W, rows_max, batches),
@@ -163,23 +218,20 @@ void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batche
 }
 
 template <typename T>
-void rf_c2r_launch(int n, hipStream_t st, int lx_max, int ly_max, int orients, const cx<T>* in, long long in_o, int nx,
-                   int ny, const OrientDesc* descs, T inv_total, StepDeviceResult* res, int* fields,
+void rf_c2r_launch(int n, Tile tile, hipStream_t st, int lx_max, int ly_max, int orients, const cx<T>* in, long long in_o,
+                   int nx, int ny, const OrientDesc* descs, T inv_total, StepDeviceResult* res, int* fields,
                    const long long* offsets, const unsigned long long* excl, double* raw, StepFilter filt,
                    const cx<T>* tw)
 {
-    if (n == 256) {
-        constexpr int N = 256, R = 16, W = 64;
+    const bool ok = dispatch_tile(C2rTiles {}, n, tile, [&](auto tt) {
+        using TT = decltype(tt);
+        constexpr int N = TT::n, R = TT::r, W = TT::w;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
         hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
-                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, raw, filt, tw);
-    }
-    else {
-        constexpr int N = 512, R = 8, W = 16;
-        const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
-        hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
-                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets, excl, raw, filt, tw);
-    }
+                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets,
+                           excl, raw, filt, tw);
+    });
+    if (!ok) throw DeviceError { "no such FFT tile" };
 }
 
 }  // namespace
@@ -200,6 +252,9 @@ public:
         descs_.resize(O_);
         use_device(ctx_);
         SP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        int lds = 0;
+        SP_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx_.device));
+        max_lds_ = std::size_t(lds);
         env_.allocate(std::size_t(ext_[0]) * ext_[1] * ext_[2]);
         SP_HIP_CHECK(hipMemsetAsync(env_.get(), 0, env_.bytes(), stream_));
         result_.allocate(sizeof(StepDeviceResult));
@@ -270,6 +325,7 @@ public:
         if (on != use_rf_) {
             use_rf_ = on;
             spectra_ready_ = false;
+            tuned_ = false;
             drop_graphs();
         }
     }
@@ -503,6 +559,12 @@ public:
         }
         drop_graphs();
         spectra_ready_ = false;
+        tuned_ = false;
+        for (int p = 0; p != kPasses; ++p)
+            if (!pinned_[p]) {
+                tile_[p] = default_tile(p, pass_length(p));
+                tile_ms_[p] = 0.f;
+            }
         prepared_ = true;
     }
 
@@ -548,7 +610,7 @@ public:
         mark("kernel_r2c_z");
         // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
         if (rf_supported(ny) && use_rf_) {
-            rf_cols_launch<T>(ny, stream_, kmax_[0], nh, slots, H1, h1o, (long long)kmax_[0], 1LL,
+            rf_cols_launch<T>(ny, default_tile(kPassInvY, ny), stream_, kmax_[0], nh, slots, H1, h1o, (long long)kmax_[0], 1LL,
                               (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)kmax_[0], 1LL,
                               (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, 1);
         }
@@ -560,6 +622,175 @@ public:
                                (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, plans_[1], W, 1);
         }
         mark("kernel_c2c_y");
+    }
+
+    // One register-FFT pass of the step (P1, P2 or P3) at a given tile.
+    template <typename T> void rf_pass(int pass, Tile tile, const int* act, int slots)
+    {
+        using T2 = cx<T>;
+        const int nx = n_[0], ny = n_[1], nz = n_[2];
+        const int nh = nz / 2 + 1;
+        T2* SB = sb_.as<T2>();
+        T2* H2 = h2_.as<T2>();
+        T2* TT = t_.as<T2>();
+        const OrientDesc* dd = d_descs_.as<OrientDesc>();
+        const long long h2o = (long long)nh * ny * kmax_[0];
+        const long long to = (long long)nh * ny * nx;
+        if (pass == kPassMul) {
+            rf_mul_launch<T>(nx, tile, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twiddles<T>(nx));
+        }
+        else if (pass == kPassInvY) {
+            rf_cols_launch<T>(ny, tile, stream_, lmax_[0], nh, slots, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
+                              (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, act, twiddles<T>(ny), 0);
+        }
+        else {
+            const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
+            const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
+                                    orient_best_.as<unsigned long long>() };
+            rf_c2r_launch<T>(nz, tile, stream_, lmax_[0], lmax_[1], slots, TT, to, nx, ny, dd, inv_total,
+                             result_.as<StepDeviceResult>(), keep_fields_ ? fields_.as<int>() : nullptr,
+                             d_offsets_.as<long long>(), excl_.as<unsigned long long>(),
+                             keep_raw_ ? raw_.as<double>() : nullptr, filt, twiddles<T>(nz));
+        }
+    }
+
+    int pass_length(int pass) const { return pass == kPassMul ? n_[0] : pass == kPassInvY ? n_[1] : n_[2]; }
+
+    bool tile_fits(int pass, Tile t) const
+    {
+        const std::size_t elem = precision_ == 64 ? sizeof(double2) : sizeof(float2);
+        return tile_lds(pass, pass_length(pass), t, elem) <= max_lds_;
+    }
+
+    // Project-written autotuner: times every fitting tile of each register-FFT
+    // pass on this session's buffers (one warm-up and three timed launches with
+    // HIP events) and keeps the fastest; choices are cached per shape in the
+    // context.  Pinned passes are left alone.  Runs outside graph capture.
+    template <typename T> void autotune_passes()
+    {
+        const int nx = n_[0], ny = n_[1], nz = n_[2];
+        bool any = false;
+        for (int pass = 0; pass != kPasses; ++pass) any = any || (!pinned_[pass] && rf_supported(pass_length(pass)));
+        if (!any) return;
+        // Tune on real data: one full step without stamping leaves the
+        // environment spectrum and the inverse-y output of the current
+        // occupancy in place.  P3 (whose cost depends on how many counts are
+        // zero) is timed first on that output and only reads it; P1 rewrites
+        // the buffer from the spectra each launch; P2 runs in place last (its
+        // cost does not depend on the values).  The next step recomputes all.
+        enqueue_step<T>(false, act_full_.as<int>(), nfit_);
+        SP_HIP_CHECK(hipGetLastError());
+        hipEvent_t e0, e1;
+        SP_HIP_CHECK(hipEventCreate(&e0));
+        SP_HIP_CHECK(hipEventCreate(&e1));
+        try {
+            for (int pass : { int(kPassC2r), int(kPassMul), int(kPassInvY) }) {
+                const int n = pass_length(pass);
+                if (pinned_[pass] || !rf_supported(n)) continue;
+                char key[160];
+                std::snprintf(key, sizeof key, "p%d n%d,%d,%d f%d k%d,%d l%d,%d s%d", pass, nx, ny, nz, precision_,
+                              kmax_[0], kmax_[1], lmax_[0], lmax_[1], nfit_);
+                auto it = ctx_.tiles.find(key);
+                if (it != ctx_.tiles.end()) {
+                    tile_[pass] = Tile { it->second.first, it->second.second };
+                    tile_ms_[pass] = -1.f;
+                    continue;
+                }
+                float best = -1.f;
+                Tile best_tile = default_tile(pass, n);
+                for (const Tile t : tile_candidates(pass, n)) {
+                    if (!tile_fits(pass, t)) continue;
+                    rf_pass<T>(pass, t, act_full_.as<int>(), nfit_);
+                    if (hipGetLastError() != hipSuccess) continue;  // launch refused: not a candidate here
+                    SP_HIP_CHECK(hipEventRecord(e0, stream_));
+                    for (int i = 0; i != 3; ++i) rf_pass<T>(pass, t, act_full_.as<int>(), nfit_);
+                    SP_HIP_CHECK(hipEventRecord(e1, stream_));
+                    SP_HIP_CHECK(hipEventSynchronize(e1));
+                    SP_HIP_CHECK(hipGetLastError());
+                    float ms = 0.f;
+                    SP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    if (best < 0.f || ms < best) {
+                        best = ms;
+                        best_tile = t;
+                    }
+                }
+                tile_[pass] = best_tile;
+                tile_ms_[pass] = best / 3.f;
+                ctx_.tiles[key] = { best_tile.r, best_tile.w };
+            }
+        }
+        catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+
+    void autotune()
+    {
+        if (!autotune_ || tuned_ || !use_rf_ || nfit_ == 0 || lmax_[0] <= 0) return;
+        const bool prof = prof_;
+        prof_ = false;
+        if (precision_ == 64) autotune_passes<double>();
+        else autotune_passes<float>();
+        prof_ = prof;
+        tuned_ = true;
+    }
+
+    void set_autotune(bool on)
+    {
+        autotune_ = on;
+        tuned_ = false;
+        if (!on)
+            for (int p = 0; p != kPasses; ++p)
+                if (!pinned_[p]) tile_[p] = default_tile(p, pass_length(p));
+        drop_graphs();
+    }
+
+    void set_tile(int pass, int r, int w)
+    {
+        prepare();
+        if (pass < 0 || pass >= kPasses) throw std::invalid_argument("pass");
+        const int n = pass_length(pass);
+        if (!rf_supported(n)) throw std::invalid_argument("the pass uses the generic FFT at this length");
+        if (r == 0 && w == 0) {
+            pinned_[pass] = false;
+            tile_[pass] = default_tile(pass, n);
+            tuned_ = false;
+            drop_graphs();
+            return;
+        }
+        bool known = false;
+        for (const Tile t : tile_candidates(pass, n)) known = known || (t.r == r && t.w == w);
+        if (!known) throw std::invalid_argument("no such tile");
+        if (!tile_fits(pass, Tile { r, w })) throw std::invalid_argument("tile exceeds the LDS of the device");
+        pinned_[pass] = true;
+        tile_[pass] = Tile { r, w };
+        tile_ms_[pass] = 0.f;
+        drop_graphs();
+    }
+
+    void get_tile(int pass, int& r, int& w, float& ms)
+    {
+        prepare();
+        if (pass < 0 || pass >= kPasses) throw std::invalid_argument("pass");
+        r = tile_[pass].r;
+        w = tile_[pass].w;
+        ms = tile_ms_[pass];
+    }
+
+    std::vector<Tile> fitting_tiles(int pass)
+    {
+        prepare();
+        if (pass < 0 || pass >= kPasses) throw std::invalid_argument("pass");
+        std::vector<Tile> out;
+        const int n = pass_length(pass);
+        if (!rf_supported(n)) return out;
+        for (const Tile t : tile_candidates(pass, n))
+            if (tile_fits(pass, t)) out.push_back(t);
+        return out;
     }
 
     template <typename T> void enqueue_step(bool stamp, const int* act, int slots)
@@ -594,7 +825,7 @@ public:
         }
         // E2: C2C along y (columns x, batch kz)
         if (rf_supported(ny) && use_rf_) {
-            rf_cols_launch<T>(ny, stream_, nx, nh, 1, SB, 0LL, (long long)nx, 1LL, (long long)nx * ny, SB, 0LL,
+            rf_cols_launch<T>(ny, default_tile(kPassInvY, ny), stream_, nx, nh, 1, SB, 0LL, (long long)nx, 1LL, (long long)nx * ny, SB, 0LL,
                               (long long)nx, 1LL, (long long)nx * ny, nx, ny, ny, nullptr, 0, nullptr, twy, 1);
             mark("env_c2c_y");
         }
@@ -623,7 +854,7 @@ public:
         if (slots > 0) {
             // P1: finish kernel x transform, product with conj, inverse x
             if (rf_supported(nx) && use_rf_) {
-                rf_mul_launch<T>(nx, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twx);
+                rf_pass<T>(kPassMul, tile_[kPassMul], act, slots);
             }
             else {
                 const int W = choose_w(nx, e, 4, false, 16, 1);
@@ -636,8 +867,7 @@ public:
         if (slots > 0 && lmax_[0] > 0) {
             // P2: inverse C2C along y for the legal columns, rows y < ly written
             if (rf_supported(ny) && use_rf_) {
-                rf_cols_launch<T>(ny, stream_, lmax_[0], nh, slots, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
-                                  (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, act, twy, 0);
+                rf_pass<T>(kPassInvY, tile_[kPassInvY], act, slots);
             }
             else {
                 const int W = choose_w(ny, e, 2, false, 16, 1);
@@ -651,9 +881,7 @@ public:
             const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
                                     orient_best_.as<unsigned long long>() };
             if (rf_supported(nz) && use_rf_) {
-                rf_c2r_launch<T>(nz, stream_, lmax_[0], lmax_[1], slots, TT, to, nx, ny, dd, inv_total, res,
-                                 keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
-                                 excl_.as<unsigned long long>(), keep_raw_ ? raw_.as<double>() : nullptr, filt, twz);
+                rf_pass<T>(kPassC2r, tile_[kPassC2r], act, slots);
             }
             else {
                 const int W3 = choose_w(nz, e, 2, true, 32, 2);
@@ -747,6 +975,7 @@ public:
         prepare();
         use_device(ctx_);
         ensure_kernel_spectra();
+        autotune();
         if (!desc_.capture_graph) {
             enqueue(stamp, subset);
             SP_HIP_CHECK(hipGetLastError());
@@ -763,6 +992,7 @@ public:
         prepare();
         use_device(ctx_);
         ensure_kernel_spectra();
+        autotune();
         if (slots <= 0 || slots > O_) throw std::invalid_argument("proposal slots");
         auto record = [&] {
             propose(stream_);
@@ -799,6 +1029,7 @@ public:
         prepare();
         use_device(ctx_);
         ensure_kernel_spectra();
+        autotune();
         prof_ = true;
         prof_events_.clear();
         prof_names_.clear();
@@ -917,6 +1148,11 @@ private:
     bool use_rf_ { true };
     bool resident_ { true };
     bool spectra_ready_ {};
+    bool autotune_ { true }, tuned_ {};
+    Tile tile_[kPasses] {};
+    float tile_ms_[kPasses] {};
+    bool pinned_[kPasses] {};
+    std::size_t max_lds_ { 65536 };
     std::vector<hipEvent_t> prof_events_;
     std::vector<const char*> prof_names_;
 };
@@ -1037,6 +1273,44 @@ int sp_hip_session_set_active(sp_hip_session* s, const uint32_t* orientations, u
 {
     if (!s || (count && !orientations)) return SP_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] { s->s->set_active(orientations, count, region_mask, regions); });
+}
+
+int sp_hip_session_set_autotune(sp_hip_session* s, int on)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_autotune(on != 0); });
+}
+
+int sp_hip_session_set_tile(sp_hip_session* s, int pass, int radix, int width)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_tile(pass, radix, width); });
+}
+
+int sp_hip_session_get_tile(sp_hip_session* s, int pass, int32_t* radix, int32_t* width, float* ms)
+{
+    if (!s || !radix || !width || !ms) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        int r = 0, w = 0;
+        float t = 0.f;
+        s->s->get_tile(pass, r, w, t);
+        *radix = r;
+        *width = w;
+        *ms = t;
+    });
+}
+
+int sp_hip_session_tile_candidates(sp_hip_session* s, int pass, int32_t* pairs, uint32_t capacity, uint32_t* count)
+{
+    if (!s || !count) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        const auto tiles = s->s->fitting_tiles(pass);
+        *count = uint32_t(tiles.size());
+        for (uint32_t i = 0; i < tiles.size() && i < capacity && pairs; ++i) {
+            pairs[2 * i] = tiles[i].r;
+            pairs[2 * i + 1] = tiles[i].w;
+        }
+    });
 }
 
 int sp_hip_session_set_resident_kernels(sp_hip_session* s, int on)

@@ -294,6 +294,23 @@ class Session:
         self._check(lib().sp_hip_session_set_kernel(self._h, C.c_uint32(o), _p(a, C.c_uint8), sh, fi))
         self.kernel_shapes[o] = tuple(int(v) for v in shape)
 
+    def set_autotune(self, on=True):
+        self._check(lib().sp_hip_session_set_autotune(self._h, C.c_int(1 if on else 0)))
+
+    def set_tile(self, pass_, radix, width):
+        self._check(lib().sp_hip_session_set_tile(self._h, C.c_int(pass_), C.c_int(radix), C.c_int(width)))
+
+    def tile(self, pass_):
+        r, w, ms = C.c_int32(), C.c_int32(), C.c_float()
+        self._check(lib().sp_hip_session_get_tile(self._h, C.c_int(pass_), C.byref(r), C.byref(w), C.byref(ms)))
+        return r.value, w.value, ms.value
+
+    def tile_candidates(self, pass_):
+        pairs = (C.c_int32 * 64)()
+        n = C.c_uint32()
+        self._check(lib().sp_hip_session_tile_candidates(self._h, C.c_int(pass_), pairs, C.c_uint32(32), C.byref(n)))
+        return [(pairs[2 * i], pairs[2 * i + 1]) for i in range(n.value)]
+
     def set_resident_kernels(self, on=True):
         self._check(lib().sp_hip_session_set_resident_kernels(self._h, C.c_int(1 if on else 0)))
 

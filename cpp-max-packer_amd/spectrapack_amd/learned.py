@@ -122,10 +122,10 @@ def region_index(u_xyz, env_shape):
 
 
 def orientation_target(minima, chosen):
-    """Target over orientations from one plain step: half on the chosen
-    orientation, half spread evenly over the orientations whose lowest
-    feasible corner reaches the lowest z of the step (the chosen one among
-    them)."""
+    """Target over orientations from one plain step: uniform over the
+    orientations whose lowest feasible corner reaches the step's lowest z
+    (the chosen orientation is one of them).  A proposal that contains any of
+    them places the part at the same height as the plain search."""
     t = np.zeros(96, np.float32)
     zs = [(m[0], o) for o, m in enumerate(minima) if m is not None]
     if not zs or chosen < 0:
@@ -133,8 +133,7 @@ def orientation_target(minima, chosen):
     zmin = min(z for z, _ in zs)
     tied = [o for z, o in zs if z == zmin]
     assert chosen in tied
-    t[tied] = 0.5 / len(tied)
-    t[chosen] += 0.5
+    t[tied] = 1.0 / len(tied)
     return t
 
 
@@ -263,8 +262,13 @@ def run_episode(scene: Scene, proposal: Proposal | None = None, top_o=8, top_r=5
     """Packs until nothing fits: the plain search, or proposals with the full
     search as fallback.  Every choice is stamped.  Returns parts, steps,
     seconds and fallbacks; steps include the final one that finds nothing."""
-    scene.reset()
     s = scene.session
+    # warm-up outside the timed loop: tuning and graph capture of both modes
+    scene.reset()
+    if proposal is not None:
+        proposal.step(s, scene.desc, top_orientations=top_o, top_regions=top_r, stamp=True, fallback=False)
+    s.step(stamp=True)
+    scene.reset()
     parts = steps = fallbacks = 0
     t0 = time.perf_counter()
     while steps < max_steps:
@@ -284,8 +288,13 @@ def run_episode(scene: Scene, proposal: Proposal | None = None, top_o=8, top_r=5
 def run_episode_static(scene: Scene, orientations, max_steps=100000):
     """Packs with a fixed orientation subset (full search as fallback): the
     proposal without a network."""
-    scene.reset()
     s = scene.session
+    scene.reset()
+    s.set_active(orientations)
+    s.step(stamp=True)
+    s.set_active()
+    s.step(stamp=True)
+    scene.reset()
     parts = steps = fallbacks = 0
     t0 = time.perf_counter()
     while steps < max_steps:

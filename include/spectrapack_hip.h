@@ -194,6 +194,19 @@ int sp_hip_session_set_fast_path(sp_hip_session* s, int on);
  * deviation from FP64 on the same occupancy; requires keep_raw. */
 int sp_hip_session_keep_raw(sp_hip_session* s, int keep);
 int sp_hip_session_raw_values(sp_hip_session* s, uint32_t orientation, double* out, uint64_t count);
+/* Autotuner of the register-FFT passes (256- and 512-point axes).  pass 0:
+ * kernel x transform + product + inverse x; 1: inverse y; 2: C2R z + argmin.
+ * Each candidate (radix, tile width) whose LDS fits the device is timed on
+ * the session's own buffers before the first step (events, 3 launches) and
+ * the fastest is kept, cached per shape for the context.  set_autotune(0)
+ * uses the fixed defaults; set_tile pins one pass (radix 0 and width 0
+ * unpin it); get_tile reports the tile in use and its measured ms per launch
+ * (0: not measured, -1: taken from the cache); tile_candidates lists the
+ * fitting (radix, width) pairs.  Every tile gives identical results. */
+int sp_hip_session_set_autotune(sp_hip_session* s, int on);
+int sp_hip_session_set_tile(sp_hip_session* s, int pass, int radix, int width);
+int sp_hip_session_get_tile(sp_hip_session* s, int pass, int32_t* radix, int32_t* width, float* ms);
+int sp_hip_session_tile_candidates(sp_hip_session* s, int pass, int32_t* pairs, uint32_t capacity, uint32_t* count);
 /* 1 (default): every orientation's kernel spectrum (forward transform up to
  * its y pass; it does not depend on the occupancy) is computed once after the
  * kernels are set and stays resident in HBM; steps start from it.  0: each
