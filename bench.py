@@ -112,6 +112,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-deviation", action="store_true", help="skip the FP32-vs-FP64 deviation step")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal packing comparison")
+    ap.add_argument("--no-incremental", action="store_true", help="skip the incremental-step timing")
     ap.add_argument("--learned-seeds", default="9000", help="held-out part seeds packed with and without proposals")
     ap.add_argument("--top-o", type=int, default=8)
     ap.add_argument("--top-r", type=int, default=512)
@@ -207,6 +208,42 @@ def main():
     s.set_resident_kernels(True)
     log(f"[bench] kernel spectra recomputed every step: {ms_step_recompute:.3f} ms/step")
 
+    # incremental steps (fields updated by each step's new cells) on a second
+    # session from the same starting occupancy, checked against this one
+    incremental = None
+    if not args.no_incremental:
+        inc = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=prec, capture_graph=not args.no_graph)
+        inc.set_environment(env)
+        for i, k in enumerate(kernels):
+            inc.set_kernel(i, k["cells"], k["shape"], k["first"])
+        inc.set_incremental(True)
+        ref = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=prec, capture_graph=not args.no_graph)
+        ref.set_environment(env)
+        for i, k in enumerate(kernels):
+            ref.set_kernel(i, k["cells"], k["shape"], k["first"])
+        for _ in range(1 + args.warmup):
+            inc.step(stamp=True)
+            ref.step(stamp=True)
+        istream = torch.cuda.ExternalStream(inc.stream())
+        g0 = torch.cuda.Event(enable_timing=True)
+        g1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        g0.record(istream)
+        inc.run(args.steps, stamp=True)
+        g1.record(istream)
+        g1.synchronize()
+        ms_inc = g0.elapsed_time(g1) / args.steps
+        ref.run(args.steps, stamp=True)
+        same = bool(np.array_equal(inc.environment(), ref.environment())) and \
+            inc.last_result()["translation"] == ref.last_result()["translation"]
+        incremental = {"ms_per_step": ms_inc, "steps_per_s": 1000.0 / ms_inc, "speedup_vs_full": ms_step / ms_inc,
+                       "same_occupancy_and_choice_as_full_steps": same,
+                       "note": "overlap fields updated by the correlation of each step's new cells "
+                               "(gap_fill.cpp:426-451,524-584); first step full"}
+        log(f"[bench] incremental steps: {ms_inc:.3f} ms/step ({1000.0 / ms_inc:.1f} steps/s), same as full: {same}")
+        inc.close()
+        ref.close()
+
     # per-kernel breakdown (no graph, events between kernels)
     prof = {}
     for _ in range(args.profile_steps):
@@ -248,6 +285,7 @@ def main():
                    "graph": not args.no_graph, "parallelism": "single GPU", "kernel_spectra": "resident",
                    "autotuned_tiles": tiles},
         "ms_per_step_kernel_spectra_recomputed": ms_step_recompute,
+        "incremental": incremental,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <numeric>
 
 #include "common.hpp"
@@ -60,7 +61,9 @@ __global__ void k_set_key(StepDeviceResult* r, unsigned long long key)
 // ORs the chosen kernel into the occupancy (gap_fill.cpp:98-121 for the chosen
 // pose) and adds the newly set cells to the environment mass.
 __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsigned char* __restrict__ kcells,
-                        const OrientDesc* __restrict__ descs, StepDeviceResult* r)
+                        const OrientDesc* __restrict__ descs, StepDeviceResult* r, unsigned char* collect = nullptr,
+                        int cx = 0, int cy = 0, int mx = 0, int my = 0, int mz = 0,
+                        unsigned long long* collect_mass = nullptr)
 {
     const unsigned long long key = r->key;
     if (key == ~0ULL) return;
@@ -78,11 +81,132 @@ __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsign
         if (*c == 0) {
             *c = 1;
             ++added;
+            // newly set cell, at kernel cell + margin in the update window
+            if (collect) collect[(x + mx) + (long long)cx * ((y + my) + (long long)cy * (z + mz))] = 1;
         }
     }
-    if (added) atomicAdd(&r->env_mass, added);
+    if (added) {
+        atomicAdd(&r->env_mass, added);
+        if (collect_mass) atomicAdd(collect_mass, added);
+    }
 }
 
+
+// ---- incremental overlap fields (gap_fill.cpp:426-451,524-584 restated) ----
+// The occupancy only grows, so after new cells N are set every orientation's
+// overlap field grows by the correlation of N with its kernel, and its set of
+// feasible corners only shrinks.  F (int32, exact) is updated by adding that
+// small correlation, computed on an update window around N; an orientation
+// whose lowest feasible key became blocked is rescanned forward from it.
+
+__device__ __forceinline__ unsigned long long pack_key(int ux, int uy, int uz, int o)
+{
+    return ((unsigned long long)uz << 48) | ((unsigned long long)uy << 32) | ((unsigned long long)ux << 16) |
+           (unsigned long long)o;
+}
+
+// U_o is the circular correlation (period n' per axis) of the new cells,
+// which sit at offsets [0, Kp) of the update window, with kernel o; n' >=
+// Kp + K_o - 1, so its shift s stands for the linear shift s when
+// s < n' - (K_o - 1) and s - n' otherwise.  F_o(u_p + shift) += U_o(s) for
+// corners inside the main legal range, u_p the pending placement's corner;
+// flags orientations whose current lowest feasible corner received overlap.
+__device__ __forceinline__ int unwrap(int s, int n, int k) { return s < n - (k - 1) ? s : s - n; }
+
+__global__ void k_apply_update(int* __restrict__ F, const OrientDesc* __restrict__ md, const long long* __restrict__ moff,
+                               const int* __restrict__ U, const OrientDesc* __restrict__ ud,
+                               const long long* __restrict__ uoff, const StepDeviceResult* __restrict__ res, int nxu,
+                               int nyu, int nzu, const unsigned long long* __restrict__ best, int* __restrict__ rescan)
+{
+    const int o = blockIdx.y;
+    const OrientDesc d = md[o], e = ud[o];
+    if (d.lx <= 0 || e.lx <= 0) return;
+    const unsigned long long key = res->key;
+    const int ox = int((key >> 16) & 0xffffULL), oy = int((key >> 32) & 0xffffULL), oz = int(key >> 48);
+    const long long n = (long long)e.lx * e.ly * e.lz;
+    const int* u = U + uoff[o];
+    int* f = F + moff[o];
+    const unsigned long long b = best[o];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int v = u[i];
+        if (v == 0) continue;
+        const int tx = ox + unwrap(int(i % e.lx), nxu, d.kx), ty = oy + unwrap(int((i / e.lx) % e.ly), nyu, d.ky),
+                  tz = oz + unwrap(int(i / ((long long)e.lx * e.ly)), nzu, d.kz);
+        if (tx < 0 || ty < 0 || tz < 0 || tx >= d.lx || ty >= d.ly || tz >= d.lz) continue;
+        f[tx + (long long)d.lx * (ty + (long long)d.ly * tz)] += v;
+        if (pack_key(tx, ty, tz, o) == b) rescan[o] = 1;
+    }
+}
+
+// Marks a rejected corner blocked for good and flags a rescan if it was the minimum.
+__global__ void k_block_corner(int* __restrict__ F, const OrientDesc* __restrict__ md, const long long* __restrict__ moff,
+                               int o, int ux, int uy, int uz, const unsigned long long* __restrict__ best,
+                               int* __restrict__ rescan)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const OrientDesc d = md[o];
+    if (d.lx <= 0 || ux >= d.lx || uy >= d.ly || uz >= d.lz) return;
+    F[moff[o] + ux + (long long)d.lx * (uy + (long long)d.ly * uz)] += 1 << 30;
+    if (pack_key(ux, uy, uz, o) == best[o]) rescan[o] = 1;
+}
+
+// One block per orientation: when flagged, the lowest (z, y, x) corner with
+// F == 0 at or after the old minimum (the feasible set only shrinks).
+__global__ __launch_bounds__(1024) void k_rescan(const int* __restrict__ F, const OrientDesc* __restrict__ md,
+                                                 const long long* __restrict__ moff,
+                                                 unsigned long long* __restrict__ best, int* __restrict__ rescan)
+{
+    __shared__ unsigned long long found;
+    const int o = blockIdx.x;
+    if (!rescan[o]) return;
+    const OrientDesc d = md[o];
+    const unsigned long long b0 = best[o];
+    if (threadIdx.x == 0) found = ~0ULL;
+    __syncthreads();
+    const int* f = F + moff[o];
+    const long long plane = (long long)d.lx * d.ly;
+    for (int z = int(b0 >> 48); z < d.lz; ++z) {
+        unsigned long long mine = ~0ULL;
+        for (long long i = threadIdx.x; i < plane; i += blockDim.x) {
+            if (f[z * plane + i] != 0) continue;
+            const unsigned long long k = pack_key(int(i % d.lx), int(i / d.lx), z, o);
+            if (k >= b0 && k < mine) mine = k;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long om = __shfl_down(mine, off, 64);
+            mine = om < mine ? om : mine;
+        }
+        if ((threadIdx.x & 63) == 0 && mine != ~0ULL) atomicMin(&found, mine);
+        __syncthreads();
+        if (found != ~0ULL) break;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        best[o] = found;
+        rescan[o] = 0;
+    }
+}
+
+// The step's choice from the maintained minima; numeric flags of the update
+// step carried over.
+__global__ void k_choose(StepDeviceResult* __restrict__ res, const unsigned long long* __restrict__ best, int orientations,
+                         const StepDeviceResult* __restrict__ upd, int take_flags)
+{
+    __shared__ unsigned long long m;
+    if (threadIdx.x == 0) m = ~0ULL;
+    __syncthreads();
+    unsigned long long mine = ~0ULL;
+    for (int o = threadIdx.x; o < orientations; o += blockDim.x) mine = best[o] < mine ? best[o] : mine;
+    atomicMin(&m, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        res->key = m;
+        if (take_flags) {
+            res->flags = upd->flags;
+            res->max_residual_bits = upd->max_residual_bits;
+        }
+    }
+}
 
 // ---- register-FFT dispatch for power-of-R lengths -------------------------
 template <typename T, int N, int R> constexpr std::size_t rf_lds(int pencils)
@@ -238,7 +362,7 @@ void rf_c2r_launch(int n, Tile tile, hipStream_t st, int lx_max, int ly_max, int
 
 class Session {
 public:
-    Session(Context& ctx, const sp_session_desc& d) : ctx_(ctx), desc_(d)
+    Session(Context& ctx, const sp_session_desc& d, hipStream_t shared_stream = nullptr) : ctx_(ctx), desc_(d)
     {
         for (int a = 0; a != 3; ++a) {
             ext_[a] = int(d.environment_shape[a]);
@@ -251,7 +375,11 @@ public:
         kernels_.resize(O_);
         descs_.resize(O_);
         use_device(ctx_);
-        SP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        if (shared_stream) stream_ = shared_stream;
+        else {
+            SP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+            owns_stream_ = true;
+        }
         int lds = 0;
         SP_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx_.device));
         max_lds_ = std::size_t(lds);
@@ -270,7 +398,8 @@ public:
     {
         drop_graphs();
         if (host_result_) (void)hipHostFree(host_result_);
-        if (stream_) (void)hipStreamDestroy(stream_);
+        upd_.reset();
+        if (stream_ && owns_stream_) (void)hipStreamDestroy(stream_);
     }
 
     void set_environment(const std::uint8_t* cells, std::uint64_t count)
@@ -287,6 +416,7 @@ public:
         r0.env_mass = mass;
         SP_HIP_CHECK(hipMemcpyAsync(result_.get(), &r0, sizeof r0, hipMemcpyHostToDevice, stream_));
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        inc_ready_ = pending_ = false;
     }
 
     void get_environment(std::uint8_t* cells, std::uint64_t count)
@@ -342,6 +472,7 @@ public:
             subset_ = false;
             return;
         }
+        if (incremental_) throw std::invalid_argument("incremental steps search every orientation");
         std::vector<int> list;
         std::vector<char> seen(O_, 0);
         if (count) {
@@ -414,6 +545,15 @@ public:
         use_device(ctx_);
         SP_HIP_CHECK(hipMemcpyAsync(excl_.get(), buf.data(), buf.size() * sizeof(unsigned long long),
                                     hipMemcpyHostToDevice, stream_));
+        if (incremental_ && inc_ready_ && inc_effective_) {
+            if (pending_) apply_pending();
+            hipLaunchKernelGGL(k_block_corner, dim3(1), dim3(64), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
+                               d_offsets_.as<long long>(), int(o), int((key >> 16) & 0xffff), int((key >> 32) & 0xffff),
+                               int(key >> 48), orient_best_.as<unsigned long long>(), rescan_.as<int>());
+            hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
+                               d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(), rescan_.as<int>());
+            SP_HIP_CHECK(hipGetLastError());
+        }
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
@@ -431,11 +571,130 @@ public:
         }
         use_device(ctx_);
         StepDeviceResult* res = result_.as<StepDeviceResult>();
+        const bool inc = incremental_ && inc_ready_ && inc_effective_;
+        if (inc && pending_) apply_pending();
         hipLaunchKernelGGL(k_set_key, dim3(1), dim3(64), 0, stream_, res, key);
-        hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
-                           ext_[2], kcells_.as<unsigned char>(), d_descs_.as<OrientDesc>(), res);
+        if (inc) enqueue_collecting_stamp();
+        else
+            hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
+                               ext_[2], kcells_.as<unsigned char>(), d_descs_.as<OrientDesc>(), res, nullptr, 0, 0, 0, 0,
+                               0, nullptr);
         SP_HIP_CHECK(hipGetLastError());
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    // ---- incremental mode ------------------------------------------------
+    bool fields_needed() const { return keep_fields_ || incremental_; }
+
+    void set_incremental(bool on)
+    {
+        if (on == incremental_) return;
+        if (on && subset_) throw std::invalid_argument("incremental steps search every orientation");
+        incremental_ = on;
+        prepared_ = false;  // fields buffer and incremental state
+    }
+
+    // Update session: a circular window of n' = good length >= 2 Kmax - 1
+    // cells per axis holding the newly set cells at offsets [0, Kp), whose
+    // fields are every circular shift; used when n'^3 is at most half the
+    // main FFT grid.
+    void inc_setup()
+    {
+        inc_setup_done_ = true;
+        inc_effective_ = false;
+        if (nfit_ == 0) return;
+        sp_session_desc d {};
+        double vol_u = 1.0, vol_m = 1.0;
+        for (int a = 0; a != 3; ++a) {
+            margin_[a] = 0;
+            const int g = good_length(2 * kmax_[a] - 1);
+            if (g == 0) return;
+            vol_u *= g;
+            vol_m *= n_[a];
+            d.environment_shape[a] = std::uint32_t(g);
+            d.environment_first[a] = 0;
+        }
+        if (2.0 * vol_u > vol_m) return;
+        d.orientations = std::uint32_t(O_);
+        d.precision = precision_;
+        d.capture_graph = desc_.capture_graph;
+        upd_ = std::make_unique<Session>(ctx_, d, stream_);
+        upd_->circular_ = true;
+        for (int o = 0; o != O_; ++o) {
+            const Kernel& k = kernels_[o];
+            const std::uint32_t sh[3] = { std::uint32_t(k.shape[0]), std::uint32_t(k.shape[1]), std::uint32_t(k.shape[2]) };
+            const std::int64_t zero[3] = { 0, 0, 0 };
+            upd_->set_kernel(std::uint32_t(o), k.cells.data(), sh, zero);
+        }
+        upd_->keep_fields(true);
+        upd_->prepare();
+        rescan_.allocate(sizeof(int) * O_);
+        SP_HIP_CHECK(hipMemsetAsync(rescan_.get(), 0, sizeof(int) * O_, stream_));
+        inc_effective_ = true;
+    }
+
+    // Stamps the result's key and collects the newly set cells into the
+    // update session's (cleared) occupancy, counting them as its mass.
+    void enqueue_collecting_stamp()
+    {
+        Session& u = *upd_;
+        SP_HIP_CHECK(hipMemsetAsync(u.env_.get(), 0, u.env_.bytes(), stream_));
+        StepDeviceResult* ur = u.result_.as<StepDeviceResult>();
+        SP_HIP_CHECK(hipMemsetAsync(&ur->env_mass, 0, sizeof(ur->env_mass), stream_));
+        hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
+                           ext_[2], kcells_.as<unsigned char>(), d_descs_.as<OrientDesc>(),
+                           result_.as<StepDeviceResult>(), u.env_.as<unsigned char>(), u.ext_[0], u.ext_[1], margin_[0],
+                           margin_[1], margin_[2], &ur->env_mass);
+        pending_ = true;
+    }
+
+    // Adds the correlation of the pending new cells with every kernel into
+    // the fields and rescans orientations whose minimum was blocked.
+    void apply_pending()
+    {
+        Session& u = *upd_;
+        u.launch_mode(false, false);
+        hipLaunchKernelGGL(k_apply_update, dim3(64, O_), dim3(kThreads), 0, stream_, fields_.as<int>(),
+                           d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), u.fields_.as<int>(),
+                           u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(), result_.as<StepDeviceResult>(),
+                           u.n_[0], u.n_[1], u.n_[2], orient_best_.as<unsigned long long>(), rescan_.as<int>());
+        hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
+                           d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(), rescan_.as<int>());
+        SP_HIP_CHECK(hipGetLastError());
+        pending_ = false;
+    }
+
+    void launch_incremental(bool stamp)
+    {
+        prepare();
+        use_device(ctx_);
+        if (!inc_setup_done_) inc_setup();
+        if (!inc_effective_) {
+            launch_mode(stamp, false);
+            return;
+        }
+        bool fresh = false;
+        if (!inc_ready_) {
+            // one full step gives the exact fields and every orientation's minimum
+            launch_mode(false, false);
+            fresh = true;
+            for (unsigned long long k : excluded_) {
+                const int o = int(k & 0xffff);
+                hipLaunchKernelGGL(k_block_corner, dim3(1), dim3(64), 0, stream_, fields_.as<int>(),
+                                   d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), o, int((k >> 16) & 0xffff),
+                                   int((k >> 32) & 0xffff), int(k >> 48), orient_best_.as<unsigned long long>(),
+                                   rescan_.as<int>());
+            }
+            inc_ready_ = true;
+            pending_ = false;
+        }
+        else if (pending_)
+            apply_pending();
+        hipLaunchKernelGGL(k_choose, dim3(1), dim3(256), 0, stream_, result_.as<StepDeviceResult>(),
+                           orient_best_.as<unsigned long long>(), O_, upd_->result_.as<StepDeviceResult>(),
+                           fresh ? 0 : 1);
+        if (stamp) enqueue_collecting_stamp();
+        SP_HIP_CHECK(hipGetLastError());
     }
 
     void keep_fields(bool keep)
@@ -502,6 +761,12 @@ public:
             d.ly = ext_[1] - k.shape[1] + 1;
             d.lz = ext_[2] - k.shape[2] + 1;
             if (d.lx <= 0 || d.ly <= 0 || d.lz <= 0) d.lx = d.ly = d.lz = 0;  // does not fit (gap_fill.cpp:382-390)
+            else if (circular_) {
+                // every circular shift of the FFT grid (window = grid)
+                d.lx = n_[0];
+                d.ly = n_[1];
+                d.lz = n_[2];
+            }
             d.cells_offset = (long long)all.size();
             d.mass = (long long)k.mass;
             all.insert(all.end(), k.cells.begin(), k.cells.end());
@@ -539,7 +804,7 @@ public:
         d_offsets_.allocate(sizeof(long long) * O_);
         SP_HIP_CHECK(hipMemcpyAsync(d_offsets_.get(), field_offsets_.data(), sizeof(long long) * O_,
                                     hipMemcpyHostToDevice, stream_));
-        if (keep_fields_) fields_.allocate(std::max<long long>(foff, 1) * sizeof(int));
+        if (fields_needed()) fields_.allocate(std::max<long long>(foff, 1) * sizeof(int));
         else fields_.release();
         if (keep_raw_) raw_.allocate(std::max<long long>(foff, 1) * sizeof(double));
         else raw_.release();
@@ -560,6 +825,8 @@ public:
         drop_graphs();
         spectra_ready_ = false;
         tuned_ = false;
+        upd_.reset();
+        inc_setup_done_ = inc_ready_ = pending_ = false;
         for (int p = 0; p != kPasses; ++p)
             if (!pinned_[p]) {
                 tile_[p] = default_tile(p, pass_length(p));
@@ -648,7 +915,7 @@ public:
             const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
                                     orient_best_.as<unsigned long long>() };
             rf_c2r_launch<T>(nz, tile, stream_, lmax_[0], lmax_[1], slots, TT, to, nx, ny, dd, inv_total,
-                             result_.as<StepDeviceResult>(), keep_fields_ ? fields_.as<int>() : nullptr,
+                             result_.as<StepDeviceResult>(), fields_needed() ? fields_.as<int>() : nullptr,
                              d_offsets_.as<long long>(), excl_.as<unsigned long long>(),
                              keep_raw_ ? raw_.as<double>() : nullptr, filt, twiddles<T>(nz));
         }
@@ -887,7 +1154,7 @@ public:
                 const int W3 = choose_w(nz, e, 2, true, 32, 2);
                 hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], slots), dim3(kThreads),
                                    lds_bytes<T>(nz, 2, W3 / 2), stream_, TT, to, nx, ny, dd, inv_total, res,
-                                   keep_fields_ ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
+                                   fields_needed() ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
                                    excl_.as<unsigned long long>(), keep_raw_ ? raw_.as<double>() : nullptr, filt, twz,
                                    plans_[2], W3);
             }
@@ -968,7 +1235,11 @@ public:
         SP_HIP_CHECK(hipGraphLaunch(g.exec, stream_));
     }
 
-    void launch(bool stamp) { launch_mode(stamp, subset_); }
+    void launch(bool stamp)
+    {
+        if (incremental_) launch_incremental(stamp);
+        else launch_mode(stamp, subset_);
+    }
 
     void launch_mode(bool stamp, bool subset)
     {
@@ -989,6 +1260,7 @@ public:
     // mask on the device, then the subset search runs; both in one graph.
     void launch_proposed(const void* owner, const std::function<void(hipStream_t)>& propose, int slots, bool stamp)
     {
+        if (incremental_) throw std::invalid_argument("proposal steps need a session without incremental steps");
         prepare();
         use_device(ctx_);
         ensure_kernel_spectra();
@@ -1083,7 +1355,7 @@ public:
 
     void overlap_field(std::uint32_t o, std::int32_t* out, std::uint64_t count, std::uint32_t legal[3])
     {
-        if (!keep_fields_ || !prepared_) throw std::invalid_argument("fields not kept");
+        if (!fields_needed() || !prepared_) throw std::invalid_argument("fields not kept");
         if (int(o) >= O_) throw std::invalid_argument("orientation index");
         const OrientDesc& d = descs_[o];
         legal[0] = std::uint32_t(std::max(d.lx, 0));
@@ -1148,6 +1420,12 @@ private:
     bool use_rf_ { true };
     bool resident_ { true };
     bool spectra_ready_ {};
+    bool owns_stream_ {};
+    bool incremental_ {}, inc_setup_done_ {}, inc_effective_ {}, inc_ready_ {}, pending_ {};
+    bool circular_ {};
+    int margin_[3] {};
+    std::unique_ptr<Session> upd_;
+    DeviceBuffer rescan_;
     bool autotune_ { true }, tuned_ {};
     Tile tile_[kPasses] {};
     float tile_ms_[kPasses] {};
@@ -1311,6 +1589,12 @@ int sp_hip_session_tile_candidates(sp_hip_session* s, int pass, int32_t* pairs, 
             pairs[2 * i + 1] = tiles[i].w;
         }
     });
+}
+
+int sp_hip_session_set_incremental(sp_hip_session* s, int on)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_incremental(on != 0); });
 }
 
 int sp_hip_session_set_resident_kernels(sp_hip_session* s, int on)

@@ -311,6 +311,9 @@ class Session:
         self._check(lib().sp_hip_session_tile_candidates(self._h, C.c_int(pass_), pairs, C.c_uint32(32), C.byref(n)))
         return [(pairs[2 * i], pairs[2 * i + 1]) for i in range(n.value)]
 
+    def set_incremental(self, on=True):
+        self._check(lib().sp_hip_session_set_incremental(self._h, C.c_int(1 if on else 0)))
+
     def set_resident_kernels(self, on=True):
         self._check(lib().sp_hip_session_set_resident_kernels(self._h, C.c_int(1 if on else 0)))
 

@@ -207,6 +207,17 @@ int sp_hip_session_set_autotune(sp_hip_session* s, int on);
 int sp_hip_session_set_tile(sp_hip_session* s, int pass, int radix, int width);
 int sp_hip_session_get_tile(sp_hip_session* s, int pass, int32_t* radix, int32_t* width, float* ms);
 int sp_hip_session_tile_candidates(sp_hip_session* s, int pass, int32_t* pairs, uint32_t capacity, uint32_t* count);
+/* Incremental steps (gap_fill.cpp:426-451,524-584: blocked offsets from
+ * small correlations).  The first step computes every orientation's overlap
+ * field in full; afterwards the fields (int32, exact) are updated by the
+ * correlation of each step's newly set cells with every kernel, computed as
+ * a circular correlation on an FFT grid of at least 2 Kmax - 1 cells per
+ * axis, and each orientation's
+ * lowest feasible corner is rescanned forward only when it became blocked.
+ * Fields, choices and occupancy are identical to full steps.  Used when the
+ * window's FFT grid is at most half the session's; otherwise steps stay
+ * full.  Not combinable with set_active or proposal steps. */
+int sp_hip_session_set_incremental(sp_hip_session* s, int on);
 /* 1 (default): every orientation's kernel spectrum (forward transform up to
  * its y pass; it does not depend on the occupancy) is computed once after the
  * kernels are set and stays resident in HBM; steps start from it.  0: each

@@ -1,0 +1,116 @@
+"""Incremental steps (fields updated by the correlation of each step's new
+cells, minima rescanned only when blocked) against the oracle and against
+full steps: overlap fields, choices and occupancy bit-exact, with accepted,
+rejected and externally stamped proposals."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make_session(ctx, env, first, kernels, precision=32, incremental=True):
+    import spectrapack_amd as sp
+    s = sp.Session(ctx, env.shape[::-1], first, len(kernels), precision=precision)
+    s.set_environment(env)
+    for i, k in enumerate(kernels):
+        s.set_kernel(i, k["cells"], k["shape"], k["first"])
+    s.set_incremental(incremental)
+    return s
+
+
+def random_scene(seed, shape=(48, 44, 40), kmax=7):
+    rs = np.random.default_rng(seed)
+    first = tuple(int(v) for v in rs.integers(-3, 3, 3))
+    env = (rs.uniform(0, 1, shape[::-1]) < 0.03).astype(np.uint8)
+    env[:3] = 1
+    kernels = []
+    for _ in range(6):
+        s = tuple(int(v) for v in rs.integers(3, kmax + 1, 3))
+        kernels.append(dict(first=tuple(int(v) for v in rs.integers(-4, 4, 3)), shape=s,
+                            cells=(rs.uniform(0, 1, s[::-1]) < 0.75).astype(np.uint8)))
+    return rs, first, env, kernels
+
+
+@pytest.mark.parametrize("seed,precision", [(0, 32), (1, 64), (2, 32)])
+def test_incremental_matches_oracle_with_verdicts(ctx, oracle, seed, precision):
+    rs, first, env, kernels = random_scene(seed)
+    s = make_session(ctx, env, first, kernels, precision)
+    excluded = []
+    for step in range(14):
+        print(f"[incremental] seed={seed} fp{precision} step={step}", flush=True)
+        o, t, fields = oracle.place_step(env, first, kernels, direct=True, fields=True, excluded=excluded)
+        r = s.step(stamp=False)
+        assert r["orientation"] == o
+        for i in range(len(kernels)):
+            g = s.overlap_field(i)
+            if fields[i] is None:
+                assert g is None
+                continue
+            # rejected corners are held blocked in the incremental fields
+            want = fields[i].copy()
+            for eo, et in excluded:
+                if eo == i:
+                    u = [int(et[a]) + kernels[i]["first"][a] - first[a] for a in range(3)]
+                    want[u[2], u[1], u[0]] += 1 << 30
+            assert np.array_equal(g, want), (step, i)
+        if o < 0:
+            break
+        assert tuple(r["translation"]) == tuple(t)
+        if step % 4 == 3:
+            excluded.append((o, tuple(int(v) for v in t)))
+            s.exclude(o, t)
+        else:
+            s.stamp(o, t)
+            env = oracle.stamp(env, first, kernels[o], t)
+        assert np.array_equal(s.environment(), env)
+
+
+def test_incremental_equals_full_steps_to_exhaustion(ctx, oracle):
+    from spectrapack_amd import workloads as W
+    dims = (64.0, 64.0, 64.0)
+    first, shape = W.box_window(dims, 1.0)
+    env = oracle.box_container(first, shape, 1.0, dims)
+    kernels = []
+    box = W.cuboid(10.0, 6.0, 4.0)
+    for q in oracle.cube_catalog():
+        f, sh, c = oracle.voxelize_object(box.vertices, box.triangles, q, 1.0)
+        kernels.append(dict(first=f, shape=sh, cells=c))
+    inc = make_session(ctx, env, first, kernels, 32, True)
+    full = make_session(ctx, env, first, kernels, 32, False)
+    steps = 0
+    while True:
+        a = inc.step(stamp=True)
+        b = full.step(stamp=True)
+        assert (a["orientation"], a["translation"]) == (b["orientation"], b["translation"]), steps
+        assert a["numeric_flags"] == 0
+        if a["orientation"] < 0:
+            break
+        steps += 1
+        if steps % 100 == 0:
+            print(f"[incremental] box scene step {steps}", flush=True)
+    print(f"[incremental] box scene packed {steps} parts", flush=True)
+    assert steps >= 400
+    assert np.array_equal(inc.environment(), full.environment())
+
+
+def test_incremental_without_benefit_stays_full(ctx, oracle):
+    rs, first, env, kernels = random_scene(5, shape=(20, 18, 16), kmax=7)
+    s = make_session(ctx, env, first, kernels, 32)
+    for step in range(5):
+        print(f"[incremental] small scene step={step}", flush=True)
+        o, t, _ = oracle.place_step(env, first, kernels, direct=True)
+        r = s.step(stamp=True)
+        assert r["orientation"] == o
+        if o < 0:
+            break
+        assert tuple(r["translation"]) == tuple(t)
+        env = oracle.stamp(env, first, kernels[o], t)
+    assert np.array_equal(s.environment(), env)
+
+
+def test_incremental_refuses_restrictions(ctx):
+    import spectrapack_amd as sp
+    _, first, env, kernels = random_scene(6)
+    s = make_session(ctx, env, first, kernels, 32)
+    with pytest.raises(sp.Failure):
+        s.set_active([0, 1])
