@@ -829,7 +829,7 @@ public:
         inc_setup_done_ = inc_ready_ = pending_ = false;
         for (int p = 0; p != kPasses; ++p)
             if (!pinned_[p]) {
-                tile_[p] = default_tile(p, pass_length(p));
+                tile_[p] = Tile {};
                 tile_ms_[p] = 0.f;
             }
         prepared_ = true;
@@ -921,6 +921,76 @@ public:
         }
     }
 
+    // One generic (LDS Stockham) pass at tile width W (pencils per block).
+    template <typename T> void gen_pass(int pass, int W, const int* act, int slots)
+    {
+        using T2 = cx<T>;
+        const int nx = n_[0], ny = n_[1], nz = n_[2];
+        const int nh = nz / 2 + 1;
+        T2* SB = sb_.as<T2>();
+        T2* H2 = h2_.as<T2>();
+        T2* TT = t_.as<T2>();
+        const OrientDesc* dd = d_descs_.as<OrientDesc>();
+        const long long h2o = (long long)nh * ny * kmax_[0];
+        const long long to = (long long)nh * ny * nx;
+        if (pass == kPassMul) {
+            hipLaunchKernelGGL((k_mul_x<T>), dim3((ny + W - 1) / W, nh, slots), dim3(kThreads), lds_bytes<T>(nx, 4, W),
+                               stream_, H2, h2o, kmax_[0], ny, SB, nx, ny, TT, to, dd, 0, 0, act, twiddles<T>(nx),
+                               plans_[0], W);
+        }
+        else if (pass == kPassInvY) {
+            hipLaunchKernelGGL((k_c2c<T>), dim3((lmax_[0] + W - 1) / W, nh, slots), dim3(kThreads), lds_bytes<T>(ny, 2, W),
+                               stream_, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to, (long long)nx, 1LL,
+                               (long long)nx * ny, 0, ny, 0, dd, 2, act, twiddles<T>(ny), plans_[1], W, 0);
+        }
+        else {
+            const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
+            const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
+                                    orient_best_.as<unsigned long long>() };
+            hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W - 1) / W, lmax_[1], slots), dim3(kThreads),
+                               lds_bytes<T>(nz, 2, W / 2), stream_, TT, to, nx, ny, dd, inv_total,
+                               result_.as<StepDeviceResult>(), fields_needed() ? fields_.as<int>() : nullptr,
+                               d_offsets_.as<long long>(), excl_.as<unsigned long long>(),
+                               keep_raw_ ? raw_.as<double>() : nullptr, filt, twiddles<T>(nz), plans_[2], W);
+        }
+    }
+
+    bool pass_uses_rf(int pass) const { return use_rf_ && rf_supported(pass_length(pass)); }
+
+    // Default generic tile width (the LDS heuristic) of a pass.
+    int generic_default_w(int pass) const
+    {
+        const std::size_t e = precision_ == 64 ? sizeof(double2) : sizeof(float2);
+        if (pass == kPassMul) return choose_w(n_[0], e, 4, false, 16, 1);
+        if (pass == kPassInvY) return choose_w(n_[1], e, 2, false, 16, 1);
+        return choose_w(n_[2], e, 2, true, 32, 2);
+    }
+
+    std::size_t generic_lds(int pass, int W) const
+    {
+        const std::size_t e = precision_ == 64 ? sizeof(double2) : sizeof(float2);
+        const int n = pass_length(pass);
+        if (pass == kPassMul) return e * (std::size_t(n) + 4 * std::size_t(n) * (W + 1));
+        if (pass == kPassInvY) return e * (std::size_t(n) + 2 * std::size_t(n) * (W + 1));
+        return e * (std::size_t(n) + 2 * std::size_t(n) * (W / 2 + 1));
+    }
+
+    std::vector<Tile> generic_candidates(int pass) const
+    {
+        std::vector<Tile> v;
+        for (int w : { 1, 2, 4, 8, 16, 32, 64 }) {
+            if (pass == kPassC2r && w < 2) continue;
+            if (generic_lds(pass, w) <= max_lds_) v.push_back(Tile { 0, w });
+        }
+        return v;
+    }
+
+    template <typename T> void run_pass(int pass, Tile tile, const int* act, int slots)
+    {
+        if (pass_uses_rf(pass)) rf_pass<T>(pass, tile.r ? tile : default_tile(pass, pass_length(pass)), act, slots);
+        else gen_pass<T>(pass, tile.r == 0 && tile.w > 0 ? tile.w : generic_default_w(pass), act, slots);
+    }
+
     int pass_length(int pass) const { return pass == kPassMul ? n_[0] : pass == kPassInvY ? n_[1] : n_[2]; }
 
     bool tile_fits(int pass, Tile t) const
@@ -937,7 +1007,7 @@ public:
     {
         const int nx = n_[0], ny = n_[1], nz = n_[2];
         bool any = false;
-        for (int pass = 0; pass != kPasses; ++pass) any = any || (!pinned_[pass] && rf_supported(pass_length(pass)));
+        for (int pass = 0; pass != kPasses; ++pass) any = any || !pinned_[pass];
         if (!any) return;
         // Tune on real data: one full step without stamping leaves the
         // environment spectrum and the inverse-y output of the current
@@ -953,10 +1023,11 @@ public:
         try {
             for (int pass : { int(kPassC2r), int(kPassMul), int(kPassInvY) }) {
                 const int n = pass_length(pass);
-                if (pinned_[pass] || !rf_supported(n)) continue;
+                if (pinned_[pass]) continue;
+                const bool rf = pass_uses_rf(pass);
                 char key[160];
-                std::snprintf(key, sizeof key, "p%d n%d,%d,%d f%d k%d,%d l%d,%d s%d", pass, nx, ny, nz, precision_,
-                              kmax_[0], kmax_[1], lmax_[0], lmax_[1], nfit_);
+                std::snprintf(key, sizeof key, "p%d%s n%d,%d,%d f%d k%d,%d l%d,%d s%d", pass, rf ? "r" : "g", nx, ny,
+                              nz, precision_, kmax_[0], kmax_[1], lmax_[0], lmax_[1], nfit_);
                 auto it = ctx_.tiles.find(key);
                 if (it != ctx_.tiles.end()) {
                     tile_[pass] = Tile { it->second.first, it->second.second };
@@ -964,13 +1035,13 @@ public:
                     continue;
                 }
                 float best = -1.f;
-                Tile best_tile = default_tile(pass, n);
-                for (const Tile t : tile_candidates(pass, n)) {
-                    if (!tile_fits(pass, t)) continue;
-                    rf_pass<T>(pass, t, act_full_.as<int>(), nfit_);
+                Tile best_tile = rf ? default_tile(pass, n) : Tile { 0, generic_default_w(pass) };
+                for (const Tile t : rf ? tile_candidates(pass, n) : generic_candidates(pass)) {
+                    if (rf && !tile_fits(pass, t)) continue;
+                    run_pass<T>(pass, t, act_full_.as<int>(), nfit_);
                     if (hipGetLastError() != hipSuccess) continue;  // launch refused: not a candidate here
                     SP_HIP_CHECK(hipEventRecord(e0, stream_));
-                    for (int i = 0; i != 3; ++i) rf_pass<T>(pass, t, act_full_.as<int>(), nfit_);
+                    for (int i = 0; i != 3; ++i) run_pass<T>(pass, t, act_full_.as<int>(), nfit_);
                     SP_HIP_CHECK(hipEventRecord(e1, stream_));
                     SP_HIP_CHECK(hipEventSynchronize(e1));
                     SP_HIP_CHECK(hipGetLastError());
@@ -997,7 +1068,7 @@ public:
 
     void autotune()
     {
-        if (!autotune_ || tuned_ || !use_rf_ || nfit_ == 0 || lmax_[0] <= 0) return;
+        if (!autotune_ || tuned_ || nfit_ == 0 || lmax_[0] <= 0) return;
         const bool prof = prof_;
         prof_ = false;
         if (precision_ == 64) autotune_passes<double>();
@@ -1012,8 +1083,15 @@ public:
         tuned_ = false;
         if (!on)
             for (int p = 0; p != kPasses; ++p)
-                if (!pinned_[p]) tile_[p] = default_tile(p, pass_length(p));
+                if (!pinned_[p]) tile_[p] = Tile {};
         drop_graphs();
+    }
+
+    Tile effective_tile(int pass) const
+    {
+        const Tile t = tile_[pass];
+        if (pass_uses_rf(pass)) return t.r ? t : default_tile(pass, pass_length(pass));
+        return Tile { 0, t.r == 0 && t.w > 0 ? t.w : generic_default_w(pass) };
     }
 
     void set_tile(int pass, int r, int w)
@@ -1021,7 +1099,6 @@ public:
         prepare();
         if (pass < 0 || pass >= kPasses) throw std::invalid_argument("pass");
         const int n = pass_length(pass);
-        if (!rf_supported(n)) throw std::invalid_argument("the pass uses the generic FFT at this length");
         if (r == 0 && w == 0) {
             pinned_[pass] = false;
             tile_[pass] = default_tile(pass, n);
@@ -1030,9 +1107,15 @@ public:
             return;
         }
         bool known = false;
-        for (const Tile t : tile_candidates(pass, n)) known = known || (t.r == r && t.w == w);
-        if (!known) throw std::invalid_argument("no such tile");
-        if (!tile_fits(pass, Tile { r, w })) throw std::invalid_argument("tile exceeds the LDS of the device");
+        if (pass_uses_rf(pass)) {
+            for (const Tile t : tile_candidates(pass, n)) known = known || (t.r == r && t.w == w);
+            if (!known) throw std::invalid_argument("no such tile");
+            if (!tile_fits(pass, Tile { r, w })) throw std::invalid_argument("tile exceeds the LDS of the device");
+        }
+        else {
+            for (const Tile t : generic_candidates(pass)) known = known || (t.r == r && t.w == w);
+            if (!known) throw std::invalid_argument("no such tile (generic passes take radix 0)");
+        }
         pinned_[pass] = true;
         tile_[pass] = Tile { r, w };
         tile_ms_[pass] = 0.f;
@@ -1043,8 +1126,9 @@ public:
     {
         prepare();
         if (pass < 0 || pass >= kPasses) throw std::invalid_argument("pass");
-        r = tile_[pass].r;
-        w = tile_[pass].w;
+        const Tile t = effective_tile(pass);
+        r = t.r;
+        w = t.w;
         ms = tile_ms_[pass];
     }
 
@@ -1054,7 +1138,7 @@ public:
         if (pass < 0 || pass >= kPasses) throw std::invalid_argument("pass");
         std::vector<Tile> out;
         const int n = pass_length(pass);
-        if (!rf_supported(n)) return out;
+        if (!pass_uses_rf(pass)) return generic_candidates(pass);
         for (const Tile t : tile_candidates(pass, n))
             if (tile_fits(pass, t)) out.push_back(t);
         return out;
@@ -1120,44 +1204,15 @@ public:
         if (slots > 0 && !resident_) enqueue_kernel_spectra<T>(act, slots);
         if (slots > 0) {
             // P1: finish kernel x transform, product with conj, inverse x
-            if (rf_supported(nx) && use_rf_) {
-                rf_pass<T>(kPassMul, tile_[kPassMul], act, slots);
-            }
-            else {
-                const int W = choose_w(nx, e, 4, false, 16, 1);
-                hipLaunchKernelGGL((k_mul_x<T>), dim3((ny + W - 1) / W, nh, slots), dim3(kThreads),
-                                   lds_bytes<T>(nx, 4, W), stream_, H2, h2o, kmax_[0], ny, SB, nx, ny, TT, to, dd, 0, 0,
-                                   act, twx, plans_[0], W);
-            }
+            run_pass<T>(kPassMul, tile_[kPassMul], act, slots);
             mark("mul_x");
         }
         if (slots > 0 && lmax_[0] > 0) {
             // P2: inverse C2C along y for the legal columns, rows y < ly written
-            if (rf_supported(ny) && use_rf_) {
-                rf_pass<T>(kPassInvY, tile_[kPassInvY], act, slots);
-            }
-            else {
-                const int W = choose_w(ny, e, 2, false, 16, 1);
-                hipLaunchKernelGGL((k_c2c<T>), dim3((lmax_[0] + W - 1) / W, nh, slots), dim3(kThreads),
-                                   lds_bytes<T>(ny, 2, W), stream_, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT,
-                                   to, (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, act, twy, plans_[1], W, 0);
-            }
+            run_pass<T>(kPassInvY, tile_[kPassInvY], act, slots);
             mark("inv_c2c_y");
             // P3: C2R along z + round + checks + feasible + lowest (z, y, x)
-            const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
-            const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
-                                    orient_best_.as<unsigned long long>() };
-            if (rf_supported(nz) && use_rf_) {
-                rf_pass<T>(kPassC2r, tile_[kPassC2r], act, slots);
-            }
-            else {
-                const int W3 = choose_w(nz, e, 2, true, 32, 2);
-                hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W3 - 1) / W3, lmax_[1], slots), dim3(kThreads),
-                                   lds_bytes<T>(nz, 2, W3 / 2), stream_, TT, to, nx, ny, dd, inv_total, res,
-                                   fields_needed() ? fields_.as<int>() : nullptr, d_offsets_.as<long long>(),
-                                   excl_.as<unsigned long long>(), keep_raw_ ? raw_.as<double>() : nullptr, filt, twz,
-                                   plans_[2], W3);
-            }
+            run_pass<T>(kPassC2r, tile_[kPassC2r], act, slots);
             mark("c2r_z_argmin");
         }
         if (stamp) {

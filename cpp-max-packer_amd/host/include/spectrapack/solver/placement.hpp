@@ -49,7 +49,8 @@ struct Proposal {
 
 struct PlacementRequest {
     std::uint64_t capacity {};
-    int precision {};  // 0 auto (a-priori bound), 32 or 64
+    int precision {};         // 0 auto (a-priori bound), 32 or 64
+    bool incremental { true };  // update overlap fields by each placement's new cells (same results)
 };
 
 struct PlacementStats {

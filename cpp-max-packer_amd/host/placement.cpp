@@ -49,6 +49,10 @@ PlacementStats place_parts(const Environment& env, std::span<const OrientedKerne
         stats.outcome = "GAP_FILL_FIELD";
         return stats;
     }
+    if (request.incremental && sp_hip_session_set_incremental(s, 1) != SP_OK) {
+        stats.outcome = "GAP_FILL_CORRELATION";
+        return stats;
+    }
     for (std::uint32_t o = 0; o != kernels.size(); ++o) {
         const auto& k = kernels[o];
         const std::int64_t first[3] = { k.first[0], k.first[1], k.first[2] };

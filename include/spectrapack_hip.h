@@ -194,7 +194,9 @@ int sp_hip_session_set_fast_path(sp_hip_session* s, int on);
  * deviation from FP64 on the same occupancy; requires keep_raw. */
 int sp_hip_session_keep_raw(sp_hip_session* s, int keep);
 int sp_hip_session_raw_values(sp_hip_session* s, uint32_t orientation, double* out, uint64_t count);
-/* Autotuner of the register-FFT passes (256- and 512-point axes).  pass 0:
+/* Autotuner of the step's three main passes.  Register-FFT passes (256- and
+ * 512-point axes) choose (radix, tile width); generic passes (other lengths)
+ * choose the tile width with radix 0.  pass 0:
  * kernel x transform + product + inverse x; 1: inverse y; 2: C2R z + argmin.
  * Each candidate (radix, tile width) whose LDS fits the device is timed on
  * the session's own buffers before the first step (events, 3 launches) and

@@ -32,10 +32,13 @@ def write_scene(path, env, first, pitch, kernels, capacity, reject_every, stop_a
         f.write(f"{capacity} {reject_every} {stop_after}\n")
 
 
-@pytest.mark.parametrize("reject_every,stop_after,capacity", [(0, 0, 1000), (3, 0, 1000), (0, 7, 1000), (2, 0, 5)])
-def test_cpp_placement_loop(tmp_path, oracle, reject_every, stop_after, capacity):
+@pytest.mark.parametrize("dims,reject_every,stop_after,capacity", [
+    ((26.0, 14.0, 12.0), 0, 0, 1000), ((26.0, 14.0, 12.0), 3, 0, 1000), ((26.0, 14.0, 12.0), 0, 7, 1000),
+    ((26.0, 14.0, 12.0), 2, 0, 5),
+    # large enough for incremental steps (update grid 27^3 against a 45^3 grid)
+    ((40.0, 40.0, 40.0), 3, 0, 24)])
+def test_cpp_placement_loop(tmp_path, oracle, dims, reject_every, stop_after, capacity):
     from spectrapack_amd import workloads as W
-    dims = (26.0, 14.0, 12.0)
     first, shape = W.box_window(dims, 1.0)
     env = oracle.box_container(first, shape, 1.0, dims)
     box = W.cuboid(10.0, 6.0, 4.0)
@@ -45,7 +48,8 @@ def test_cpp_placement_loop(tmp_path, oracle, reject_every, stop_after, capacity
         kernels.append(dict(q=q, first=f, shape=s, cells=c))
     path = tmp_path / "scene.txt"
     write_scene(path, env, first, 1.0, kernels, capacity, reject_every, stop_after)
-    print(f"[host] place reject_every={reject_every} stop_after={stop_after} capacity={capacity}", flush=True)
+    print(f"[host] place dims={dims} reject_every={reject_every} stop_after={stop_after} capacity={capacity}",
+          flush=True)
     r = subprocess.run([DRIVER, "place", str(path)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = r.stdout.strip().splitlines()
