@@ -63,7 +63,7 @@ __global__ void k_set_key(StepDeviceResult* r, unsigned long long key)
 __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsigned char* __restrict__ kcells,
                         const OrientDesc* __restrict__ descs, StepDeviceResult* r, unsigned char* collect = nullptr,
                         int cx = 0, int cy = 0, int mx = 0, int my = 0, int mz = 0,
-                        unsigned long long* collect_mass = nullptr)
+                        unsigned long long* collect_mass = nullptr, int shift = 0)
 {
     const unsigned long long key = r->key;
     if (key == ~0ULL) return;
@@ -75,7 +75,7 @@ __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsign
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         if (!kcells[d.cells_offset + i]) continue;
         const int x = int(i % d.kx), y = int((i / d.kx) % d.ky), z = int(i / ((long long)d.kx * d.ky));
-        const int gx = ux + x, gy = uy + y, gz = uz + z;
+        const int gx = ux - shift + x, gy = uy - shift + y, gz = uz - shift + z;
         if (gx < 0 || gy < 0 || gz < 0 || gx >= ex || gy >= ey || gz >= ez) continue;
         unsigned char* c = env + gx + (long long)ex * (gy + (long long)ey * gz);
         if (*c == 0) {
@@ -91,6 +91,46 @@ __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsign
     }
 }
 
+
+// ---- pair clearance (conservative_fields.cpp:189-204, 398-434, 1015-1122) --
+// A placed copy blocks every cell within the clearance stencil of its cells.
+// Each orientation's stamp kernel is its kernel dilated by the stencil, in a
+// window grown by the halo on every side; stamping places it at the corner
+// minus the halo.
+
+// validation_kernel.cpp:3492-3522: an offset d can reach within the clearance
+// unless pitch^2 * sum(max(|d_a| - 1, 0)^2), rounded down, exceeds
+// clearance^2, rounded up.
+bool offset_may_reach(const int d[3], double pitch, double clearance)
+{
+    std::uint64_t sum = 0;
+    for (int a = 0; a != 3; ++a) {
+        const std::uint64_t m = std::uint64_t(d[a] < 0 ? -d[a] : d[a]);
+        const std::uint64_t g = m > 1 ? m - 1 : 0;
+        sum += g * g;
+    }
+    const double p2 = std::nextafter(pitch * pitch, -HUGE_VAL);
+    const double dist = std::nextafter(p2 * static_cast<double>(sum), -HUGE_VAL);
+    const double c2 = std::nextafter(clearance * clearance, HUGE_VAL);
+    return !(dist > c2);
+}
+
+__global__ void k_dilate(const unsigned char* __restrict__ kcells, const OrientDesc* __restrict__ kd,
+                         const OrientDesc* __restrict__ sd, unsigned char* __restrict__ scells, const int* __restrict__ st,
+                         int count, int halo)
+{
+    const int o = blockIdx.y;
+    const OrientDesc k = kd[o], d = sd[o];
+    const long long n = (long long)k.kx * k.ky * k.kz;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (!kcells[k.cells_offset + i]) continue;
+        const int x = int(i % k.kx) + halo, y = int((i / k.kx) % k.ky) + halo, z = int(i / ((long long)k.kx * k.ky)) + halo;
+        for (int j = 0; j < count; ++j) {
+            const int sx = x + st[3 * j], sy = y + st[3 * j + 1], sz = z + st[3 * j + 2];
+            scells[d.cells_offset + sx + (long long)d.kx * (sy + (long long)d.ky * sz)] = 1;
+        }
+    }
+}
 
 // ---- incremental overlap fields (gap_fill.cpp:426-451,524-584 restated) ----
 // The occupancy only grows, so after new cells N are set every orientation's
@@ -116,13 +156,15 @@ __device__ __forceinline__ int unwrap(int s, int n, int k) { return s < n - (k -
 __global__ void k_apply_update(int* __restrict__ F, const OrientDesc* __restrict__ md, const long long* __restrict__ moff,
                                const int* __restrict__ U, const OrientDesc* __restrict__ ud,
                                const long long* __restrict__ uoff, const StepDeviceResult* __restrict__ res, int nxu,
-                               int nyu, int nzu, const unsigned long long* __restrict__ best, int* __restrict__ rescan)
+                               int nyu, int nzu, const unsigned long long* __restrict__ best, int* __restrict__ rescan,
+                               int shift)
 {
     const int o = blockIdx.y;
     const OrientDesc d = md[o], e = ud[o];
     if (d.lx <= 0 || e.lx <= 0) return;
     const unsigned long long key = res->key;
-    const int ox = int((key >> 16) & 0xffffULL), oy = int((key >> 32) & 0xffffULL), oz = int(key >> 48);
+    const int ox = int((key >> 16) & 0xffffULL) - shift, oy = int((key >> 32) & 0xffffULL) - shift,
+              oz = int(key >> 48) - shift;
     const long long n = (long long)e.lx * e.ly * e.lz;
     const int* u = U + uoff[o];
     int* f = F + moff[o];
@@ -575,10 +617,70 @@ public:
         if (inc && pending_) apply_pending();
         hipLaunchKernelGGL(k_set_key, dim3(1), dim3(64), 0, stream_, res, key);
         if (inc) enqueue_collecting_stamp();
-        else
-            hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
-                               ext_[2], kcells_.as<unsigned char>(), d_descs_.as<OrientDesc>(), res, nullptr, 0, 0, 0, 0,
-                               0, nullptr);
+        else enqueue_stamp(nullptr, 0, 0, nullptr);
+        SP_HIP_CHECK(hipGetLastError());
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    // Stamps the result's key: the kernel, or with a pair clearance its
+    // dilated stamp kernel at the corner minus the halo; optionally collects
+    // the newly set cells (at stamp-kernel coordinates) and counts them.
+    void enqueue_stamp(unsigned char* collect, int cx, int cy, unsigned long long* cmass)
+    {
+        const bool dil = halo_ > 0;
+        hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
+                           ext_[2], dil ? scells_.as<unsigned char>() : kcells_.as<unsigned char>(),
+                           dil ? sdescs_.as<OrientDesc>() : d_descs_.as<OrientDesc>(), result_.as<StepDeviceResult>(),
+                           collect, cx, cy, 0, 0, 0, cmass, halo_);
+    }
+
+    // Pair clearance between placed copies (0: none).
+    void set_pair_clearance(double clearance, double pitch)
+    {
+        if (!std::isfinite(clearance) || clearance < 0.0) throw std::invalid_argument("clearance must be finite and >= 0");
+        if (!std::isfinite(pitch) || !(pitch > 0.0)) throw std::invalid_argument("pitch must be finite and > 0");
+        pair_clearance_ = clearance;
+        pitch_ = pitch;
+        prepared_ = false;
+    }
+
+    // Dilated stamp kernels for the pair clearance (after the kernels are uploaded).
+    void prepare_stamp_kernels()
+    {
+        halo_ = 0;
+        if (pair_clearance_ == 0.0) return;
+        const long double r = std::ceil(static_cast<long double>(pair_clearance_) / pitch_) + 1.0L;
+        if (!(r < 4096.0L)) throw std::invalid_argument("clearance halo too large");
+        halo_ = int(r);
+        std::vector<int> st;
+        for (int z = -halo_; z <= halo_; ++z)
+            for (int y = -halo_; y <= halo_; ++y)
+                for (int x = -halo_; x <= halo_; ++x) {
+                    const int d[3] = { x, y, z };
+                    if (offset_may_reach(d, pitch_, pair_clearance_)) {
+                        st.push_back(x);
+                        st.push_back(y);
+                        st.push_back(z);
+                    }
+                }
+        std::vector<OrientDesc> sd(descs_);
+        long long off = 0;
+        for (int o = 0; o != O_; ++o) {
+            sd[o].kx = descs_[o].kx + 2 * halo_;
+            sd[o].ky = descs_[o].ky + 2 * halo_;
+            sd[o].kz = descs_[o].kz + 2 * halo_;
+            sd[o].cells_offset = off;
+            off += (long long)sd[o].kx * sd[o].ky * sd[o].kz;
+        }
+        scells_.allocate(std::size_t(std::max(off, 1LL)));
+        sdescs_.allocate(sizeof(OrientDesc) * O_);
+        stencil_.allocate(sizeof(int) * std::max<std::size_t>(st.size(), 3));
+        SP_HIP_CHECK(hipMemsetAsync(scells_.get(), 0, scells_.bytes(), stream_));
+        SP_HIP_CHECK(hipMemcpyAsync(sdescs_.get(), sd.data(), sizeof(OrientDesc) * O_, hipMemcpyHostToDevice, stream_));
+        SP_HIP_CHECK(hipMemcpyAsync(stencil_.get(), st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice, stream_));
+        hipLaunchKernelGGL(k_dilate, dim3(64, O_), dim3(kThreads), 0, stream_, kcells_.as<unsigned char>(),
+                           d_descs_.as<OrientDesc>(), sdescs_.as<OrientDesc>(), scells_.as<unsigned char>(),
+                           stencil_.as<int>(), int(st.size() / 3), halo_);
         SP_HIP_CHECK(hipGetLastError());
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
     }
@@ -607,7 +709,7 @@ public:
         double vol_u = 1.0, vol_m = 1.0;
         for (int a = 0; a != 3; ++a) {
             margin_[a] = 0;
-            const int g = good_length(2 * kmax_[a] - 1);
+            const int g = good_length(2 * kmax_[a] + 2 * halo_ - 1);
             if (g == 0) return;
             vol_u *= g;
             vol_m *= n_[a];
@@ -641,10 +743,7 @@ public:
         SP_HIP_CHECK(hipMemsetAsync(u.env_.get(), 0, u.env_.bytes(), stream_));
         StepDeviceResult* ur = u.result_.as<StepDeviceResult>();
         SP_HIP_CHECK(hipMemsetAsync(&ur->env_mass, 0, sizeof(ur->env_mass), stream_));
-        hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
-                           ext_[2], kcells_.as<unsigned char>(), d_descs_.as<OrientDesc>(),
-                           result_.as<StepDeviceResult>(), u.env_.as<unsigned char>(), u.ext_[0], u.ext_[1], margin_[0],
-                           margin_[1], margin_[2], &ur->env_mass);
+        enqueue_stamp(u.env_.as<unsigned char>(), u.ext_[0], u.ext_[1], &ur->env_mass);
         pending_ = true;
     }
 
@@ -657,7 +756,7 @@ public:
         hipLaunchKernelGGL(k_apply_update, dim3(64, O_), dim3(kThreads), 0, stream_, fields_.as<int>(),
                            d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), u.fields_.as<int>(),
                            u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(), result_.as<StepDeviceResult>(),
-                           u.n_[0], u.n_[1], u.n_[2], orient_best_.as<unsigned long long>(), rescan_.as<int>());
+                           u.n_[0], u.n_[1], u.n_[2], orient_best_.as<unsigned long long>(), rescan_.as<int>(), halo_);
         hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
                            d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(), rescan_.as<int>());
         SP_HIP_CHECK(hipGetLastError());
@@ -787,6 +886,7 @@ public:
         SP_HIP_CHECK(hipMemcpyAsync(kcells_.get(), all.data(), all.size(), hipMemcpyHostToDevice, stream_));
         d_descs_.allocate(sizeof(OrientDesc) * O_);
         SP_HIP_CHECK(hipMemcpyAsync(d_descs_.get(), descs_.data(), sizeof(OrientDesc) * O_, hipMemcpyHostToDevice, stream_));
+        prepare_stamp_kernels();
         // active lists: [count, regions on, regions x, y, z, orientation per slot]
         std::vector<int> fit { 0, 0, 1, 1, 1 };
         for (int o = 0; o != O_; ++o)
@@ -1216,8 +1316,7 @@ public:
             mark("c2r_z_argmin");
         }
         if (stamp) {
-            hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
-                               ext_[2], kcells_.as<unsigned char>(), dd, res);
+            enqueue_stamp(nullptr, 0, 0, nullptr);
             mark("stamp");
         }
     }
@@ -1478,6 +1577,9 @@ private:
     bool owns_stream_ {};
     bool incremental_ {}, inc_setup_done_ {}, inc_effective_ {}, inc_ready_ {}, pending_ {};
     bool circular_ {};
+    double pair_clearance_ {}, pitch_ { 1.0 };
+    int halo_ {};
+    DeviceBuffer scells_, sdescs_, stencil_;
     int margin_[3] {};
     std::unique_ptr<Session> upd_;
     DeviceBuffer rescan_;
@@ -1644,6 +1746,12 @@ int sp_hip_session_tile_candidates(sp_hip_session* s, int pass, int32_t* pairs, 
             pairs[2 * i + 1] = tiles[i].w;
         }
     });
+}
+
+int sp_hip_session_set_pair_clearance(sp_hip_session* s, double clearance_mm, double pitch_mm)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_pair_clearance(clearance_mm, pitch_mm); });
 }
 
 int sp_hip_session_set_incremental(sp_hip_session* s, int on)

@@ -51,6 +51,7 @@ struct PlacementRequest {
     std::uint64_t capacity {};
     int precision {};         // 0 auto (a-priori bound), 32 or 64
     bool incremental { true };  // update overlap fields by each placement's new cells (same results)
+    double pair_clearance_mm {};  // copies block cells within this distance (constraints().pair_clearance_mm)
 };
 
 struct PlacementStats {

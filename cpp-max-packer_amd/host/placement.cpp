@@ -49,6 +49,11 @@ PlacementStats place_parts(const Environment& env, std::span<const OrientedKerne
         stats.outcome = "GAP_FILL_FIELD";
         return stats;
     }
+    if (request.pair_clearance_mm != 0.0 &&
+        sp_hip_session_set_pair_clearance(s, request.pair_clearance_mm, env.pitch_mm) != SP_OK) {
+        stats.outcome = "GAP_FILL_FIELD";
+        return stats;
+    }
     if (request.incremental && sp_hip_session_set_incremental(s, 1) != SP_OK) {
         stats.outcome = "GAP_FILL_CORRELATION";
         return stats;

@@ -311,6 +311,9 @@ class Session:
         self._check(lib().sp_hip_session_tile_candidates(self._h, C.c_int(pass_), pairs, C.c_uint32(32), C.byref(n)))
         return [(pairs[2 * i], pairs[2 * i + 1]) for i in range(n.value)]
 
+    def set_pair_clearance(self, clearance_mm, pitch_mm):
+        self._check(lib().sp_hip_session_set_pair_clearance(self._h, C.c_double(clearance_mm), C.c_double(pitch_mm)))
+
     def set_incremental(self, on=True):
         self._check(lib().sp_hip_session_set_incremental(self._h, C.c_int(1 if on else 0)))
 

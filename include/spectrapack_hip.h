@@ -209,6 +209,12 @@ int sp_hip_session_set_autotune(sp_hip_session* s, int on);
 int sp_hip_session_set_tile(sp_hip_session* s, int pass, int radix, int width);
 int sp_hip_session_get_tile(sp_hip_session* s, int pass, int32_t* radix, int32_t* width, float* ms);
 int sp_hip_session_tile_candidates(sp_hip_session* s, int pass, int32_t* pairs, uint32_t capacity, uint32_t* count);
+/* Pair clearance between placed copies (conservative_fields.cpp:189-204,
+ * 398-434, 1015-1122): a stamped copy also blocks every cell within the
+ * clearance stencil of its cells (offsets within halo = ceil(c / pitch) + 1
+ * cells that validation_kernel.cpp:3492-3522 does not rule out).  0 (the
+ * default) stamps the kernel's own cells.  Takes effect at the next step. */
+int sp_hip_session_set_pair_clearance(sp_hip_session* s, double clearance_mm, double pitch_mm);
 /* Incremental steps (gap_fill.cpp:426-451,524-584: blocked offsets from
  * small correlations).  The first step computes every orientation's overlap
  * field in full; afterwards the fields (int32, exact) are updated by the

@@ -3,6 +3,8 @@
 #include <cstring>
 #include <thread>
 
+#include <cmath>
+
 #include "sp_oracle.hpp"
 
 using namespace sporacle;
@@ -300,6 +302,30 @@ void orc_stamp(std::uint8_t* env, const std::int64_t* ef, const std::uint32_t* e
     k.cells.assign(kcells, kcells + std::size_t(ks[0]) * ks[1] * ks[2]);
     stamp(e, { ef[0], ef[1], ef[2] }, { es[0], es[1], es[2] }, k, { t[0], t[1], t[2] });
     std::memcpy(env, e.data(), e.size());
+}
+
+// The pair-clearance stencil (conservative_fields.cpp:189-204 halo,
+// 398-434 make_stencil): offsets within the halo that euclidean_offset_reaches
+// does not rule out.  out holds up to capacity (x, y, z) triples; returns the
+// count (or -1 for an invalid clearance).
+int orc_clearance_stencil(double pitch, double clearance, std::int64_t* out, std::uint32_t capacity)
+{
+    if (!std::isfinite(clearance) || clearance < 0.0) return -1;
+    std::int64_t halo = 0;
+    if (clearance != 0.0) halo = static_cast<std::int64_t>(std::ceil(static_cast<long double>(clearance) / pitch) + 1.0L);
+    int count = 0;
+    for (std::int64_t z = -halo; z <= halo; ++z)
+        for (std::int64_t y = -halo; y <= halo; ++y)
+            for (std::int64_t x = -halo; x <= halo; ++x)
+                if (sporacle::euclidean_offset_reaches({ x, y, z }, pitch, clearance) != sporacle::Decision::no) {
+                    if (std::uint32_t(count) < capacity) {
+                        out[3 * count] = x;
+                        out[3 * count + 1] = y;
+                        out[3 * count + 2] = z;
+                    }
+                    ++count;
+                }
+    return count;
 }
 
 void orc_proximity(const std::uint8_t* occ, const std::uint32_t* shape, double* values, std::uint64_t* squared)

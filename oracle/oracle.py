@@ -285,7 +285,33 @@ def correlate_orientations(env, env_first, kernels, begin, end, threads=1):
     return done, chk.value
 
 
-def stamp(env, env_first, kernel, t):
+def clearance_stencil(pitch, clearance):
+    """Pair-clearance stencil offsets (x, y, z) as an int64 [n, 3] array
+    (conservative_fields.cpp:189-204, 398-434; validation_kernel.cpp:3492-3522)."""
+    cap = 1 << 20
+    out = np.zeros((cap, 3), np.int64)
+    n = lib().orc_clearance_stencil(C.c_double(pitch), C.c_double(clearance), _p(out, C.c_int64), C.c_uint32(cap))
+    if n < 0:
+        raise Failure("FIELD_INPUT", {})
+    return out[:n].copy()
+
+
+def stamp(env, env_first, kernel, t, stencil=None):
+    """Sets the kernel's cells at translation t; with a clearance stencil,
+    every cell within the stencil of a kernel cell (a placed copy grown by
+    the pair clearance, conservative_fields.cpp:1015-1122)."""
+    if stencil is not None and len(stencil):
+        env = np.ascontiguousarray(env, dtype=np.uint8).copy()
+        ez, ey, ex = env.shape
+        z, y, x = np.nonzero(np.asarray(kernel["cells"]) != 0)
+        base = np.stack([x, y, z], axis=1) + (np.asarray(t, np.int64) + np.asarray(kernel["first"], np.int64) -
+                                              np.asarray(env_first, np.int64))
+        for d in np.asarray(stencil, np.int64):
+            p = base + d
+            ok = (p >= 0).all(axis=1) & (p[:, 0] < ex) & (p[:, 1] < ey) & (p[:, 2] < ez)
+            p = p[ok]
+            env[p[:, 2], p[:, 1], p[:, 0]] = 1
+        return env
     env = np.ascontiguousarray(env, dtype=np.uint8).copy()
     es = _u32(env.shape[::-1])
     ef = _i64(env_first)

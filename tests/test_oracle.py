@@ -372,3 +372,24 @@ def test_step_rule_brute_force(oracle):
         assert o == -1
     else:
         assert o == best[1] and tuple(t) == best[2]
+
+
+def test_clearance_stencil_closed_form(oracle):
+    """Pair-clearance stencil: an offset is kept unless pitch^2 * sum(max(|d|-1, 0)^2)
+    exceeds clearance^2 (with the reference's outward rounding); zero clearance
+    keeps only the cell itself."""
+    assert oracle.clearance_stencil(1.0, 0.0).tolist() == [[0, 0, 0]]
+    for pitch, clearance in ((1.0, 1.5), (0.5, 1.0), (0.7, 2.1), (1.0, 2.0)):
+        got = {tuple(v) for v in oracle.clearance_stencil(pitch, clearance).tolist()}
+        halo = int(np.ceil(clearance / pitch)) + 1
+        want = set()
+        for z in range(-halo, halo + 1):
+            for y in range(-halo, halo + 1):
+                for x in range(-halo, halo + 1):
+                    g2 = sum(max(abs(v) - 1, 0) ** 2 for v in (x, y, z))
+                    if pitch * pitch * g2 <= clearance * clearance * (1 + 1e-12):
+                        want.add((x, y, z))
+        assert got == want, (pitch, clearance)
+    # clearance exactly on a gap boundary stays inclusive (outward rounding)
+    assert (2, 0, 0) in {tuple(v) for v in oracle.clearance_stencil(1.0, 1.0).tolist()}
+    assert (3, 0, 0) not in {tuple(v) for v in oracle.clearance_stencil(1.0, 1.0).tolist()}
