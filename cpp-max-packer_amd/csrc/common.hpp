@@ -101,6 +101,30 @@ struct Context {
 // Apply the context's device for the current host thread.
 void use_device(const Context& c);
 
+// validation_kernel.cpp:3492-3522: an offset d (cells) can reach within the
+// clearance unless pitch^2 * sum(max(|d_a| - 1, 0)^2), rounded down, exceeds
+// clearance^2, rounded up.
+inline bool offset_may_reach(const int d[3], double pitch, double clearance)
+{
+    std::uint64_t sum = 0;
+    for (int a = 0; a != 3; ++a) {
+        const std::uint64_t m = std::uint64_t(d[a] < 0 ? -d[a] : d[a]);
+        const std::uint64_t g = m > 1 ? m - 1 : 0;
+        sum += g * g;
+    }
+    const double p2 = std::nextafter(pitch * pitch, -HUGE_VAL);
+    const double dist = std::nextafter(p2 * static_cast<double>(sum), -HUGE_VAL);
+    const double c2 = std::nextafter(clearance * clearance, HUGE_VAL);
+    return !(dist > c2);
+}
+
+// Clearance halo in cells (conservative_fields.cpp:189-204): 0 for no clearance.
+inline int clearance_halo_cells(double clearance, double pitch)
+{
+    if (clearance == 0.0) return 0;
+    return int(std::ceil(static_cast<long double>(clearance) / pitch) + 1.0L);
+}
+
 // A-priori FP32 bound (see the header).
 double fp32_error_bound(std::uint64_t fft_cells, std::uint64_t env_mass, std::uint64_t kernel_mass);
 

@@ -740,6 +740,24 @@ __global__ void k_assign(const Orient* ors, unsigned char* tags, const unsigned 
     tags[o.offset + i] = t;
 }
 
+// Wall clearance of a mesh container (cf:436-504 with invert_material): an
+// output cell is blocked when some stencil offset d finds a blocked cell at
+// (t - d) of the work window, which is the requested window grown by halo.
+__global__ void k_dilate_blocked(const unsigned char* __restrict__ work, int wx, int wy, unsigned char* __restrict__ out,
+                                 int sx, int sy, int sz, const int* __restrict__ st, int count, int halo)
+{
+    const long long n = (long long)sx * sy * sz;
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = int(i % sx) + halo, y = int((i / sx) % sy) + halo, z = int(i / ((long long)sx * sy)) + halo;
+    unsigned char v = 0;
+    for (int j = 0; j < count && !v; ++j) {
+        const int px = x - st[3 * j], py = y - st[3 * j + 1], pz = z - st[3 * j + 2];
+        v = work[px + (long long)wx * (py + (long long)wy * pz)];
+    }
+    out[i] = v;
+}
+
 // Analytic box container (vk:3457-3490 per cell, cf:1146-1198).
 __global__ void k_box_container(long long f0, long long f1, long long f2, int sx, int sy, int sz, double o0, double o1,
                                 double o2, double pitch, double d0, double d1, double d2, double clearance,
@@ -1089,8 +1107,8 @@ int sp_hip_voxelize_mesh_container(sp_hip_ctx* ctx, const double* vertices, uint
     if (!ctx || !first || !shape || !origin || !cells) return SP_INVALID_ARGUMENT;
     if (int rc = validate_mesh(vertices, vertex_count, triangles, triangle_count)) return rc;
     if (!(pitch > 0) || !std::isfinite(pitch) || !std::isfinite(clearance) || clearance < 0) return SP_FIELD_INPUT;
-    // Only zero wall clearance is built on the device (halo 0, identity stencil).
-    if (clearance != 0.0) return SP_INVALID_ARGUMENT;
+    if (static_cast<long double>(clearance) / pitch > 4096.0L) return SP_FIELD_INDEX_OVERFLOW;
+    const int halo = clearance_halo_cells(clearance, pitch);
     auto& C = ctx->c;
     try {
         use_device(C);
@@ -1116,16 +1134,40 @@ int sp_hip_voxelize_mesh_container(sp_hip_ctx* ctx, const double* vertices, uint
         hipLaunchKernelGGL(k_place, dim3(1), dim3(256), 0, C.stream, mh, dors.as<Orient>(), C.scratch[2].as<double>());
         SP_HIP_CHECK(hipMemcpyAsync(ors.data(), dors.get(), sizeof(Orient), hipMemcpyDeviceToHost, C.stream));
         SP_HIP_CHECK(hipStreamSynchronize(C.stream));
+        // the work window: the requested one grown by the clearance halo
         for (int a = 0; a != 3; ++a) {
-            o.first[a] = first[a];
-            o.shape[a] = int(shape[a]);
+            o.first[a] = first[a] - halo;
+            o.shape[a] = int(shape[a]) + 2 * halo;
         }
         o.offset = 0;
         const long long total = (long long)shape[0] * shape[1] * shape[2];
-        if (total > 0xffffffffLL) return SP_FIELD_INDEX_OVERFLOW;
+        const long long work = (long long)o.shape[0] * o.shape[1] * o.shape[2];
+        if (work > 0xffffffffLL) return SP_FIELD_INDEX_OVERFLOW;
         DeviceBuffer tags;
-        run_fields(C, mh, dors, ors, origin, pitch, 1, tags, total, int(total), nullptr);
-        SP_HIP_CHECK(hipMemcpyAsync(cells, tags.get(), std::size_t(total), hipMemcpyDeviceToHost, C.stream));
+        run_fields(C, mh, dors, ors, origin, pitch, 1, tags, work, int(work), nullptr);
+        if (halo == 0) {
+            SP_HIP_CHECK(hipMemcpyAsync(cells, tags.get(), std::size_t(total), hipMemcpyDeviceToHost, C.stream));
+            SP_HIP_CHECK(hipStreamSynchronize(C.stream));
+            return SP_OK;
+        }
+        std::vector<int> st;
+        for (int z = -halo; z <= halo; ++z)
+            for (int y = -halo; y <= halo; ++y)
+                for (int x = -halo; x <= halo; ++x) {
+                    const int d[3] = { x, y, z };
+                    if (offset_may_reach(d, pitch, clearance)) {
+                        st.push_back(x);
+                        st.push_back(y);
+                        st.push_back(z);
+                    }
+                }
+        DeviceBuffer dst { sizeof(int) * st.size() }, out { std::size_t(total) };
+        SP_HIP_CHECK(hipMemcpyAsync(dst.get(), st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice, C.stream));
+        hipLaunchKernelGGL(k_dilate_blocked, dim3(unsigned((total + 255) / 256)), dim3(256), 0, C.stream,
+                           tags.as<unsigned char>(), o.shape[0], o.shape[1], out.as<unsigned char>(), int(shape[0]),
+                           int(shape[1]), int(shape[2]), dst.as<int>(), int(st.size() / 3), halo);
+        SP_HIP_CHECK(hipGetLastError());
+        SP_HIP_CHECK(hipMemcpyAsync(cells, out.get(), std::size_t(total), hipMemcpyDeviceToHost, C.stream));
         SP_HIP_CHECK(hipStreamSynchronize(C.stream));
         return SP_OK;
     }

@@ -98,23 +98,6 @@ __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsign
 // window grown by the halo on every side; stamping places it at the corner
 // minus the halo.
 
-// validation_kernel.cpp:3492-3522: an offset d can reach within the clearance
-// unless pitch^2 * sum(max(|d_a| - 1, 0)^2), rounded down, exceeds
-// clearance^2, rounded up.
-bool offset_may_reach(const int d[3], double pitch, double clearance)
-{
-    std::uint64_t sum = 0;
-    for (int a = 0; a != 3; ++a) {
-        const std::uint64_t m = std::uint64_t(d[a] < 0 ? -d[a] : d[a]);
-        const std::uint64_t g = m > 1 ? m - 1 : 0;
-        sum += g * g;
-    }
-    const double p2 = std::nextafter(pitch * pitch, -HUGE_VAL);
-    const double dist = std::nextafter(p2 * static_cast<double>(sum), -HUGE_VAL);
-    const double c2 = std::nextafter(clearance * clearance, HUGE_VAL);
-    return !(dist > c2);
-}
-
 __global__ void k_dilate(const unsigned char* __restrict__ kcells, const OrientDesc* __restrict__ kd,
                          const OrientDesc* __restrict__ sd, unsigned char* __restrict__ scells, const int* __restrict__ st,
                          int count, int halo)

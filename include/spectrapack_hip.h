@@ -267,8 +267,11 @@ int sp_hip_voxelize_objects(sp_hip_ctx* ctx, const double* vertices, uint32_t ve
  * 1 = blocked, 0 = certified inside the box shrunk by the clearance. */
 int sp_hip_voxelize_box_container(sp_hip_ctx* ctx, const int64_t first[3], const uint32_t shape[3], const double origin[3],
                                   double pitch, const double dims[3], double clearance, uint8_t* cells);
-/* voxelize_container, closed mesh (conservative_fields.cpp:1200-1275) at zero
- * wall clearance: only cells certified inside the solid are free (0). */
+/* voxelize_container, closed mesh (conservative_fields.cpp:1200-1275): only
+ * cells certified inside the solid are free (0), and with a wall clearance
+ * every cell within the clearance stencil of a cell that is not certified
+ * inside is blocked too (raster and fill over the window grown by the halo,
+ * cf:189-204 and 436-504). */
 int sp_hip_voxelize_mesh_container(sp_hip_ctx* ctx, const double* vertices, uint32_t vertex_count,
                                    const uint32_t* triangles, uint32_t triangle_count, const int64_t first[3],
                                    const uint32_t shape[3], const double origin[3], double pitch, double clearance,

@@ -63,3 +63,17 @@ def test_sphere_container(ctx, oracle):
         g = ctx.mesh_container(sph.vertices, sph.triangles, first, shape, pitch)
         c = oracle.mesh_container(sph.vertices, sph.triangles, first, shape, pitch)
         assert np.array_equal(g, c)
+
+
+@pytest.mark.parametrize("clearance,pitch", [(0.5, 1.0), (2.0, 1.0), (1.3, 0.7)])
+def test_mesh_container_wall_clearance(ctx, oracle, clearance, pitch):
+    from spectrapack_amd import workloads as W
+    print(f"[raster] sphere container clearance={clearance} pitch={pitch}", flush=True)
+    sph = W.sphere(30.0, n_seg=32, n_lat=15)
+    first, shape = W.mesh_window(sph, pitch)
+    g = ctx.mesh_container(sph.vertices, sph.triangles, first, shape, pitch, clearance=clearance)
+    c = oracle.mesh_container(sph.vertices, sph.triangles, first, shape, pitch, clearance=clearance)
+    assert np.array_equal(g, c)
+    base = ctx.mesh_container(sph.vertices, sph.triangles, first, shape, pitch)
+    # clearance only blocks more: free cells are a subset, and some were taken away
+    assert ((g == 0) <= (base == 0)).all() and (g != base).any()
