@@ -11,9 +11,11 @@
 #include <functional>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 
 #include "common.hpp"
 #include "fft_kernels.hpp"
+#include "rank_kernels.hpp"
 #include "rf_kernels.hpp"
 #include "session.hpp"
 
@@ -442,6 +444,7 @@ public:
         SP_HIP_CHECK(hipMemcpyAsync(result_.get(), &r0, sizeof r0, hipMemcpyHostToDevice, stream_));
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
         inc_ready_ = pending_ = false;
+        ++env_rev_;
     }
 
     void get_environment(std::uint8_t* cells, std::uint64_t count)
@@ -598,6 +601,7 @@ public:
         StepDeviceResult* res = result_.as<StepDeviceResult>();
         const bool inc = incremental_ && inc_ready_ && inc_effective_;
         if (inc && pending_) apply_pending();
+        ++env_rev_;
         hipLaunchKernelGGL(k_set_key, dim3(1), dim3(64), 0, stream_, res, key);
         if (inc) enqueue_collecting_stamp();
         else enqueue_stamp(nullptr, 0, 0, nullptr);
@@ -748,6 +752,7 @@ public:
 
     void launch_incremental(bool stamp)
     {
+        if (stamp) ++env_rev_;
         prepare();
         use_device(ctx_);
         if (!inc_setup_done_) inc_setup();
@@ -909,6 +914,7 @@ public:
         spectra_ready_ = false;
         tuned_ = false;
         upd_.reset();
+        prox_.reset();
         inc_setup_done_ = inc_ready_ = pending_ = false;
         for (int p = 0; p != kPasses; ++p)
             if (!pinned_[p]) {
@@ -1227,6 +1233,21 @@ public:
         return out;
     }
 
+    template <typename T, typename In> void enqueue_env_r2c(const In* src, cx<T>* SB, const cx<T>* twz)
+    {
+        const int nx = n_[0], ny = n_[1], nz = n_[2];
+        if (rf_supported(nz) && use_rf_) {
+            rf_r2c_launch<T, In>(nz, stream_, nx, ny, 1, src, ext_[0], ext_[1], ext_[0], ext_[1], ext_[2], SB, 0LL, nx,
+                                 ny, nx, ny, nullptr, nullptr, twz);
+        }
+        else {
+            const int W = choose_w(nz, sizeof(cx<T>), 2, true, 32, 2);
+            hipLaunchKernelGGL((k_r2c_z<T, In>), dim3((nx + W - 1) / W, ny, 1), dim3(kThreads),
+                               lds_bytes<T>(nz, 2, W / 2), stream_, src, 0LL, ext_[0], ext_[1], ext_[0], ext_[1], ext_[2],
+                               SB, 0LL, nx, ny, nx, ny, nullptr, nullptr, twz, plans_[2], W);
+        }
+    }
+
     template <typename T> void enqueue_step(bool stamp, const int* act, int slots)
     {
         using T2 = cx<T>;
@@ -1244,19 +1265,15 @@ public:
         mark("start");
         hipLaunchKernelGGL(k_reset_result, dim3(1), dim3(256), 0, stream_, res, orient_best_.as<unsigned long long>(), O_);
         mark("reset");
-        // E1: environment R2C along z (uint8 -> spectrum planes)
-        if (rf_supported(nz) && use_rf_) {
-            rf_r2c_launch<T, unsigned char>(nz, stream_, nx, ny, 1, env_.as<unsigned char>(), ext_[0], ext_[1], ext_[0],
-                                            ext_[1], ext_[2], SB, 0LL, nx, ny, nx, ny, nullptr, nullptr, twz);
-            mark("env_r2c_z");
+        // E1: environment R2C along z (uint8 occupancy, or the real-valued
+        // field of a proximity session -> spectrum planes)
+        if (real_env_) {
+            if constexpr (std::is_same_v<T, double>) enqueue_env_r2c<T, double>(envd_.as<double>(), SB, twz);
+            else throw std::logic_error("real-valued environments run in FP64");
         }
-        else {
-            const int W = choose_w(nz, e, 2, true, 32, 2);
-            hipLaunchKernelGGL((k_r2c_z<T, unsigned char>), dim3((nx + W - 1) / W, ny, 1), dim3(kThreads),
-                               lds_bytes<T>(nz, 2, W / 2), stream_, env_.as<unsigned char>(), 0LL, ext_[0], ext_[1],
-                               ext_[0], ext_[1], ext_[2], SB, 0LL, nx, ny, nx, ny, nullptr, nullptr, twz, plans_[2], W);
-            mark("env_r2c_z");
-        }
+        else
+            enqueue_env_r2c<T, unsigned char>(env_.as<unsigned char>(), SB, twz);
+        mark("env_r2c_z");
         // E2: C2C along y (columns x, batch kz)
         if (rf_supported(ny) && use_rf_) {
             rf_cols_launch<T>(ny, default_tile(kPassInvY, ny), stream_, nx, nh, 1, SB, 0LL, (long long)nx, 1LL, (long long)nx * ny, SB, 0LL,
@@ -1380,6 +1397,7 @@ public:
 
     void launch_mode(bool stamp, bool subset)
     {
+        if (stamp) ++env_rev_;
         prepare();
         use_device(ctx_);
         ensure_kernel_spectra();
@@ -1403,6 +1421,7 @@ public:
         ensure_kernel_spectra();
         autotune();
         if (slots <= 0 || slots > O_) throw std::invalid_argument("proposal slots");
+        if (stamp) ++env_rev_;
         auto record = [&] {
             propose(stream_);
             if (precision_ == 64) enqueue_step<double>(stamp, act_sub_.as<int>(), slots);
@@ -1507,6 +1526,317 @@ public:
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
+    // ---- ranked pages (spectral_pipeline.cpp:1343,1563,1740-1857) ---------
+    // Score constants and each orientation's oriented-bounds max z.  Ranking
+    // keeps the binary overlap of every legal corner (fields) and adds an FP64
+    // proximity session sharing this stream.
+    void set_ranking(const sp_ranking_desc& r, const double* max_z, std::uint32_t count)
+    {
+        if (count != std::uint32_t(O_)) throw std::invalid_argument("one oriented max z per orientation");
+        const double height = r.container_max_z_mm - r.container_min_z_mm;
+        if (!std::isfinite(r.lattice_origin_z_mm) || !std::isfinite(r.pitch_mm) || !(r.pitch_mm > 0.0) ||
+            !std::isfinite(r.container_min_z_mm) || !std::isfinite(height) || !(height > 0.0))
+            throw std::domain_error("SPECTRAL_PIPELINE_CONTAINER_HEIGHT");
+        for (std::uint32_t o = 0; o != count; ++o)
+            if (!std::isfinite(max_z[o])) throw std::invalid_argument("oriented max z must be finite");
+        for (int a = 0; a != 3; ++a)
+            if (ext_[a] > kMaxRankExtent) throw std::invalid_argument("ranking needs at most 1024 cells per axis");
+        rank_desc_ = r;
+        rank_max_z_.assign(max_z, max_z + count);
+        rank_on_ = true;
+        keep_fields(true);
+        prox_.reset();
+    }
+
+    // Binary overlaps and the proximity correlation of every legal corner for
+    // the current occupancy: one binary step (full or incremental, no stamp),
+    // the proximity field, and one FP64 step of the proximity session.
+    void rank()
+    {
+        if (!rank_on_) throw std::invalid_argument("ranking not set");
+        if (subset_) throw std::invalid_argument("ranking searches every orientation");
+        prepare();
+        use_device(ctx_);
+        if (incremental_) launch_incremental(false);
+        else launch_mode(false, false);
+        ensure_prox();
+        enqueue_edt();
+        mark("proximity_field");
+        prox_->launch_mode(false, false);
+        SP_HIP_CHECK(hipGetLastError());
+        rank_rev_ = env_rev_;
+    }
+
+    void ensure_prox()
+    {
+        if (prox_) return;
+        sp_session_desc d = desc_;
+        d.precision = SP_PRECISION_FP64;
+        auto p = std::make_unique<Session>(ctx_, d, stream_);
+        const std::size_t cells = std::size_t(ext_[0]) * ext_[1] * ext_[2];
+        p->real_env_ = true;
+        p->envd_.allocate(sizeof(double) * cells);
+        for (int o = 0; o != O_; ++o) {
+            const Kernel& k = kernels_[o];
+            const std::uint32_t sh[3] = { std::uint32_t(k.shape[0]), std::uint32_t(k.shape[1]), std::uint32_t(k.shape[2]) };
+            p->set_kernel(std::uint32_t(o), k.cells.data(), sh, k.first);
+        }
+        p->keep_raw(true);
+        p->set_fast_path(use_rf_);
+        p->set_autotune(autotune_);
+        p->prepare();
+        edt_.allocate(sizeof(unsigned) * cells);
+        // exp(-sqrt(d2)/2) for every squared distance the grid can hold, by the
+        // reference's own expression (field_proximity.hpp:129)
+        std::uint64_t dmax = 0;
+        for (int a = 0; a != 3; ++a) dmax += std::uint64_t(ext_[a] - 1) * std::uint64_t(ext_[a] - 1);
+        std::vector<double> table(dmax + 1);
+        for (std::uint64_t v = 0; v <= dmax; ++v) table[v] = std::exp(-std::sqrt(static_cast<double>(v)) / 2.);
+        edt_table_.allocate(sizeof(double) * table.size());
+        SP_HIP_CHECK(hipMemcpyAsync(edt_table_.get(), table.data(), sizeof(double) * table.size(), hipMemcpyHostToDevice,
+                                    stream_));
+        std::vector<long long> kfz(O_);
+        for (int o = 0; o != O_; ++o) kfz[o] = (long long)kernels_[o].first[2];
+        rank_max_z_d_.allocate(sizeof(double) * O_);
+        rank_kfz_d_.allocate(sizeof(long long) * O_);
+        SP_HIP_CHECK(hipMemcpyAsync(rank_max_z_d_.get(), rank_max_z_.data(), sizeof(double) * O_, hipMemcpyHostToDevice,
+                                    stream_));
+        SP_HIP_CHECK(hipMemcpyAsync(rank_kfz_d_.get(), kfz.data(), sizeof(long long) * O_, hipMemcpyHostToDevice, stream_));
+        rank_olist_.allocate(sizeof(int) * O_);
+        rank_cursor_.allocate(sizeof(PageKey) * O_);
+        rank_partial_.allocate(sizeof(PageKey) * std::size_t(O_) * kMaxPageBlocks * kPageSize);
+        rank_pcount_.allocate(sizeof(int) * std::size_t(O_) * kMaxPageBlocks);
+        rank_pages_.allocate(sizeof(PageKey) * std::size_t(O_) * kPageSize);
+        rank_count_.allocate(sizeof(int) * O_);
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        prox_ = std::move(p);
+    }
+
+    static int edt_tile_width(int n)
+    {
+        int tx = 64;
+        while (tx > 1 && std::size_t(n) * tx * sizeof(unsigned) > 65536) tx >>= 1;
+        return tx;
+    }
+
+    // build_proximity on the device: x rows, then y and z lines (the last
+    // writing the FP64 field into the proximity session's environment).
+    void enqueue_edt()
+    {
+        const long long rows = (long long)ext_[1] * ext_[2];
+        hipLaunchKernelGGL(k_edt_rows, dim3(unsigned((rows + 3) / 4)), dim3(256), 0, stream_, env_.as<unsigned char>(),
+                           ext_[0], rows, edt_.as<unsigned>());
+        int tx = edt_tile_width(ext_[1]);
+        hipLaunchKernelGGL(k_edt_lines<false>, dim3((ext_[0] + tx - 1) / tx, ext_[2]), dim3(256),
+                           std::size_t(ext_[1]) * tx * sizeof(unsigned), stream_, edt_.as<unsigned>(), ext_[0], ext_[1],
+                           (long long)ext_[0], (long long)ext_[0] * ext_[1], tx, nullptr, nullptr);
+        tx = edt_tile_width(ext_[2]);
+        hipLaunchKernelGGL(k_edt_lines<true>, dim3((ext_[0] + tx - 1) / tx, ext_[1]), dim3(256),
+                           std::size_t(ext_[2]) * tx * sizeof(unsigned), stream_, edt_.as<unsigned>(), ext_[0], ext_[2],
+                           (long long)ext_[0] * ext_[1], (long long)ext_[0], tx, edt_table_.as<double>(),
+                           prox_->envd_.as<double>());
+        SP_HIP_CHECK(hipGetLastError());
+    }
+
+    // One page per listed orientation (mode 0 ordinary, 1 low overlap), each
+    // after its cursor when has_cursor[i] is set.  Recomputes the fields when
+    // the occupancy changed since the last rank().
+    void ranked_pages(int mode, const std::uint32_t* orients, std::uint32_t count, const sp_ranked_candidate* cursors,
+                      const std::uint8_t* has_cursor, sp_ranked_candidate* out, std::uint32_t* counts)
+    {
+        if (mode != 0 && mode != 1) throw std::invalid_argument("page mode");
+        if (!rank_on_) throw std::invalid_argument("ranking not set");
+        prepare();
+        if (!prox_ || rank_rev_ != env_rev_) rank();
+        use_device(ctx_);
+        std::vector<int> olist;
+        std::vector<int> slot_of(count, -1);
+        std::vector<PageKey> keys;
+        bool any_cursor = false;
+        std::vector<char> listed(O_, 0);
+        for (std::uint32_t i = 0; i != count; ++i) {
+            const std::uint32_t o = orients[i];
+            if (o >= std::uint32_t(O_)) throw std::invalid_argument("orientation index");
+            if (listed[o]) throw std::invalid_argument("orientation listed twice");
+            listed[o] = 1;
+            counts[i] = 0;
+            if (descs_[o].lx <= 0) continue;  // a kernel wider than the window has no legal translation
+            PageKey k { ~0ull, ~0ull, 0xffffffffu, 0u };
+            if (cursors && has_cursor && has_cursor[i]) {
+                const sp_ranked_candidate& c = cursors[i];
+                unsigned long long pos = 0;
+                const int shift[3] = { 0, 16, 32 };
+                for (int a = 0; a != 3; ++a) {
+                    const std::int64_t u = c.translation[a] + kernels_[o].first[a] - efirst_[a];
+                    if (u < 0 || u > 0xffff) throw std::invalid_argument("cursor translation outside the window");
+                    pos |= (unsigned long long)u << shift[a];
+                }
+                if (!std::isfinite(c.score)) throw std::invalid_argument("cursor score must be finite");
+                double ov = mode ? c.binary_overlap : 0.0;
+                if (!(ov >= 0.0) || ov > 4294967295.0 || ov != std::floor(ov))
+                    throw std::invalid_argument("cursor overlap must be a non-negative integer");
+                k = PageKey { score_order_bits(c.score), pos, unsigned(ov), 1u };
+                any_cursor = true;
+            }
+            slot_of[i] = int(olist.size());
+            olist.push_back(int(o));
+            keys.push_back(k);
+        }
+        const int slots = int(olist.size());
+        if (slots == 0) return;
+        long long most = 0;
+        for (int o : olist) most = std::max(most, (long long)descs_[o].lx * descs_[o].ly * descs_[o].lz);
+        long long blocks = std::max<long long>(1, (4096 + slots - 1) / slots);
+        blocks = std::min<long long>(blocks, kMaxPageBlocks);
+        blocks = std::max<long long>(1, std::min<long long>(blocks, (most + kPagePerIter - 1) / kPagePerIter));
+        SP_HIP_CHECK(hipMemcpyAsync(rank_olist_.get(), olist.data(), sizeof(int) * slots, hipMemcpyHostToDevice, stream_));
+        SP_HIP_CHECK(hipMemcpyAsync(rank_cursor_.get(), keys.data(), sizeof(PageKey) * slots, hipMemcpyHostToDevice,
+                                    stream_));
+        RankParams rp {};
+        rp.origin_z = rank_desc_.lattice_origin_z_mm;
+        rp.pitch = rank_desc_.pitch_mm;
+        rp.container_min_z = rank_desc_.container_min_z_mm;
+        rp.container_height = rank_desc_.container_max_z_mm - rank_desc_.container_min_z_mm;
+        rp.env_first_z = (long long)efirst_[2];
+        rp.mode = mode;
+        rp.any_cursor = any_cursor ? 1 : 0;
+        hipLaunchKernelGGL(k_page_partial, dim3(unsigned(blocks), unsigned(slots)), dim3(kPageBlockThreads), 0, stream_,
+                           fields_.as<int>(), prox_->raw_.as<double>(), d_offsets_.as<long long>(),
+                           d_descs_.as<OrientDesc>(), rank_olist_.as<int>(), rp, rank_max_z_d_.as<double>(),
+                           rank_kfz_d_.as<long long>(), rank_cursor_.as<PageKey>(), excl_.as<unsigned long long>(),
+                           rank_partial_.as<PageKey>(), rank_pcount_.as<int>());
+        hipLaunchKernelGGL(k_page_merge, dim3(unsigned(slots)), dim3(kPageBlockThreads), 0, stream_,
+                           rank_partial_.as<PageKey>(), rank_pcount_.as<int>(), int(blocks), rank_olist_.as<int>(),
+                           d_descs_.as<OrientDesc>(), kcells_.as<unsigned char>(), env_.as<unsigned char>(), ext_[0],
+                           ext_[1], mode, rank_pages_.as<PageKey>(), rank_count_.as<int>());
+        SP_HIP_CHECK(hipGetLastError());
+        std::vector<PageKey> pages(std::size_t(slots) * kPageSize);
+        std::vector<int> pc(slots);
+        SP_HIP_CHECK(hipMemcpyAsync(pages.data(), rank_pages_.get(), sizeof(PageKey) * pages.size(),
+                                    hipMemcpyDeviceToHost, stream_));
+        SP_HIP_CHECK(hipMemcpyAsync(pc.data(), rank_count_.get(), sizeof(int) * slots, hipMemcpyDeviceToHost, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        for (std::uint32_t i = 0; i != count; ++i) {
+            const int sl = slot_of[i];
+            if (sl < 0) continue;
+            const int o = olist[sl];
+            counts[i] = std::uint32_t(pc[sl]);
+            for (int j = 0; j != pc[sl]; ++j) {
+                const PageKey& k = pages[std::size_t(sl) * kPageSize + j];
+                sp_ranked_candidate& c = out[std::size_t(i) * kPageSize + j];
+                const std::int64_t u[3] = { std::int64_t(k.pos & 0xffff), std::int64_t((k.pos >> 16) & 0xffff),
+                                            std::int64_t(k.pos >> 32) };
+                for (int a = 0; a != 3; ++a) c.translation[a] = efirst_[a] + u[a] - kernels_[o].first[a];
+                c.score = score_from_order_bits(k.score);
+                c.binary_overlap = double(k.overlap);
+                c.orientation = std::uint32_t(o);
+                c.direct_zero_overlap = k.direct;
+            }
+        }
+    }
+
+    // spectral.cpp:1485-1590, the ordinary part of the greedy trial loop:
+    // fresh pages of every orientation for the current occupancy, the best
+    // page head by rank_less (score, orientation, translation z, y, x) taken
+    // in turn, a consumed full page refilled after its cursor, and the first
+    // head that passed the direct recheck and `validate` (the caller's exact
+    // check; all accepted when null) stamped -- then everything again for the
+    // new occupancy, until no page has a candidate or max_copies are placed.
+    std::vector<sp_placement> ranked_fill(std::uint32_t max_copies, sp_validate_fn validate, void* user)
+    {
+        if (!rank_on_) throw std::invalid_argument("ranking not set");
+        prepare();
+        const auto rank_less = [](const sp_ranked_candidate& a, const sp_ranked_candidate& b) {
+            if (a.score != b.score) return a.score < b.score;
+            if (a.orientation != b.orientation) return a.orientation < b.orientation;
+            return std::tie(a.translation[2], a.translation[1], a.translation[0]) <
+                   std::tie(b.translation[2], b.translation[1], b.translation[0]);
+        };
+        std::vector<std::uint32_t> all(O_);
+        std::iota(all.begin(), all.end(), 0u);
+        std::vector<sp_ranked_candidate> buf(std::size_t(O_) * kPageSize);
+        std::vector<std::uint32_t> cnt(O_);
+        std::vector<sp_placement> placed;
+        while (placed.size() < max_copies) {
+            ranked_pages(0, all.data(), std::uint32_t(O_), nullptr, nullptr, buf.data(), cnt.data());
+            std::vector<std::vector<sp_ranked_candidate>> pages(O_);
+            std::vector<std::size_t> index(O_, 0);
+            std::vector<char> exhausted(O_);
+            std::vector<sp_ranked_candidate> cursor(O_);
+            for (int o = 0; o != O_; ++o) {
+                pages[o].assign(buf.begin() + std::ptrdiff_t(o) * kPageSize,
+                                buf.begin() + std::ptrdiff_t(o) * kPageSize + cnt[o]);
+                exhausted[o] = pages[o].empty();
+            }
+            bool admitted = false;
+            for (;;) {
+                std::vector<std::uint32_t> refill;
+                for (int o = 0; o != O_; ++o) {
+                    if (exhausted[o] || index[o] != pages[o].size()) continue;
+                    // a page shorter than a full one was the last of its orientation
+                    if (pages[o].size() < std::size_t(kPageSize)) exhausted[o] = 1;
+                    else refill.push_back(std::uint32_t(o));
+                }
+                if (!refill.empty()) {
+                    std::vector<sp_ranked_candidate> cur(refill.size());
+                    std::vector<std::uint8_t> has(refill.size(), 1);
+                    for (std::size_t i = 0; i != refill.size(); ++i) cur[i] = cursor[refill[i]];
+                    ranked_pages(0, refill.data(), std::uint32_t(refill.size()), cur.data(), has.data(), buf.data(),
+                                 cnt.data());
+                    for (std::size_t i = 0; i != refill.size(); ++i) {
+                        const std::uint32_t o = refill[i];
+                        pages[o].assign(buf.begin() + std::ptrdiff_t(i) * kPageSize,
+                                        buf.begin() + std::ptrdiff_t(i) * kPageSize + cnt[i]);
+                        index[o] = 0;
+                        exhausted[o] = cnt[i] == 0;
+                    }
+                }
+                int best = -1;
+                for (int o = 0; o != O_; ++o)
+                    if (index[o] < pages[o].size() &&
+                        (best < 0 || rank_less(pages[o][index[o]], pages[best][index[best]])))
+                        best = o;
+                if (best < 0) break;
+                const sp_ranked_candidate c = pages[best][index[best]++];
+                cursor[best] = c;
+                if (!c.direct_zero_overlap) continue;
+                if (validate && !validate(user, c.orientation, c.translation)) continue;
+                stamp_at(c.orientation, c.translation);
+                sp_placement pl {};
+                for (int a = 0; a != 3; ++a) pl.translation[a] = c.translation[a];
+                pl.orientation = c.orientation;
+                pl.score = c.score;
+                placed.push_back(pl);
+                admitted = true;
+                break;
+            }
+            if (!admitted) break;
+        }
+        return placed;
+    }
+
+    // The proximity field of the last rank(): exp(-sqrt(d2)/2) and d2
+    // (0x7fffffff where nothing is occupied), X-fastest over the window.
+    void proximity_field(double* values, std::uint32_t* squared, std::uint64_t count)
+    {
+        if (!prox_) throw std::invalid_argument("no ranking run");
+        const std::uint64_t cells = std::uint64_t(ext_[0]) * ext_[1] * ext_[2];
+        if (count < cells) throw std::length_error("capacity");
+        use_device(ctx_);
+        if (values)
+            SP_HIP_CHECK(hipMemcpyAsync(values, prox_->envd_.get(), cells * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        if (squared)
+            SP_HIP_CHECK(hipMemcpyAsync(squared, edt_.get(), cells * sizeof(unsigned), hipMemcpyDeviceToHost, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    // The proximity correlation of the last rank() at orientation o's legal corners.
+    void proximity_values(std::uint32_t o, double* out, std::uint64_t count)
+    {
+        if (!prox_) throw std::invalid_argument("no ranking run");
+        prox_->raw_values(o, out, count);
+    }
+
     hipStream_t stream() const { return stream_; }
     std::uint64_t device_bytes() const
     {
@@ -1573,6 +1903,15 @@ private:
     std::size_t max_lds_ { 65536 };
     std::vector<hipEvent_t> prof_events_;
     std::vector<const char*> prof_names_;
+    // occupancy revision (set_environment and every stamp) and ranking state
+    std::uint64_t env_rev_ {}, rank_rev_ { ~0ull };
+    bool real_env_ {}, rank_on_ {};
+    DeviceBuffer envd_;
+    sp_ranking_desc rank_desc_ {};
+    std::vector<double> rank_max_z_;
+    std::unique_ptr<Session> prox_;
+    DeviceBuffer edt_, edt_table_, rank_max_z_d_, rank_kfz_d_, rank_olist_, rank_cursor_, rank_partial_, rank_pcount_,
+        rank_pages_, rank_count_;
 };
 
 }  // namespace sphip
@@ -1836,5 +2175,51 @@ int sp_hip_session_profile_step(sp_hip_session* s, int stamp, float* kernel_ms, 
 void* sp_hip_session_stream(sp_hip_session* s) { return s ? static_cast<void*>(s->s->stream()) : nullptr; }
 
 uint64_t sp_hip_session_device_bytes(const sp_hip_session* s) { return s ? s->s->device_bytes() : 0; }
+
+
+int sp_hip_session_set_ranking(sp_hip_session* s, const sp_ranking_desc* desc, const double* object_max_z_mm,
+                               uint32_t count)
+{
+    if (!s || !desc || (!object_max_z_mm && count)) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_ranking(*desc, object_max_z_mm, count); });
+}
+
+int sp_hip_session_rank(sp_hip_session* s)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->rank(); });
+}
+
+int sp_hip_session_ranked_pages(sp_hip_session* s, int mode, const uint32_t* orientations, uint32_t count,
+                                const sp_ranked_candidate* cursors, const uint8_t* has_cursor,
+                                sp_ranked_candidate* pages, uint32_t* page_counts)
+{
+    if (!s || (count && (!orientations || !pages || !page_counts))) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx,
+                   [&] { s->s->ranked_pages(mode, orientations, count, cursors, has_cursor, pages, page_counts); });
+}
+
+int sp_hip_session_proximity_field(sp_hip_session* s, double* values, uint32_t* squared, uint64_t count)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->proximity_field(values, squared, count); });
+}
+
+int sp_hip_session_proximity_values(sp_hip_session* s, uint32_t o, double* values, uint64_t count)
+{
+    if (!s || !values) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->proximity_values(o, values, count); });
+}
+
+int sp_hip_session_ranked_fill(sp_hip_session* s, uint32_t max_copies, sp_validate_fn validate, void* user,
+                               sp_placement* placements, uint32_t capacity, uint32_t* placed)
+{
+    if (!s || !placed || (capacity && !placements)) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        const auto v = s->s->ranked_fill(std::min(max_copies, capacity), validate, user);
+        for (std::size_t i = 0; i != v.size(); ++i) placements[i] = v[i];
+        *placed = std::uint32_t(v.size());
+    });
+}
 
 }  // extern "C"

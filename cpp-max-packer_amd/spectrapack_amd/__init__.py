@@ -58,6 +58,27 @@ class StepResult(C.Structure):
                 ("precision", C.c_int32), ("error_bound", C.c_double)]
 
 
+class RankingDesc(C.Structure):
+    _fields_ = [("lattice_origin_z_mm", C.c_double), ("pitch_mm", C.c_double), ("container_min_z_mm", C.c_double),
+                ("container_max_z_mm", C.c_double)]
+
+
+class RankedCandidate(C.Structure):
+    _fields_ = [("translation", C.c_int64 * 3), ("score", C.c_double), ("binary_overlap", C.c_double),
+                ("orientation", C.c_uint32), ("direct_zero_overlap", C.c_uint32)]
+
+
+class Placement(C.Structure):
+    _fields_ = [("translation", C.c_int64 * 3), ("orientation", C.c_uint32), ("reserved", C.c_uint32),
+                ("score", C.c_double)]
+
+
+VALIDATE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(C.c_int64))
+
+PAGE_SIZE = 32
+PAGE_ORDINARY, PAGE_LOW_OVERLAP = 0, 1
+
+
 class Failure(Exception):
     """A CorrelationFailure / RepresentationFailure variant (code string + stats)."""
 
@@ -401,6 +422,80 @@ class Session:
         if lx * ly * lz == 0:
             return None
         return a[: lx * ly * lz].reshape(lz, ly, lx)
+
+    # ---- ranked pages (spectral_pipeline.cpp:1343,1563,1740-1857) ----------
+    def set_ranking(self, lattice_origin_z_mm, pitch_mm, container_min_z_mm, container_max_z_mm, object_max_z_mm):
+        """Score constants of the ranked pages and each orientation's oriented-bounds max z (mm)."""
+        d = RankingDesc(float(lattice_origin_z_mm), float(pitch_mm), float(container_min_z_mm),
+                        float(container_max_z_mm))
+        mz = np.ascontiguousarray(np.asarray(object_max_z_mm, dtype=np.float64).ravel())
+        self._check(lib().sp_hip_session_set_ranking(self._h, C.byref(d), _p(mz, C.c_double), C.c_uint32(mz.size)))
+
+    def rank(self):
+        self._check(lib().sp_hip_session_rank(self._h))
+
+    def ranked_pages(self, orientations=None, mode=PAGE_ORDINARY, cursors=None):
+        """One page per orientation: {o: [candidate dict, ...]} in rank order.
+        cursors: {o: candidate dict} -- each page starts after its cursor."""
+        ol = list(range(self.orientations)) if orientations is None else [int(o) for o in orientations]
+        n = len(ol)
+        if n == 0:
+            return {}
+        oa = np.ascontiguousarray(np.asarray(ol, dtype=np.uint32))
+        cur = (RankedCandidate * n)()
+        has = np.zeros(n, np.uint8)
+        for i, o in enumerate(ol):
+            c = (cursors or {}).get(o)
+            if c is not None:
+                cur[i] = RankedCandidate((C.c_int64 * 3)(*[int(v) for v in c["translation"]]), float(c["score"]),
+                                         float(c["binary_overlap"]), C.c_uint32(o), C.c_uint32(0))
+                has[i] = 1
+        pages = (RankedCandidate * (n * PAGE_SIZE))()
+        counts = np.zeros(n, np.uint32)
+        self._check(lib().sp_hip_session_ranked_pages(self._h, C.c_int(mode), _p(oa, C.c_uint32), C.c_uint32(n), cur,
+                                                      _p(has, C.c_uint8), pages, _p(counts, C.c_uint32)))
+        out = {}
+        for i, o in enumerate(ol):
+            out[o] = [dict(translation=tuple(pages[i * PAGE_SIZE + j].translation), score=pages[i * PAGE_SIZE + j].score,
+                           binary_overlap=pages[i * PAGE_SIZE + j].binary_overlap, orientation=o,
+                           direct_zero_overlap=bool(pages[i * PAGE_SIZE + j].direct_zero_overlap))
+                      for j in range(int(counts[i]))]
+        return out
+
+    def ranked_fill(self, max_copies=1 << 20, validate=None):
+        """The ordinary greedy loop over ranked pages (native); returns
+        [(orientation, translation, score)].  validate(o, t) -> bool is the
+        caller's exact check (every head accepted when None)."""
+        cap = int(min(max_copies, 1 << 20))
+        out = (Placement * max(cap, 1))()
+        n = C.c_uint32()
+        cb = None
+        if validate is not None:
+            cb = VALIDATE_FN(lambda user, o, t: 1 if validate(int(o), (t[0], t[1], t[2])) else 0)
+        self._check(lib().sp_hip_session_ranked_fill(self._h, C.c_uint32(cap), cb, None, out, C.c_uint32(cap),
+                                                     C.byref(n)))
+        return [(int(out[i].orientation), tuple(out[i].translation), float(out[i].score)) for i in range(n.value)]
+
+    def proximity_field(self):
+        """(values, squared) of the last rank, each [z, y, x]."""
+        ex, ey, ez = self.env_shape
+        v = np.zeros(ex * ey * ez, np.float64)
+        q = np.zeros(ex * ey * ez, np.uint32)
+        self._check(lib().sp_hip_session_proximity_field(self._h, _p(v, C.c_double), _p(q, C.c_uint32),
+                                                         C.c_uint64(v.size)))
+        return v.reshape(ez, ey, ex), q.reshape(ez, ey, ex)
+
+    def proximity_values(self, o):
+        """Proximity correlation of the last rank at orientation o's legal corners, [lz, ly, lx]."""
+        ex, ey, ez = self.env_shape
+        kx, ky, kz = self.kernel_shapes[o]
+        lx, ly, lz = ex - kx + 1, ey - ky + 1, ez - kz + 1
+        if min(lx, ly, lz) <= 0:
+            return None
+        a = np.zeros(lx * ly * lz, np.float64)
+        self._check(lib().sp_hip_session_proximity_values(self._h, C.c_uint32(o), _p(a, C.c_double),
+                                                          C.c_uint64(a.size)))
+        return a.reshape(lz, ly, lx)
 
     def profile_step(self, stamp=True):
         """One step without the graph; per-kernel milliseconds from HIP events."""

@@ -15,6 +15,9 @@
  *     voxelize_object / voxelize_container — the rasteriser.
  *   engine/pack_solver/src/orientation_cube.hpp:10-80, orientations.cpp:44-172 —
  *     orientation catalogs.
+ *   engine/pack_solver/src/field_proximity.hpp:32-132, spectral_pipeline.hpp:26-51,
+ *     spectral_pipeline.cpp:1740-1857, spectral.cpp:1485-1590 — proximity field,
+ *     ranked candidate pages and the greedy loop over them.
  */
 #ifndef SPECTRAPACK_HIP_H
 #define SPECTRAPACK_HIP_H
@@ -249,6 +252,80 @@ int sp_hip_session_orientation_minima(sp_hip_session* s, uint64_t* keys, uint32_
 void* sp_hip_session_stream(sp_hip_session* s);
 /* Device bytes held by the session. */
 uint64_t sp_hip_session_device_bytes(const sp_hip_session* s);
+
+/* ---- ranked candidate pages ------------------------------------------------
+ * The pack_solver greedy path's ranking (spectral_pipeline.hpp:26-51
+ * RankedCandidate / CandidatePageMode / CandidatePageQuery;
+ * spectral_pipeline.cpp:1343 build_proximity, :1563 correlate_proximity_cpu,
+ * :1740-1857 the page), batched over orientations on one session:
+ *   proximity field  exp(-sqrt(d2)/2) of the occupancy (field_proximity.hpp:32-132),
+ *   proximity        its FP64 correlation with each kernel at every legal corner,
+ *   score            .65 * top - .35 * proximity / kernel ones, where
+ *                    top = (origin_z + pitch * t_z + object_max_z - container_min_z)
+ *                          / (container_max_z - container_min_z),
+ *   page             the 32 best candidates after the cursor by (score,
+ *                    translation z, y, x); ordinary pages keep only zero binary
+ *                    overlap and recheck each entry directly against the
+ *                    occupancy; low-overlap pages order by overlap first.
+ * Rejected translations (sp_hip_session_exclude) never appear in a page. */
+typedef struct sp_ranking_desc {
+    double lattice_origin_z_mm;  /* lattice.origin_mm[2] */
+    double pitch_mm;             /* lattice.pitch_mm */
+    double container_min_z_mm;   /* container bounds min z */
+    double container_max_z_mm;   /* container bounds max z (height must be > 0) */
+} sp_ranking_desc;
+
+typedef struct sp_ranked_candidate {
+    int64_t translation[3];
+    double score;
+    double binary_overlap;        /* exact count; 0 in ordinary pages */
+    uint32_t orientation;
+    uint32_t direct_zero_overlap; /* ordinary pages: the direct recheck found no overlap */
+} sp_ranked_candidate;
+
+#define SP_PAGE_SIZE 32
+#define SP_PAGE_ORDINARY 0
+#define SP_PAGE_LOW_OVERLAP 1
+
+/* Enables ranking: the constants above and each orientation's oriented-bounds
+ * max z in mm (geometry::oriented_bounds(...).bounds_mm.max[2]), count =
+ * orientations.  Environments of at most 1024 cells per axis.  Keeps the
+ * binary overlap fields. */
+int sp_hip_session_set_ranking(sp_hip_session* s, const sp_ranking_desc* desc, const double* object_max_z_mm,
+                               uint32_t count);
+/* Binary overlaps, proximity field and proximity correlation of the current
+ * occupancy (ranked_pages runs this by itself when the occupancy changed). */
+int sp_hip_session_rank(sp_hip_session* s);
+/* One page per listed orientation: pages[i * SP_PAGE_SIZE + j], j <
+ * page_counts[i], in rank order; cursors[i] is used when has_cursor[i] is
+ * nonzero (both may be null).  An orientation whose kernel does not fit the
+ * window has an empty page. */
+int sp_hip_session_ranked_pages(sp_hip_session* s, int mode, const uint32_t* orientations, uint32_t count,
+                                const sp_ranked_candidate* cursors, const uint8_t* has_cursor,
+                                sp_ranked_candidate* pages, uint32_t* page_counts);
+/* spectral.cpp:1485-1590, the ordinary greedy loop over ranked pages: fresh
+ * pages of every orientation for the current occupancy; the best page head by
+ * (score, orientation, translation z, y, x) in turn; a consumed full page
+ * refilled after its cursor; the first head whose direct recheck found no
+ * overlap and that `validate` accepts (the caller's exact check, nonzero =
+ * accept; every head accepted when null) is stamped, and the search starts
+ * over.  Stops when no page has a candidate or min(max_copies, capacity)
+ * copies are placed; *placed = copies written to placements. */
+typedef int (*sp_validate_fn)(void* user, uint32_t orientation, const int64_t translation[3]);
+typedef struct sp_placement {
+    int64_t translation[3];
+    uint32_t orientation;
+    uint32_t reserved;
+    double score;
+} sp_placement;
+int sp_hip_session_ranked_fill(sp_hip_session* s, uint32_t max_copies, sp_validate_fn validate, void* user,
+                               sp_placement* placements, uint32_t capacity, uint32_t* placed);
+/* The proximity field of the last rank: values (may be null) and exact squared
+ * distances (may be null; 0x7fffffff where no cell is occupied), X-fastest. */
+int sp_hip_session_proximity_field(sp_hip_session* s, double* values, uint32_t* squared, uint64_t count);
+/* The proximity correlation of the last rank at orientation o's legal corners
+ * (X-fastest, shape environment_shape - kernel_shape + 1). */
+int sp_hip_session_proximity_values(sp_hip_session* s, uint32_t o, double* values, uint64_t count);
 
 /* ---- conservative rasteriser (conservative_fields.hpp:100-164) -------------
  * Mesh: vertex_count x 3 doubles (local frame), triangle_count x 3 indices.
