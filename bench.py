@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--top-r", type=int, default=512)
     ap.add_argument("--only-learned", action="store_true",
                     help="run only learned-proposal episodes (no plain step timing, no plain episodes); for profiling")
+    ap.add_argument("--no-ranked", action="store_true", help="skip the ranked-page (proximity) timing")
+    ap.add_argument("--only-ranked", action="store_true", help="run only the ranked-page timing; for profiling")
+    ap.add_argument("--ranked-placements", type=int, default=10, help="greedy placements timed on ranked pages")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -160,6 +163,10 @@ def main():
     t_raster = time.perf_counter() - t0
     log(f"[bench] container {t_container:.3f}s raster {len(kernels)} orientations {t_raster:.3f}s")
 
+    if args.only_ranked:
+        print(json.dumps({"ranked_pages": ranked_timing(sp, ctx, w, env, kernels, args.precision or 0, args)}),
+              flush=True)
+        return
     s = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=args.precision or
                    (32 if w.precision == "fp32" else 0), capture_graph=not args.no_graph)
     s.set_environment(env)
@@ -244,6 +251,10 @@ def main():
         inc.close()
         ref.close()
 
+    ranked = None
+    if not args.no_ranked:
+        ranked = ranked_timing(sp, ctx, w, env, kernels, prec, args)
+
     # per-kernel breakdown (no graph, events between kernels)
     prof = {}
     for _ in range(args.profile_steps):
@@ -286,6 +297,7 @@ def main():
                    "autotuned_tiles": tiles},
         "ms_per_step_kernel_spectra_recomputed": ms_step_recompute,
         "incremental": incremental,
+        "ranked_pages": ranked,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
@@ -309,6 +321,61 @@ def main():
         out["cpu_baseline"] = cpu_baseline(w, env, kernels, args.cpu_threads, max(1, args.cpu_threads))
         out["cpu_baseline"]["host"] = platform.processor() or platform.machine()
     print(json.dumps(out), flush=True)
+
+
+def ranked_timing(sp, ctx, w, env, kernels, prec, args):
+    """The reference's ranked-page path (spectral_pipeline.cpp:1343-1857 +
+    spectral.cpp:1485-1590) on the same workload: proximity field, FP64
+    proximity correlation, binary overlaps (incremental fields), a 32-entry
+    page per orientation, and greedy placements over the pages."""
+    import numpy as np
+    import torch
+    from spectrapack_amd import workloads as W
+    s = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=prec, capture_graph=not args.no_graph)
+    s.set_environment(env)
+    for i, k in enumerate(kernels):
+        s.set_kernel(i, k["cells"], k["shape"], k["first"])
+    s.set_incremental(True)
+    cmin, cmax = W.container_z_range(w)
+    s.set_ranking(0.0, w.pitch, cmin, cmax, W.oriented_max_z(w.part, w.catalog))
+    t0 = time.perf_counter()
+    s.ranked_fill(max_copies=1)     # autotune, graph capture, first fields
+    t_first = time.perf_counter() - t0
+    mass0 = int(s.environment().sum())
+    stream = torch.cuda.ExternalStream(s.stream())
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = max(3, min(args.steps, 10))
+    s.rank()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        s.rank()
+    e1.record(stream)
+    e1.synchronize()
+    ms_rank = e0.elapsed_time(e1) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pages = s.ranked_pages()
+    ms_pages = (time.perf_counter() - t0) / reps * 1e3
+    t0 = time.perf_counter()
+    placed = s.ranked_fill(max_copies=args.ranked_placements)
+    ms_fill = (time.perf_counter() - t0) / max(len(placed), 1) * 1e3
+    mass1 = int(s.environment().sum())
+    masses = [int(k["cells"].sum()) for k in kernels]
+    added = sum(masses[o] for o, _, _ in placed)
+    heads = sorted((pages[o][0]["score"], o) for o in pages if pages[o])
+    out = {"ms_rank": ms_rank, "ms_pages_all_orientations": ms_pages, "ms_per_placement": ms_fill,
+           "placements_per_s": 1000.0 / ms_fill if placed else 0.0, "placements": len(placed),
+           "first_fill_s": t_first, "nonempty_pages": sum(1 for o in pages if pages[o]),
+           "best_head_score": heads[0][0] if heads else None,
+           "mass_added_equals_placed_kernels": mass1 - mass0 == added,
+           "note": "rank = binary overlaps (incremental fields) + proximity field + FP64 proximity correlation; "
+                   "pages = 32 best per orientation + direct recheck; a placement = rank + pages + choice + stamp"}
+    log(f"[bench] ranked pages: rank {ms_rank:.3f} ms, pages {ms_pages:.3f} ms, {ms_fill:.3f} ms/placement "
+        f"({len(placed)} placed), mass check {out['mass_added_equals_placed_kernels']}")
+    s.close()
+    return out
 
 
 def learned_comparison(ctx, args, plain=True):

@@ -20,7 +20,8 @@ namespace sphip {
 constexpr unsigned kEdtInf = 0x7fffffffu;
 constexpr int kMaxRankExtent = 1024;
 constexpr int kPageSize = 32;
-constexpr int kPageCap = 2048;        // LDS candidate buffer of a page block
+constexpr int kPageCap = 2048;        // LDS merge buffer (partial pages of one orientation)
+constexpr int kPartialCap = 2048;     // LDS candidate buffer of a page block (a power of two)
 constexpr int kPageBlockThreads = 256;
 constexpr int kPagePerIter = 4 * kPageBlockThreads;
 constexpr int kMaxPageBlocks = kPageCap / kPageSize;
@@ -218,7 +219,7 @@ __global__ __launch_bounds__(kPageBlockThreads) void k_page_partial(
     const double* __restrict__ max_z, const long long* __restrict__ kfirst_z, const PageKey* __restrict__ cursors,
     const unsigned long long* __restrict__ excl, PageKey* __restrict__ partial, int* __restrict__ partial_count)
 {
-    __shared__ PageKey buf[kPageCap];
+    __shared__ PageKey buf[kPartialCap];
     __shared__ int count;
     __shared__ PageKey thr;
     const int slot = blockIdx.y;
@@ -242,26 +243,57 @@ __global__ __launch_bounds__(kPageBlockThreads) void k_page_partial(
     const bool any_excluded = excl[0] != 0;
     const double mz = max_z[o];
     const double mass = double(d.mass);
+    // corner -> (z, y, x) by FP64 reciprocals, corrected to the exact
+    // quotient (indices stay below 2^30, so the estimate is off by at most 1)
+    const unsigned uplane = unsigned(plane), ulx = unsigned(d.lx > 0 ? d.lx : 1);
+    const double inv_plane = 1.0 / double(uplane > 0 ? uplane : 1u), inv_lx = 1.0 / double(ulx);
+    constexpr int J = kPagePerIter / kPageBlockThreads;
+    // the overlaps of the next iteration load while this one is scored, and
+    // the proximity loads of its eligible corners are issued together
+    int ov[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const long long i = begin + j * kPageBlockThreads + threadIdx.x;
+        ov[j] = i < end ? fo[i] : (1 << 30);
+    }
     for (long long i0 = begin; i0 < end; i0 += kPagePerIter) {
-        for (int j = 0; j < kPagePerIter / kPageBlockThreads; ++j) {
-            const long long i = i0 + j * kPageBlockThreads + threadIdx.x;
-            if (i >= end) break;
-            const int ov = fo[i];
-            if (ov >= (1 << 30) || (rp.mode == 0 && ov != 0)) continue;
-            const int z = int(i / plane);
-            const long long rem = i - (long long)z * plane;
-            const int y = int(rem / d.lx);
-            const int x = int(rem - (long long)y * d.lx);
+        bool ok[J];
+        double pv[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            ok[j] = ov[j] < (1 << 30) && (rp.mode != 0 || ov[j] == 0);
+            pv[j] = ok[j] ? po[i0 + j * kPageBlockThreads + threadIdx.x] : 0.0;
+        }
+        // issued after the proximity loads: memory counters complete in
+        // order, so scoring waits for those while these stay in flight
+        int nv[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const long long i = i0 + kPagePerIter + j * kPageBlockThreads + threadIdx.x;
+            nv[j] = i < end ? fo[i] : (1 << 30);
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if (!ok[j]) continue;
+            const unsigned ui = unsigned(i0 + j * kPageBlockThreads + threadIdx.x);
+            unsigned z = unsigned(double(ui) * inv_plane);
+            if (z * uplane > ui) --z;
+            else if ((z + 1u) * uplane <= ui) ++z;
+            const unsigned rem = ui - z * uplane;
+            unsigned y = unsigned(double(rem) * inv_lx);
+            if (y * ulx > rem) --y;
+            else if ((y + 1u) * ulx <= rem) ++y;
+            const unsigned x = rem - y * ulx;
             if (any_excluded) {
                 const unsigned long long skey = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                                 ((unsigned long long)x << 16) | (unsigned long long)o;
                 if (excluded(excl, skey)) continue;
             }
             const double s = page_score(rp.origin_z, rp.pitch, rp.container_min_z, rp.container_height, mz,
-                                        rp.env_first_z + z - kfirst_z[o], po[i], mass);
+                                        rp.env_first_z + z - kfirst_z[o], pv[j], mass);
             const PageKey k { score_order_bits(s),
                               ((unsigned long long)z << 32) | ((unsigned long long)y << 16) | (unsigned long long)x,
-                              rp.mode ? unsigned(ov) : 0u, 0u };
+                              rp.mode ? unsigned(ov[j]) : 0u, 0u };
             if (has_cur && !key_less(cur, k)) continue;
             if (!key_less(k, thr)) continue;
             const int at = atomicAdd(&count, 1);
@@ -272,7 +304,9 @@ __global__ __launch_bounds__(kPageBlockThreads) void k_page_partial(
         __syncthreads();
         const int filled = count;
         __syncthreads();
-        if (filled > kPageCap - kPagePerIter) page_flush(buf, &count, &thr);
+        if (filled > kPartialCap - kPagePerIter) page_flush(buf, &count, &thr);
+#pragma unroll
+        for (int j = 0; j < J; ++j) ov[j] = nv[j];
     }
     __syncthreads();
     if (count > 0) page_flush(buf, &count, &thr);
@@ -283,17 +317,14 @@ __global__ __launch_bounds__(kPageBlockThreads) void k_page_partial(
 }
 
 // Stage 2: one block per slot merges the partial pages (at most
-// kMaxPageBlocks of them) into the page and, for ordinary pages, rechecks
-// every entry directly against the occupancy (spectral_pipeline.cpp:1825-1855).
-__global__ __launch_bounds__(kPageBlockThreads) void k_page_merge(
-    const PageKey* __restrict__ partial, const int* __restrict__ partial_count, int blocks,
-    const int* __restrict__ olist, const OrientDesc* __restrict__ descs, const unsigned char* __restrict__ kcells,
-    const unsigned char* __restrict__ env, int ex, int ey, int mode, PageKey* __restrict__ pages,
-    int* __restrict__ page_count)
+// kMaxPageBlocks of them) into the page.
+__global__ __launch_bounds__(kPageBlockThreads) void k_page_merge(const PageKey* __restrict__ partial,
+                                                                  const int* __restrict__ partial_count, int blocks,
+                                                                  PageKey* __restrict__ pages,
+                                                                  int* __restrict__ page_count)
 {
     __shared__ PageKey buf[kPageCap];
     __shared__ int base[kMaxPageBlocks + 1];
-    __shared__ unsigned hit[kPageSize];
     const int slot = blockIdx.x;
     if (threadIdx.x == 0) {
         int s = 0;
@@ -303,7 +334,6 @@ __global__ __launch_bounds__(kPageBlockThreads) void k_page_merge(
         }
         base[blocks] = s;
     }
-    if (threadIdx.x < kPageSize) hit[threadIdx.x] = 0;
     __syncthreads();
     const int n = base[blocks];
     for (int b = 0; b < blocks; ++b) {
@@ -317,27 +347,39 @@ __global__ __launch_bounds__(kPageBlockThreads) void k_page_merge(
     __syncthreads();
     page_sort(buf, m);
     const int keep = n < kPageSize ? n : kPageSize;
-    if (mode == 0 && keep > 0) {
-        const OrientDesc d = descs[olist[slot]];
-        const unsigned char* kc = kcells + d.cells_offset;
-        const int kcount = d.kx * d.ky * d.kz;
-        for (int e = 0; e < keep; ++e) {
-            const unsigned long long p = buf[e].pos;
-            const int ux = int(p & 0xffff), uy = int((p >> 16) & 0xffff), uz = int(p >> 32);
-            for (int c = threadIdx.x; c < kcount; c += blockDim.x) {
-                if (!kc[c]) continue;
-                const int cx = c % d.kx, cy = (c / d.kx) % d.ky, cz = c / (d.kx * d.ky);
-                if (env[(ux + cx) + (long long)ex * ((uy + cy) + (long long)ey * (uz + cz))]) hit[e] = 1u;
-            }
-        }
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < keep; i += blockDim.x) {
-        PageKey k = buf[i];
-        k.direct = mode == 0 ? (hit[i] ? 0u : 1u) : 0u;
-        pages[(long long)slot * kPageSize + i] = k;
-    }
+    for (int i = threadIdx.x; i < keep; i += blockDim.x) pages[(long long)slot * kPageSize + i] = buf[i];
     if (threadIdx.x == 0) page_count[slot] = keep;
+}
+
+// Stage 3, ordinary pages: block (entry, slot) rechecks one page entry
+// directly against the occupancy (spectral_pipeline.cpp:1825-1855); each wave
+// walks kernel rows (y, z), its lanes the row's cells.
+__global__ __launch_bounds__(kPageBlockThreads) void k_page_recheck(const int* __restrict__ olist,
+                                                                    const OrientDesc* __restrict__ descs,
+                                                                    const unsigned char* __restrict__ kcells,
+                                                                    const unsigned char* __restrict__ env, int ex, int ey,
+                                                                    PageKey* __restrict__ pages,
+                                                                    const int* __restrict__ page_count)
+{
+    const int slot = blockIdx.y, e = blockIdx.x;
+    if (e >= page_count[slot]) return;
+    PageKey* k = pages + (long long)slot * kPageSize + e;
+    const unsigned long long p = k->pos;
+    const int ux = int(p & 0xffff), uy = int((p >> 16) & 0xffff), uz = int(p >> 32);
+    const OrientDesc d = descs[olist[slot]];
+    const unsigned char* kc = kcells + d.cells_offset;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    int hit = 0;
+    for (int q = wave; q < d.ky * d.kz && !hit; q += waves) {
+        const int cz = q / d.ky, cy = q - cz * d.ky;
+        const unsigned char* krow = kc + (long long)q * d.kx;
+        const unsigned char* erow = env + ux + (long long)ex * ((uy + cy) + (long long)ey * (uz + cz));
+        for (int cx = lane; cx < d.kx; cx += 64)
+            if (krow[cx] && erow[cx]) hit = 1;
+        hit = __any(hit);
+    }
+    hit = __syncthreads_or(hit);
+    if (threadIdx.x == 0) k->direct = hit ? 0u : 1u;
 }
 
 }  // namespace sphip

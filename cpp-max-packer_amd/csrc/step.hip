@@ -1706,9 +1706,13 @@ public:
                            rank_kfz_d_.as<long long>(), rank_cursor_.as<PageKey>(), excl_.as<unsigned long long>(),
                            rank_partial_.as<PageKey>(), rank_pcount_.as<int>());
         hipLaunchKernelGGL(k_page_merge, dim3(unsigned(slots)), dim3(kPageBlockThreads), 0, stream_,
-                           rank_partial_.as<PageKey>(), rank_pcount_.as<int>(), int(blocks), rank_olist_.as<int>(),
-                           d_descs_.as<OrientDesc>(), kcells_.as<unsigned char>(), env_.as<unsigned char>(), ext_[0],
-                           ext_[1], mode, rank_pages_.as<PageKey>(), rank_count_.as<int>());
+                           rank_partial_.as<PageKey>(), rank_pcount_.as<int>(), int(blocks), rank_pages_.as<PageKey>(),
+                           rank_count_.as<int>());
+        if (mode == 0)
+            hipLaunchKernelGGL(k_page_recheck, dim3(unsigned(kPageSize), unsigned(slots)), dim3(kPageBlockThreads), 0,
+                               stream_, rank_olist_.as<int>(), d_descs_.as<OrientDesc>(), kcells_.as<unsigned char>(),
+                               env_.as<unsigned char>(), ext_[0], ext_[1], rank_pages_.as<PageKey>(),
+                               rank_count_.as<int>());
         SP_HIP_CHECK(hipGetLastError());
         std::vector<PageKey> pages(std::size_t(slots) * kPageSize);
         std::vector<int> pc(slots);

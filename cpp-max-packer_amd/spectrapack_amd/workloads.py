@@ -203,6 +203,20 @@ def rotation_matrix(q) -> np.ndarray:
                      [2 * (x * z - y * w), 2 * (y * z + x * w), w * w - x * x - y * y + z * z]])
 
 
+def oriented_max_z(mesh: Mesh, quaternions) -> np.ndarray:
+    """Per orientation, the rotated part's highest vertex z in mm (the max z of
+    its oriented bounds, geometry::oriented_bounds(...).bounds_mm.max[2])."""
+    return np.array([float((mesh.vertices @ rotation_matrix(q).T)[:, 2].max()) for q in quaternions])
+
+
+def container_z_range(w) -> tuple:
+    """Container bounds in z (mm): the mesh's bounds, or [0, height] of a box."""
+    if w.container is not None:
+        lo, hi = w.container.bounds()
+        return float(lo[2]), float(hi[2])
+    return 0.0, float(w.box_dims[2])
+
+
 def canonical(q) -> np.ndarray:
     """orientations.cpp:44-73: unit norm, w > 0 (else first nonzero > 0), no -0."""
     q = np.asarray(q, dtype=np.float64).copy()
