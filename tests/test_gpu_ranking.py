@@ -237,3 +237,60 @@ def test_ranking_rejects_large_grids_and_bad_heights(ctx):
         s.set_ranking(0.0, 1.0, 2.0, 2.0, [1.0])
     with pytest.raises(sp.Failure):
         s.set_ranking(0.0, 1.0, 0.0, 4.0, [1.0, 2.0])
+
+
+def test_pages_at_full_size_bracket(ctx):
+    """256^3 x 96 (L-bracket in the sphere), after a few greedy placements:
+    every page is ordered by its key, every ordinary entry has zero overlap by
+    a direct count, every score is the pipeline expression of the session's
+    own proximity value (bit-exact), the proximity at the page corners is a
+    direct sum of the proximity field within 1e-10, and the second page
+    continues strictly after the first."""
+    import spectrapack_amd as sp
+    from spectrapack_amd import workloads as W
+    w = W.bracket_in_sphere()
+    env = ctx.mesh_container(w.container.vertices, w.container.triangles, w.env_first, w.env_shape, w.pitch)
+    kernels = ctx.voxelize_objects(w.part.vertices, w.part.triangles, w.catalog, w.pitch)
+    s = make_session(ctx, env, w.env_first, kernels, incremental=True)
+    cmin, cmax = W.container_z_range(w)
+    max_z = W.oriented_max_z(w.part, w.catalog)
+    s.set_ranking(0.0, w.pitch, cmin, cmax, max_z)
+    print("[ranking] full size: 3 greedy placements", flush=True)
+    placed = s.ranked_fill(max_copies=3)
+    assert len(placed) == 3
+    pages = s.ranked_pages()
+    occ = s.environment()
+    field, _ = s.proximity_field()
+    first = w.env_first
+    checked = 0
+    for o, page in pages.items():
+        if not page:
+            continue
+        k = kernels[o]
+        kc = np.asarray(k["cells"]) != 0
+        mass = int(kc.sum())
+        prox = s.proximity_values(o)
+        tf = [first[a] - k["first"][a] for a in range(3)]
+        keys = []
+        for c in page:
+            u = [c["translation"][a] - tf[a] for a in range(3)]
+            keys.append((c["score"], u[2], u[1], u[0]))
+            win = occ[u[2]:u[2] + kc.shape[0], u[1]:u[1] + kc.shape[1], u[0]:u[0] + kc.shape[2]]
+            assert not win[kc].any() and c["direct_zero_overlap"]
+            want = PG.page_scores(prox[u[2]:u[2] + 1, u[1]:u[1] + 1, u[0]:u[0] + 1], mass, 0.0, w.pitch, cmin, cmax,
+                                  max_z[o], tf[2] + u[2])[0, 0, 0]
+            assert np.float64(c["score"]).view(np.uint64) == np.float64(want).view(np.uint64)
+            fw = field[u[2]:u[2] + kc.shape[0], u[1]:u[1] + kc.shape[1], u[0]:u[0] + kc.shape[2]]
+            direct = float(fw[kc].sum())
+            assert abs(prox[u[2], u[1], u[0]] - direct) <= 1e-10 * max(1.0, direct)
+            checked += 1
+        assert keys == sorted(keys)
+    print(f"[ranking] full size: {checked} page entries checked", flush=True)
+    assert checked >= 32 * 90
+    some = [o for o in pages if len(pages[o]) == PG.PAGE_SIZE][:8]
+    more = s.ranked_pages(orientations=some, cursors={o: pages[o][-1] for o in some})
+    for o in some:
+        a, b = pages[o][-1], more[o][0]
+        ka = (a["score"], a["translation"][2], a["translation"][1], a["translation"][0])
+        kb = (b["score"], b["translation"][2], b["translation"][1], b["translation"][0])
+        assert ka < kb
