@@ -103,3 +103,48 @@ def test_bracket_workload_fast_equals_generic_and_properties(ctx, oracle):
         assert not (cur[uz:uz + kz, uy:uy + ky, ux:ux + kx] & k["cells"]).any()
         cur[uz:uz + kz, uy:uy + ky, ux:ux + kx] |= k["cells"]
         assert np.array_equal(sess[True].environment(), cur)
+
+
+def test_knot_512_fp64_sampled_steps(ctx):
+    """The 512^3 torus-knot workload, 24 cube orientations, FP64 chosen by the
+    bound: on two stamped steps, sampled overlap counts of three orientations
+    (the chosen one included) equal direct sums, each sampled field's first zero
+    in (z, y, x) order is that orientation's reported minimum, the chosen
+    corner is free and is the lowest minimum over all orientations."""
+    import spectrapack_amd as sp
+    from spectrapack_amd import workloads as W
+    w = W.knot_in_fine_box()
+    env = ctx.box_container(w.env_first, w.env_shape, w.pitch, w.box_dims)
+    kernels = ctx.voxelize_objects(w.part.vertices, w.part.triangles, w.catalog, w.pitch)
+    s = sp.Session(ctx, w.env_shape, w.env_first, len(kernels))
+    s.set_environment(env)
+    for i, k in enumerate(kernels):
+        s.set_kernel(i, k["cells"], k["shape"], k["first"])
+    s.keep_fields(True)
+    rs = np.random.default_rng(12)
+    cur = env.copy()
+    for step in range(2):
+        print(f"[fast] knot 512^3 step={step}", flush=True)
+        r = s.step(stamp=True)
+        assert r["precision"] == 64 and r["error_bound"] > 0.4 and r["numeric_flags"] == 0
+        o = r["orientation"]
+        minima = s.orientation_minima()
+        keys = [(m, i) for i, m in enumerate(minima) if m is not None]
+        assert min(keys)[1] == o
+        for i in sorted({o, int(rs.integers(0, 24)), int(rs.integers(0, 24))}):
+            f = s.overlap_field(i)
+            k = kernels[i]
+            kz, ky, kx = k["cells"].shape
+            for _ in range(30):
+                uz, uy, ux = (int(rs.integers(0, f.shape[a])) for a in range(3))
+                assert f[uz, uy, ux] == int((cur[uz:uz + kz, uy:uy + ky, ux:ux + kx] & k["cells"]).sum())
+            zeros = np.argwhere(f == 0)
+            assert (tuple(zeros[0]) if len(zeros) else None) == minima[i]
+        uz, uy, ux = minima[o]
+        k = kernels[o]
+        kz, ky, kx = k["cells"].shape
+        assert not (cur[uz:uz + kz, uy:uy + ky, ux:ux + kx] & k["cells"]).any()
+        cur[uz:uz + kz, uy:uy + ky, ux:ux + kx] |= k["cells"]
+        del f
+    assert np.array_equal(s.environment(), cur)
+    s.close()
