@@ -297,8 +297,11 @@ __global__ void __launch_bounds__(256) k_mul_x(const cx<T>* __restrict__ H, long
 // C2R along z for the legal columns (x < lx, y < ly), fused with rounding, the
 // reference's per-value checks (residual < 0.25, 0 <= v <= min(mass_B,
 // mass_K)), the feasible mask (round(v) == 0) and the lowest (z, y, x) choice.
-// Grid: x = column tiles (W of x), y = row y, z = orientation.
-template <typename T>
+// Grid: x = column tiles (W of x), y = row y, z = orientation.  OUT: the
+// overlap fields and/or raw values are written (else only the choice).
+// Rounding is to nearest-even: it differs from round-half-away only at an
+// exact .5, whose residual 0.5 already fails the step's check.
+template <typename T, bool OUT>
 __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ in, long long in_o, int nx, int ny,
                                                       const OrientDesc* __restrict__ descs, T inv_total,
                                                       StepDeviceResult* __restrict__ result, int* __restrict__ fields,
@@ -340,25 +343,31 @@ __global__ void __launch_bounds__(256) k_c2r_z_argmin(const cx<T>* __restrict__ 
     unsigned long long mine = ~0ULL;
     float my_res = 0.f;
     unsigned int my_flags = 0;
-    int* fo = fields ? fields + field_offsets[o] : nullptr;
-    for (int i = threadIdx.x; i < d.lz * W; i += blockDim.x) {
-        const int z = i / W, w = i % W;
-        const int x = x0 + w;
-        if (x >= d.lx) continue;
-        const T2 r = R[z * sw + (w % P)];
-        const T v = (w < P ? r.x : r.y) * inv_total;
-        const T rv = round(v);
-        const T res = fabs(v - rv);
-        my_res = fmaxf(my_res, float(res));
-        if (!(res < T(0.25))) my_flags |= 1u;
-        if (rv < T(0) || rv > max_out) my_flags |= 2u;
-        if (fo) fo[x + (long long)d.lx * (y + (long long)d.ly * z)] = int(rv);
-        if (raw) raw[field_offsets[o] + x + (long long)d.lx * (y + (long long)d.ly * z)] = double(v);
-        if (rv == T(0)) {
-            const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
-                                           ((unsigned long long)x << 16) | (unsigned long long)o;
-            if (!excluded(excl, key) && region_allowed(filt, x, y, z))
-                mine = key < mine ? key : mine;
+    int* fo = OUT && fields ? fields + field_offsets[o] : nullptr;
+    const bool any_excluded = excl[0] != 0;
+    const bool regions_on = filt.act && filt.act[1];
+    // blockDim.x is a multiple of W: each thread keeps one column w
+    const int w = threadIdx.x % W;
+    const int x = x0 + w;
+    if (x < d.lx) {
+        for (int z = threadIdx.x / W; z < d.lz; z += blockDim.x / W) {
+            const T2 r = R[z * sw + (w % P)];
+            const T v = (w < P ? r.x : r.y) * inv_total;
+            const T rv = rint(v);
+            const T res = fabs(v - rv);
+            my_res = fmaxf(my_res, float(res));
+            if (!(res < T(0.25))) my_flags |= 1u;
+            if (rv < T(0) || rv > max_out) my_flags |= 2u;
+            if (OUT) {
+                if (fo) fo[x + (long long)d.lx * (y + (long long)d.ly * z)] = int(rv);
+                if (raw) raw[field_offsets[o] + x + (long long)d.lx * (y + (long long)d.ly * z)] = double(v);
+            }
+            if (rv == T(0)) {
+                const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
+                                               ((unsigned long long)x << 16) | (unsigned long long)o;
+                if ((!any_excluded || !excluded(excl, key)) && (!regions_on || region_allowed(filt, x, y, z)))
+                    mine = key < mine ? key : mine;
+            }
         }
     }
     if (mine != ~0ULL) atomicMin(&best, mine);

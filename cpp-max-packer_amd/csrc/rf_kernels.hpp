@@ -215,8 +215,9 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_r2c_z(const In* __restri
 }
 
 // C2R along z of column pairs + rounding, the reference's per-value checks,
-// the feasible mask and the lowest (z, y, x) key (see k_c2r_z_argmin).
-template <typename T, int N, int R, int W>
+// the feasible mask and the lowest (z, y, x) key (see k_c2r_z_argmin; OUT and
+// the rounding as there).
+template <typename T, int N, int R, int W, bool OUT>
 __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>* __restrict__ in, long long in_o,
                                                                       int nx, int ny, const OrientDesc* __restrict__ descs,
                                                                       T inv_total, StepDeviceResult* __restrict__ result,
@@ -258,21 +259,24 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
     T2* stage = lds;  // NH * SROW elements, reused by the FFT exchange afterwards
     const int lim = d.lx - x0;
     {
-        // all loads of this thread first (memory-level parallelism), then LDS
+        // all loads of this thread first (memory-level parallelism), then LDS;
+        // the block is a whole number of rows, so each thread keeps one column
+        // and steps KSTEP rows per load
         constexpr int BLOCK = (W / 2) * (N / R);
-        constexpr int ITER = (NH * W + BLOCK - 1) / BLOCK;
+        static_assert(BLOCK % W == 0, "block must cover whole rows");
+        constexpr int KSTEP = BLOCK / W;
+        constexpr int ITER = (NH + KSTEP - 1) / KSTEP;
+        const int c = threadIdx.x % W, k0 = threadIdx.x / W;
+        const bool col_in = c < lim;
+        const T2* p = src + k0 * plane + c;
+        const long long step = KSTEP * plane;
         T2 tmp[ITER];
 #pragma unroll
-        for (int j = 0; j < ITER; ++j) {
-            const int i = threadIdx.x + j * BLOCK;
-            const int k = i / W, c = i % W;
-            tmp[j] = (i < NH * W && c < lim) ? src[k * plane + c] : T2 { 0, 0 };
-        }
+        for (int j = 0; j < ITER; ++j)
+            tmp[j] = (k0 + j * KSTEP < NH && col_in) ? p[j * step] : T2 { 0, 0 };
 #pragma unroll
-        for (int j = 0; j < ITER; ++j) {
-            const int i = threadIdx.x + j * BLOCK;
-            if (i < NH * W) stage[(i / W) * SROW + (i % W)] = tmp[j];
-        }
+        for (int j = 0; j < ITER; ++j)
+            if (k0 + j * KSTEP < NH) stage[(k0 + j * KSTEP) * SROW + c] = tmp[j];
     }
     __syncthreads();
     T2 v[R];
@@ -291,7 +295,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
     unsigned long long mine = ~0ULL;
     float my_res = 0.f;
     unsigned int my_flags = 0;
-    int* fo = fields ? fields + field_offsets[o] : nullptr;
+    int* fo = OUT && fields ? fields + field_offsets[o] : nullptr;
     const bool any_excluded = excl[0] != 0;
     const bool regions_on = filt.act && filt.act[1];
 #pragma unroll
@@ -303,13 +307,15 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
             const int x = half ? xb : xa;
             if (x >= d.lx) continue;
             const T val = (half ? v[s].y : v[s].x) * inv_total;
-            const T rv = round(val);
+            const T rv = rint(val);
             const T res = fabs(val - rv);
             my_res = fmaxf(my_res, float(res));
             if (!(res < T(0.25))) my_flags |= 1u;
             if (rv < T(0) || rv > max_out) my_flags |= 2u;
-            if (fo) fo[x + (long long)d.lx * (y + (long long)d.ly * z)] = int(rv);
-            if (raw) raw[field_offsets[o] + x + (long long)d.lx * (y + (long long)d.ly * z)] = double(val);
+            if (OUT) {
+                if (fo) fo[x + (long long)d.lx * (y + (long long)d.ly * z)] = int(rv);
+                if (raw) raw[field_offsets[o] + x + (long long)d.lx * (y + (long long)d.ly * z)] = double(val);
+            }
             if (rv == T(0)) {
                 const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                                ((unsigned long long)x << 16) | (unsigned long long)o;

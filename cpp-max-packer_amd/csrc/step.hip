@@ -378,9 +378,14 @@ void rf_c2r_launch(int n, Tile tile, hipStream_t st, int lx_max, int ly_max, int
         using TT = decltype(tt);
         constexpr int N = TT::n, R = TT::r, W = TT::w;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
-        hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W>), dim3((lx_max + W - 1) / W, ly_max, orients),
-                           dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields, offsets,
-                           excl, raw, filt, tw);
+        if (fields || raw)
+            hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W, true>), dim3((lx_max + W - 1) / W, ly_max, orients),
+                               dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields,
+                               offsets, excl, raw, filt, tw);
+        else
+            hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W, false>), dim3((lx_max + W - 1) / W, ly_max, orients),
+                               dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields,
+                               offsets, excl, raw, filt, tw);
     });
     if (!ok) throw DeviceError { "no such FFT tile" };
 }
@@ -1036,11 +1041,19 @@ public:
             const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
             const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
                                     orient_best_.as<unsigned long long>() };
-            hipLaunchKernelGGL((k_c2r_z_argmin<T>), dim3((lmax_[0] + W - 1) / W, lmax_[1], slots), dim3(kThreads),
-                               lds_bytes<T>(nz, 2, W / 2), stream_, TT, to, nx, ny, dd, inv_total,
-                               result_.as<StepDeviceResult>(), fields_needed() ? fields_.as<int>() : nullptr,
-                               d_offsets_.as<long long>(), excl_.as<unsigned long long>(),
-                               keep_raw_ ? raw_.as<double>() : nullptr, filt, twiddles<T>(nz), plans_[2], W);
+            if (W < 2 || kThreads % W != 0) throw std::invalid_argument("inverse z tile width must divide the block");
+            int* fo = fields_needed() ? fields_.as<int>() : nullptr;
+            double* ro = keep_raw_ ? raw_.as<double>() : nullptr;
+            if (fo || ro)
+                hipLaunchKernelGGL((k_c2r_z_argmin<T, true>), dim3((lmax_[0] + W - 1) / W, lmax_[1], slots),
+                                   dim3(kThreads), lds_bytes<T>(nz, 2, W / 2), stream_, TT, to, nx, ny, dd, inv_total,
+                                   result_.as<StepDeviceResult>(), fo, d_offsets_.as<long long>(),
+                                   excl_.as<unsigned long long>(), ro, filt, twiddles<T>(nz), plans_[2], W);
+            else
+                hipLaunchKernelGGL((k_c2r_z_argmin<T, false>), dim3((lmax_[0] + W - 1) / W, lmax_[1], slots),
+                                   dim3(kThreads), lds_bytes<T>(nz, 2, W / 2), stream_, TT, to, nx, ny, dd, inv_total,
+                                   result_.as<StepDeviceResult>(), fo, d_offsets_.as<long long>(),
+                                   excl_.as<unsigned long long>(), ro, filt, twiddles<T>(nz), plans_[2], W);
         }
     }
 
