@@ -258,25 +258,45 @@ def train(proposal: Proposal, data, train_idx, epochs=6, batch=32, lr=1e-3, heat
     return history
 
 
-def run_episode(scene: Scene, proposal: Proposal | None = None, top_o=8, top_r=512, max_steps=100000):
+def run_episode(scene: Scene, proposal: Proposal | None = None, top_o=8, top_r=512, max_steps=100000,
+                verify_z=False):
     """Packs until nothing fits: the plain search, or proposals with the full
     search as fallback.  Every choice is stamped.  Returns parts, steps,
-    seconds and fallbacks; steps include the final one that finds nothing."""
+    seconds and fallbacks; steps include the final one that finds nothing.
+
+    verify_z: a proposal's choice is kept only when it lies at the lowest z
+    any step can still reach, else the step is the full search.  The lowest
+    key over every orientation never decreases (occupancy only grows), so the
+    z of the last full search's choice bounds every later choice from below;
+    a proposal at that z is at the plain rule's z.  The full search's choice
+    raises the bound."""
     s = scene.session
+    ez = scene.w.env_first[2]
     # warm-up outside the timed loop: tuning and graph capture of both modes
     scene.reset()
     if proposal is not None:
-        proposal.step(s, scene.desc, top_orientations=top_o, top_regions=top_r, stamp=True, fallback=False)
+        proposal.step(s, scene.desc, top_orientations=top_o, top_regions=top_r, stamp=not verify_z,
+                      fallback=False)
     s.step(stamp=True)
     scene.reset()
     parts = steps = fallbacks = 0
+    lb_z = -1
     t0 = time.perf_counter()
     while steps < max_steps:
         if proposal is None:
             r = s.step(stamp=True)
-        else:
+        elif not verify_z:
             r = proposal.step(s, scene.desc, top_orientations=top_o, top_regions=top_r, stamp=True, fallback=True)
             fallbacks += r["fallback"]
+        else:
+            r = proposal.step(s, scene.desc, top_orientations=top_o, top_regions=top_r, stamp=False, fallback=False)
+            if r["orientation"] >= 0 and r["key"][2] - ez <= lb_z:
+                s.stamp(r["orientation"], r["translation"])
+            else:
+                r = s.step(stamp=True)
+                fallbacks += 1
+                if r["orientation"] >= 0:
+                    lb_z = r["key"][2] - ez
         steps += 1
         if r["orientation"] < 0:
             break
