@@ -1126,10 +1126,14 @@ public:
             for (int pass : { int(kPassC2r), int(kPassMul), int(kPassInvY) }) {
                 const int n = pass_length(pass);
                 if (pinned_[pass]) continue;
+                // the inverse z pass pairs columns into complex transforms by
+                // their place in the tile, so its width sets the arithmetic too
+                if (fixed_radix_ && pass == kPassC2r) continue;
                 const bool rf = pass_uses_rf(pass);
                 char key[160];
-                std::snprintf(key, sizeof key, "p%d%s n%d,%d,%d f%d k%d,%d l%d,%d s%d", pass, rf ? "r" : "g", nx, ny,
-                              nz, precision_, kmax_[0], kmax_[1], lmax_[0], lmax_[1], nfit_);
+                std::snprintf(key, sizeof key, "p%d%s%s n%d,%d,%d f%d k%d,%d l%d,%d s%d", pass, rf ? "r" : "g",
+                              fixed_radix_ ? "d" : "", nx, ny, nz, precision_, kmax_[0], kmax_[1], lmax_[0], lmax_[1],
+                              nfit_);
                 auto it = ctx_.tiles.find(key);
                 if (it != ctx_.tiles.end()) {
                     tile_[pass] = Tile { it->second.first, it->second.second };
@@ -1140,6 +1144,8 @@ public:
                 Tile best_tile = rf ? default_tile(pass, n) : Tile { 0, generic_default_w(pass) };
                 for (const Tile t : rf ? tile_candidates(pass, n) : generic_candidates(pass)) {
                     if (rf && !tile_fits(pass, t)) continue;
+                    // the radix sets the arithmetic; the tile width does not
+                    if (rf && fixed_radix_ && t.r != default_tile(pass, n).r) continue;
                     run_pass<T>(pass, t, act_full_.as<int>(), nfit_);
                     if (hipGetLastError() != hipSuccess) continue;  // launch refused: not a candidate here
                     SP_HIP_CHECK(hipEventRecord(e0, stream_));
@@ -1569,6 +1575,9 @@ public:
         if (!rank_on_) throw std::invalid_argument("ranking not set");
         if (subset_) throw std::invalid_argument("ranking searches every orientation");
         prepare();
+        // spectral_pipeline.cpp:1735-1740: the mean proximity divides by the kernel's ones
+        for (int o = 0; o != O_; ++o)
+            if (descs_[o].lx > 0 && descs_[o].mass == 0) throw std::domain_error("SPECTRAL_PIPELINE_EMPTY_KERNEL");
         use_device(ctx_);
         if (incremental_) launch_incremental(false);
         else launch_mode(false, false);
@@ -1595,6 +1604,11 @@ public:
             p->set_kernel(std::uint32_t(o), k.cells.data(), sh, k.first);
         }
         p->keep_raw(true);
+        // unrounded values: every process must compute the same bits, so only
+        // the tile widths of the product and inverse y passes are tuned (the
+        // radix and the inverse z tile set the arithmetic; integer fields are
+        // indifferent to both)
+        p->fixed_radix_ = true;
         p->set_fast_path(use_rf_);
         p->set_autotune(autotune_);
         p->prepare();
@@ -1922,7 +1936,7 @@ private:
     std::vector<const char*> prof_names_;
     // occupancy revision (set_environment and every stamp) and ranking state
     std::uint64_t env_rev_ {}, rank_rev_ { ~0ull };
-    bool real_env_ {}, rank_on_ {};
+    bool real_env_ {}, rank_on_ {}, fixed_radix_ {};
     DeviceBuffer envd_;
     sp_ranking_desc rank_desc_ {};
     std::vector<double> rank_max_z_;

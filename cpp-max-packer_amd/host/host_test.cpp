@@ -3,7 +3,12 @@
 //   spectrapack_host_test place <input.txt>  — the placement loop on a scene
 //     read from whitespace-separated numbers; prints one line per proposal
 //     "P orientation tx ty tz verdict" and a final "S steps proposals accepted outcome".
+//   spectrapack_host_test rank <input.txt>   — the ranked loop on a scene (the
+//     same numbers, then origin_z cmin_z cmax_z, one max z per orientation,
+//     capacity and reject_every); prints "R orientation tx ty tz score_bits"
+//     per placement and a final "S placed validations outcome mass".
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <stop_token>
@@ -77,6 +82,76 @@ static int correlate_checks()
     return failures == 0 ? 0 : 1;
 }
 
+static bool read_scene(std::istream& in, sg::Environment& env, std::vector<sg::OrientedKernel>& kernels)
+{
+    for (auto& v : env.shape) in >> v;
+    for (auto& v : env.first) in >> v;
+    in >> env.pitch_mm;
+    env.occupied.resize(std::size_t(env.shape[0]) * env.shape[1] * env.shape[2]);
+    for (auto& c : env.occupied) {
+        int v;
+        in >> v;
+        c = std::uint8_t(v);
+    }
+    std::size_t n;
+    in >> n;
+    kernels.resize(n);
+    for (auto& k : kernels) {
+        for (auto& v : k.rotation_xyzw) in >> v;
+        for (auto& v : k.first) in >> v;
+        for (auto& v : k.shape) in >> v;
+        k.cells.resize(std::size_t(k.shape[0]) * k.shape[1] * k.shape[2]);
+        for (auto& c : k.cells) {
+            int v;
+            in >> v;
+            c = std::uint8_t(v);
+        }
+    }
+    return bool(in);
+}
+
+static int rank(const char* path)
+{
+    std::ifstream in(path);
+    sg::Environment env;
+    std::vector<sg::OrientedKernel> kernels;
+    if (!read_scene(in, env, kernels)) {
+        std::printf("bad input\n");
+        return 2;
+    }
+    sg::RankingRequest req;
+    in >> req.lattice_origin_z_mm >> req.container_min_z_mm >> req.container_max_z_mm;
+    std::vector<double> max_z(kernels.size());
+    for (auto& v : max_z) in >> v;
+    int reject_every;
+    in >> req.capacity >> reject_every;
+    if (!in) {
+        std::printf("bad input\n");
+        return 2;
+    }
+    std::uint64_t calls = 0;
+    std::vector<sg::RankedPlacement> placed;
+    std::vector<std::uint8_t> final_occ;
+    const auto stats = sg::rank_parts(
+        env, kernels, max_z, req, {},
+        [&](const sg::Proposal&) {
+            ++calls;
+            return !(reject_every > 0 && calls % reject_every == 0);
+        },
+        nullptr, &placed, &final_occ);
+    for (const auto& p : placed) {
+        unsigned long long bits;
+        std::memcpy(&bits, &p.score, sizeof bits);
+        std::printf("R %u %lld %lld %lld %016llx\n", p.orientation, (long long)p.translation[0],
+                    (long long)p.translation[1], (long long)p.translation[2], bits);
+    }
+    std::uint64_t mass = 0;
+    for (auto c : final_occ) mass += c;
+    std::printf("S %llu %llu %s %llu\n", (unsigned long long)stats.placed, (unsigned long long)stats.validations,
+                std::string(stats.outcome).c_str(), (unsigned long long)mass);
+    return 0;
+}
+
 static int place(const char* path)
 {
     std::ifstream in(path);
@@ -136,6 +211,7 @@ int main(int argc, char** argv)
 {
     if (argc >= 2 && std::string(argv[1]) == "correlate") return correlate_checks();
     if (argc >= 3 && std::string(argv[1]) == "place") return place(argv[2]);
-    std::printf("usage: %s correlate | place <input>\n", argv[0]);
+    if (argc >= 3 && std::string(argv[1]) == "rank") return rank(argv[2]);
+    std::printf("usage: %s correlate | place <input> | rank <input>\n", argv[0]);
     return 2;
 }

@@ -1,11 +1,18 @@
-// The plain greedy spectral placement loop on one MI355X — the role
-// engine/pack_solver/src/gap_fill.cpp:241-588 (fill_gaps) plays, recomputing
-// every orientation's feasible mask from the current occupancy each step
-// instead of patching it with blocked-offset correlations.  Each proposal is
-// the lowest kernel-window corner in (z, y, x) order over all orientations,
-// the earliest orientation winning ties (gap_fill.cpp:461-498); the caller's
-// verdict decides whether it is stamped (accepted), blocked for that
-// orientation (rejected) or ends the run (stop), as at gap_fill.cpp:512-523.
+// The greedy spectral placement loops on one MI355X.
+//
+// place_parts: the plain loop, the role engine/pack_solver/src/gap_fill.cpp:241-588
+// (fill_gaps) plays.  Each proposal is the lowest kernel-window corner in
+// (z, y, x) order over all orientations, the earliest orientation winning ties
+// (gap_fill.cpp:461-498); the caller's verdict decides whether it is stamped
+// (accepted), blocked for that orientation (rejected) or ends the run (stop),
+// as at gap_fill.cpp:512-523.  With `incremental` the overlap fields are
+// patched by the correlation of each placement's new cells, as fill_gaps does
+// with its blocked offsets; results are identical to recomputing them.
+//
+// rank_parts: the ranked loop of the spectral pipeline (spectral.cpp:1485-1590
+// over spectral_pipeline.cpp:1740-1857), for containers given as meshes: the
+// best page head by (.65 * top - .35 * mean proximity, orientation, z, y, x)
+// whose direct recheck finds no overlap and that `validate` accepts is placed.
 #pragma once
 
 #include <array>
@@ -69,6 +76,40 @@ PlacementStats place_parts(const Environment& environment, std::span<const Orien
                            const PlacementRequest& request, const runtime::OperationControl& control,
                            const std::function<GapVerdict(const Proposal&)>& accept,
                            compute::FftExecution* execution = nullptr, std::vector<std::uint8_t>* final_occupancy = nullptr);
+
+struct RankingRequest {
+    std::uint64_t capacity {};
+    int precision {};             // 0 auto (a-priori bound), 32 or 64: the binary overlaps
+    bool incremental { true };
+    double pair_clearance_mm {};
+    double lattice_origin_z_mm {};  // lattice.origin_mm[2]
+    double container_min_z_mm {};   // container bounds in z
+    double container_max_z_mm {};
+};
+
+struct RankedPlacement {
+    std::uint32_t orientation {};
+    std::array<double, 4> rotation_xyzw {};
+    compute::Index3 translation {};
+    std::array<double, 3> translation_mm {};
+    double score {};
+};
+
+struct RankingStats {
+    std::uint64_t placed {}, validations {};
+    std::string_view outcome { "SPECTRAL_NOT_RUN" };
+};
+
+// Places copies until capacity (SPECTRAL_COPY_LIMIT), no page has a candidate
+// (SPECTRAL_PIPELINE_COMPLETE), the control's stop/deadline (polled between
+// placements: SPECTRAL_INTERRUPTED) or a device failure (SPECTRAL_DEVICE).
+// object_max_z_mm: each orientation's oriented-bounds max z in mm.
+// validate(p) is the caller's exact check (spectral.cpp:1576 try_pose).
+RankingStats rank_parts(const Environment& environment, std::span<const OrientedKernel> kernels,
+                        std::span<const double> object_max_z_mm, const RankingRequest& request,
+                        const runtime::OperationControl& control, const std::function<bool(const Proposal&)>& validate,
+                        compute::FftExecution* execution = nullptr, std::vector<RankedPlacement>* placements = nullptr,
+                        std::vector<std::uint8_t>* final_occupancy = nullptr);
 
 // Rasterises a closed mesh (local frame) for every rotation on the device
 // (conservative_fields.cpp:919-1003).  Empty on failure.

@@ -1,4 +1,4 @@
-// Plain placement loop over the batched device step (see placement.hpp).
+// Placement loops over the batched device step (see placement.hpp).
 #include "spectrapack/solver/placement.hpp"
 
 #include <algorithm>
@@ -7,6 +7,67 @@
 #include "spectrapack_hip.h"
 
 namespace spectrapack::solver::gpu {
+
+namespace {
+
+// A device session over the environment and kernels; owns the execution when
+// the caller passes none.  `failure` names the step that failed.
+struct OpenSession {
+    std::unique_ptr<compute::FftExecution> owned;
+    sp_hip_session* s {};
+    const char* failure {};
+    OpenSession(const OpenSession&) = delete;
+    OpenSession& operator=(const OpenSession&) = delete;
+    OpenSession(const Environment& env, std::span<const OrientedKernel> kernels, int precision, bool incremental,
+                double pair_clearance_mm, compute::FftExecution* execution)
+    {
+        if (!execution) {
+            auto made = compute::FftExecution::open(0);
+            if (std::holds_alternative<compute::CorrelationFailure>(made)) {
+                failure = "device";
+                return;
+            }
+            owned = std::move(std::get<std::unique_ptr<compute::FftExecution>>(made));
+            execution = owned.get();
+        }
+        sp_session_desc desc {};
+        for (int a = 0; a != 3; ++a) {
+            desc.environment_shape[a] = env.shape[a];
+            desc.environment_first[a] = env.first[a];
+        }
+        desc.orientations = std::uint32_t(kernels.size());
+        desc.precision = precision;
+        desc.capture_graph = 1;
+        if (sp_hip_session_create(execution->device_context(), &desc, &s) != SP_OK) {
+            s = nullptr;
+            failure = "device";
+            return;
+        }
+        if (sp_hip_session_set_environment(s, env.occupied.data(), env.occupied.size()) != SP_OK ||
+            (pair_clearance_mm != 0.0 && sp_hip_session_set_pair_clearance(s, pair_clearance_mm, env.pitch_mm) != SP_OK)) {
+            failure = "field";
+            return;
+        }
+        if (incremental && sp_hip_session_set_incremental(s, 1) != SP_OK) {
+            failure = "device";
+            return;
+        }
+        for (std::uint32_t o = 0; o != kernels.size(); ++o) {
+            const auto& k = kernels[o];
+            const std::int64_t first[3] = { k.first[0], k.first[1], k.first[2] };
+            if (sp_hip_session_set_kernel(s, o, k.cells.data(), k.shape.data(), first) != SP_OK) {
+                failure = "field";
+                return;
+            }
+        }
+    }
+    ~OpenSession()
+    {
+        if (s) sp_hip_session_destroy(s);
+    }
+};
+
+}  // namespace
 
 PlacementStats place_parts(const Environment& env, std::span<const OrientedKernel> kernels,
                            const PlacementRequest& request, const runtime::OperationControl& control,
@@ -18,54 +79,12 @@ PlacementStats place_parts(const Environment& env, std::span<const OrientedKerne
         stats.outcome = "GAP_FILL_UNAVAILABLE";
         return stats;
     }
-    std::unique_ptr<compute::FftExecution> owned;
-    if (!execution) {
-        auto made = compute::FftExecution::open(0);
-        if (auto* f = std::get_if<compute::CorrelationFailure>(&made)) {
-            stats.outcome = "GAP_FILL_CORRELATION";
-            return stats;
-        }
-        owned = std::move(std::get<std::unique_ptr<compute::FftExecution>>(made));
-        execution = owned.get();
-    }
-    sp_session_desc desc {};
-    for (int a = 0; a != 3; ++a) {
-        desc.environment_shape[a] = env.shape[a];
-        desc.environment_first[a] = env.first[a];
-    }
-    desc.orientations = std::uint32_t(kernels.size());
-    desc.precision = request.precision;
-    desc.capture_graph = 1;
-    sp_hip_session* s = nullptr;
-    if (sp_hip_session_create(execution->device_context(), &desc, &s) != SP_OK) {
-        stats.outcome = "GAP_FILL_CORRELATION";
+    OpenSession open(env, kernels, request.precision, request.incremental, request.pair_clearance_mm, execution);
+    if (open.failure) {
+        stats.outcome = std::string_view(open.failure) == "field" ? "GAP_FILL_FIELD" : "GAP_FILL_CORRELATION";
         return stats;
     }
-    struct Closer {
-        sp_hip_session* s;
-        ~Closer() { sp_hip_session_destroy(s); }
-    } closer { s };
-    if (sp_hip_session_set_environment(s, env.occupied.data(), env.occupied.size()) != SP_OK) {
-        stats.outcome = "GAP_FILL_FIELD";
-        return stats;
-    }
-    if (request.pair_clearance_mm != 0.0 &&
-        sp_hip_session_set_pair_clearance(s, request.pair_clearance_mm, env.pitch_mm) != SP_OK) {
-        stats.outcome = "GAP_FILL_FIELD";
-        return stats;
-    }
-    if (request.incremental && sp_hip_session_set_incremental(s, 1) != SP_OK) {
-        stats.outcome = "GAP_FILL_CORRELATION";
-        return stats;
-    }
-    for (std::uint32_t o = 0; o != kernels.size(); ++o) {
-        const auto& k = kernels[o];
-        const std::int64_t first[3] = { k.first[0], k.first[1], k.first[2] };
-        if (sp_hip_session_set_kernel(s, o, k.cells.data(), k.shape.data(), first) != SP_OK) {
-            stats.outcome = "GAP_FILL_FIELD";
-            return stats;
-        }
-    }
+    sp_hip_session* s = open.s;
     control.phase(runtime::Phase::placing);
     std::uint64_t copies = 0;
     while (copies < request.capacity) {
@@ -113,6 +132,89 @@ PlacementStats place_parts(const Environment& env, std::span<const OrientedKerne
         }
     }
     if (copies >= request.capacity) stats.outcome = "GAP_FILL_CAPACITY";
+    if (final_occupancy) {
+        final_occupancy->resize(env.occupied.size());
+        (void)sp_hip_session_get_environment(s, final_occupancy->data(), final_occupancy->size());
+    }
+    return stats;
+}
+
+RankingStats rank_parts(const Environment& env, std::span<const OrientedKernel> kernels,
+                        std::span<const double> object_max_z_mm, const RankingRequest& request,
+                        const runtime::OperationControl& control, const std::function<bool(const Proposal&)>& validate,
+                        compute::FftExecution* execution, std::vector<RankedPlacement>* placements,
+                        std::vector<std::uint8_t>* final_occupancy)
+{
+    RankingStats stats;
+    if (kernels.empty() || object_max_z_mm.size() != kernels.size()) {
+        stats.outcome = "SPECTRAL_UNAVAILABLE";
+        return stats;
+    }
+    OpenSession open(env, kernels, request.precision, request.incremental, request.pair_clearance_mm, execution);
+    if (open.failure) {
+        stats.outcome = "SPECTRAL_DEVICE";
+        return stats;
+    }
+    sp_hip_session* s = open.s;
+    const sp_ranking_desc rank { request.lattice_origin_z_mm, env.pitch_mm, request.container_min_z_mm,
+                                 request.container_max_z_mm };
+    if (sp_hip_session_set_ranking(s, &rank, object_max_z_mm.data(), std::uint32_t(object_max_z_mm.size())) != SP_OK) {
+        stats.outcome = "SPECTRAL_PIPELINE_CONTAINER_HEIGHT";
+        return stats;
+    }
+    struct Hook {
+        const std::function<bool(const Proposal&)>* validate;
+        std::span<const OrientedKernel> kernels;
+        double pitch;
+        std::uint64_t calls;
+        static int call(void* user, std::uint32_t o, const std::int64_t t[3])
+        {
+            auto* h = static_cast<Hook*>(user);
+            ++h->calls;
+            if (!*h->validate) return 1;
+            Proposal p;
+            p.orientation = o;
+            p.rotation_xyzw = h->kernels[o].rotation_xyzw;
+            for (int a = 0; a != 3; ++a) {
+                p.translation[a] = t[a];
+                p.translation_mm[a] = h->pitch * static_cast<double>(t[a]);
+            }
+            return (*h->validate)(p) ? 1 : 0;
+        }
+    } hook { &validate, kernels, env.pitch_mm, 0 };
+    control.phase(runtime::Phase::placing);
+    stats.outcome = "SPECTRAL_COPY_LIMIT";
+    while (stats.placed < request.capacity) {
+        if (control.poll() != runtime::StopCause::none) {
+            stats.outcome = "SPECTRAL_INTERRUPTED";
+            break;
+        }
+        // one placement per call: the pages start over after every placement
+        sp_placement pl {};
+        std::uint32_t n = 0;
+        if (sp_hip_session_ranked_fill(s, 1, &Hook::call, &hook, &pl, 1, &n) != SP_OK) {
+            stats.outcome = "SPECTRAL_DEVICE";
+            break;
+        }
+        if (n == 0) {
+            stats.outcome = "SPECTRAL_PIPELINE_COMPLETE";
+            break;
+        }
+        ++stats.placed;
+        if (placements) {
+            RankedPlacement r;
+            r.orientation = pl.orientation;
+            r.rotation_xyzw = kernels[pl.orientation].rotation_xyzw;
+            for (int a = 0; a != 3; ++a) {
+                r.translation[a] = pl.translation[a];
+                r.translation_mm[a] = env.pitch_mm * static_cast<double>(pl.translation[a]);
+            }
+            r.score = pl.score;
+            placements->push_back(r);
+        }
+        control.progress("placing", stats.placed, request.capacity);
+    }
+    stats.validations = hook.calls;
     if (final_occupancy) {
         final_occupancy->resize(env.occupied.size());
         (void)sp_hip_session_get_environment(s, final_occupancy->data(), final_occupancy->size());

@@ -19,7 +19,7 @@ def test_cpp_correlation_dropin():
     assert r.returncode == 0, r.stdout + r.stderr
 
 
-def write_scene(path, env, first, pitch, kernels, capacity, reject_every, stop_after):
+def write_scene(path, env, first, pitch, kernels, capacity, reject_every, stop_after, tail=None):
     with open(path, "w") as f:
         f.write(" ".join(str(v) for v in env.shape[::-1]) + "\n")
         f.write(" ".join(str(int(v)) for v in first) + f" {pitch!r}\n")
@@ -29,7 +29,7 @@ def write_scene(path, env, first, pitch, kernels, capacity, reject_every, stop_a
             f.write(" ".join(repr(float(v)) for v in k["q"]) + " ")
             f.write(" ".join(str(int(v)) for v in k["first"]) + " " + " ".join(str(int(v)) for v in k["shape"]) + "\n")
             f.write(" ".join(str(int(v)) for v in k["cells"].ravel()) + "\n")
-        f.write(f"{capacity} {reject_every} {stop_after}\n")
+        f.write(tail if tail is not None else f"{capacity} {reject_every} {stop_after}\n")
 
 
 @pytest.mark.parametrize("dims,reject_every,stop_after,capacity", [
@@ -88,3 +88,50 @@ def test_cpp_placement_loop(tmp_path, oracle, dims, reject_every, stop_after, ca
     assert summary[4] == outcome
     assert int(summary[3]) == accepted
     assert int(summary[5]) == int(cur.sum())
+
+
+@pytest.mark.parametrize("reject_every,capacity", [(0, 1000), (3, 1000), (2, 6)])
+def test_cpp_ranked_loop(tmp_path, ctx, reject_every, capacity):
+    """rank_parts (the C++ mirror of the spectral pipeline's ranked loop) in a
+    sphere-mesh container makes the same placements, with the same score bits,
+    as the session's native ranked loop under the same validation verdicts."""
+    import spectrapack_amd as sp
+    from spectrapack_amd import workloads as W
+    sph = W.sphere(14.0, n_seg=24, n_lat=11)
+    first, shape = W.mesh_window(sph, 1.0)
+    env = ctx.mesh_container(sph.vertices, sph.triangles, first, shape, 1.0)
+    part = W.star(7, r_min=1.5, r_max=3.5)
+    cat = W.cube_catalog()[:6]
+    raw = ctx.voxelize_objects(part.vertices, part.triangles, cat, 1.0)
+    kernels = [dict(q=q, first=k["first"], shape=k["shape"], cells=k["cells"]) for q, k in zip(cat, raw)]
+    lo, hi = sph.bounds()
+    max_z = W.oriented_max_z(part, cat)
+    tail = f"{0.0!r} {float(lo[2])!r} {float(hi[2])!r}\n" + " ".join(repr(float(v)) for v in max_z) + \
+        f"\n{capacity} {reject_every}\n"
+    path = tmp_path / "scene.txt"
+    write_scene(path, env, first, 1.0, kernels, 0, 0, 0, tail=tail)
+    print(f"[host] rank reject_every={reject_every} capacity={capacity}", flush=True)
+    r = subprocess.run([DRIVER, "rank", str(path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    got = [(int(l.split()[1]), tuple(int(v) for v in l.split()[2:5]), l.split()[5]) for l in lines
+           if l.startswith("R ")]
+    summary = [l for l in lines if l.startswith("S ")][0].split()
+    s = sp.Session(ctx, shape, first, len(kernels))
+    s.set_environment(env)
+    for i, k in enumerate(kernels):
+        s.set_kernel(i, k["cells"], k["shape"], k["first"])
+    s.set_incremental(True)
+    s.set_ranking(0.0, 1.0, float(lo[2]), float(hi[2]), max_z)
+    calls = [0]
+
+    def validate(o, t):
+        calls[0] += 1
+        return not (reject_every and calls[0] % reject_every == 0)
+
+    want = s.ranked_fill(max_copies=capacity, validate=validate)
+    assert len(want) >= min(capacity, 8)
+    assert got == [(o, tuple(t), f"{np.float64(v).view(np.uint64):016x}") for o, t, v in want]
+    assert int(summary[1]) == len(want) and int(summary[2]) == calls[0]
+    assert summary[3] == ("SPECTRAL_COPY_LIMIT" if len(want) == capacity else "SPECTRAL_PIPELINE_COMPLETE")
+    assert int(summary[4]) == int(s.environment().sum())

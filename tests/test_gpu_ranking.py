@@ -237,6 +237,15 @@ def test_ranking_rejects_large_grids_and_bad_heights(ctx):
         s.set_ranking(0.0, 1.0, 2.0, 2.0, [1.0])
     with pytest.raises(sp.Failure):
         s.set_ranking(0.0, 1.0, 0.0, 4.0, [1.0, 2.0])
+    # an orientation whose kernel has no occupied cell (spectral_pipeline.cpp:1735-1740)
+    s = sp.Session(ctx, (8, 8, 8), (0, 0, 0), 2)
+    s.set_environment(np.zeros((8, 8, 8), np.uint8))
+    s.set_kernel(0, np.ones((2, 2, 2), np.uint8), (2, 2, 2), (0, 0, 0))
+    s.set_kernel(1, np.zeros((2, 2, 2), np.uint8), (2, 2, 2), (0, 0, 0))
+    s.set_ranking(0.0, 1.0, 0.0, 8.0, [1.0, 1.0])
+    with pytest.raises(sp.Failure) as e:
+        s.ranked_pages()
+    assert "EMPTY_KERNEL" in str(e.value)
 
 
 def test_pages_at_full_size_bracket(ctx):
