@@ -320,6 +320,18 @@ def main():
         log("[bench] CPU baseline sample ...")
         out["cpu_baseline"] = cpu_baseline(w, env, kernels, args.cpu_threads, max(1, args.cpu_threads))
         out["cpu_baseline"]["host"] = platform.processor() or platform.machine()
+        # the same restatement with orientations in parallel over the host's
+        # cores (at most 16, this GPU's share of the host)
+        # each correlation holds about 40 bytes per padded cell (two FP64
+        # complex buffers and the output); keep the parallel sample under 64 GB
+        kmax = [max(k["shape"][a] for k in kernels) for a in range(3)]
+        padded = 1
+        for a in range(3):
+            padded *= w.env_shape[a] + kmax[a] - 1
+        n_all = max(1, min(16, os.cpu_count() or 1, int(64e9 // (40 * padded))))
+        if n_all > 1 and args.cpu_threads == 1:
+            log(f"[bench] CPU baseline on {n_all} cores ...")
+            out["cpu_baseline_all_cores"] = cpu_baseline(w, env, kernels, n_all, n_all)
     print(json.dumps(out), flush=True)
 
 
@@ -354,6 +366,7 @@ def ranked_timing(sp, ctx, w, env, kernels, prec, args):
     e1.record(stream)
     e1.synchronize()
     ms_rank = e0.elapsed_time(e1) / reps
+    pages = s.ranked_pages()   # the binding's first call sets up its numpy/ctypes views (tens of ms, once)
     t0 = time.perf_counter()
     for _ in range(reps):
         pages = s.ranked_pages()

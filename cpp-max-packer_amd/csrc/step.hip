@@ -678,7 +678,9 @@ public:
     }
 
     // ---- incremental mode ------------------------------------------------
-    bool fields_needed() const { return keep_fields_ || incremental_; }
+    // Incremental steps keep the fields unless the update window turned out
+    // to have no benefit (then steps stay full and need none).
+    bool fields_needed() const { return keep_fields_ || (incremental_ && (!inc_setup_done_ || inc_effective_)); }
 
     void set_incremental(bool on)
     {

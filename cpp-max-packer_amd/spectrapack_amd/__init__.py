@@ -68,6 +68,11 @@ class RankedCandidate(C.Structure):
                 ("orientation", C.c_uint32), ("direct_zero_overlap", C.c_uint32)]
 
 
+RANKED_DTYPE = np.dtype([("translation", np.int64, 3), ("score", np.float64), ("binary_overlap", np.float64),
+                         ("orientation", np.uint32), ("direct_zero_overlap", np.uint32)])
+assert RANKED_DTYPE.itemsize == C.sizeof(RankedCandidate)
+
+
 class Placement(C.Structure):
     _fields_ = [("translation", C.c_int64 * 3), ("orientation", C.c_uint32), ("reserved", C.c_uint32),
                 ("score", C.c_double)]
@@ -450,16 +455,20 @@ class Session:
                 cur[i] = RankedCandidate((C.c_int64 * 3)(*[int(v) for v in c["translation"]]), float(c["score"]),
                                          float(c["binary_overlap"]), C.c_uint32(o), C.c_uint32(0))
                 has[i] = 1
-        pages = (RankedCandidate * (n * PAGE_SIZE))()
+        pages = np.zeros(n * PAGE_SIZE, RANKED_DTYPE)
         counts = np.zeros(n, np.uint32)
         self._check(lib().sp_hip_session_ranked_pages(self._h, C.c_int(mode), _p(oa, C.c_uint32), C.c_uint32(n), cur,
-                                                      _p(has, C.c_uint8), pages, _p(counts, C.c_uint32)))
+                                                      _p(has, C.c_uint8), pages.ctypes.data_as(C.c_void_p),
+                                                      _p(counts, C.c_uint32)))
         out = {}
+        tr = pages["translation"].tolist()
+        sc = pages["score"].tolist()
+        ov = pages["binary_overlap"].tolist()
+        dz = pages["direct_zero_overlap"].tolist()
         for i, o in enumerate(ol):
-            out[o] = [dict(translation=tuple(pages[i * PAGE_SIZE + j].translation), score=pages[i * PAGE_SIZE + j].score,
-                           binary_overlap=pages[i * PAGE_SIZE + j].binary_overlap, orientation=o,
-                           direct_zero_overlap=bool(pages[i * PAGE_SIZE + j].direct_zero_overlap))
-                      for j in range(int(counts[i]))]
+            b = i * PAGE_SIZE
+            out[o] = [dict(translation=tuple(tr[b + j]), score=sc[b + j], binary_overlap=ov[b + j], orientation=o,
+                           direct_zero_overlap=bool(dz[b + j])) for j in range(int(counts[i]))]
         return out
 
     def ranked_fill(self, max_copies=1 << 20, validate=None):
