@@ -303,3 +303,25 @@ def test_pages_at_full_size_bracket(ctx):
         ka = (a["score"], a["translation"][2], a["translation"][1], a["translation"][0])
         kb = (b["score"], b["translation"][2], b["translation"][1], b["translation"][0])
         assert ka < kb
+
+
+def test_page_queries_reject_bad_arguments(ctx):
+    import spectrapack_amd as sp
+    first, env, kernels = random_scene(8, shape=(16, 18, 20), n=3, kmax=4)
+    s = make_session(ctx, env, first, kernels)
+    set_ranking(s, env, first, kernels)
+    pages = s.ranked_pages()
+    c = dict(pages[0][0])
+    with pytest.raises(sp.Failure):
+        s.ranked_pages(orientations=[0, 0])                     # listed twice
+    with pytest.raises(sp.Failure):
+        s.ranked_pages(orientations=[7])                        # no such orientation
+    with pytest.raises(sp.Failure):
+        s.ranked_pages(mode=2)                                  # no such mode
+    with pytest.raises(sp.Failure):
+        s.ranked_pages(orientations=[0], cursors={0: dict(c, translation=(10 ** 6, 0, 0))})
+    with pytest.raises(sp.Failure):
+        s.ranked_pages(orientations=[0], mode=1, cursors={0: dict(c, binary_overlap=0.5)})
+    # the session still answers correctly afterwards
+    again = s.ranked_pages()
+    assert [p["translation"] for p in again[0]] == [p["translation"] for p in pages[0]]
