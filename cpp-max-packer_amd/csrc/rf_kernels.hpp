@@ -15,6 +15,24 @@ template <typename T> __device__ __forceinline__ void load_tw_lds(cx<T>* dst, co
 
 // Strided pencils (consecutive pencils contiguous in memory).  Same arguments
 // as k_c2c.  Lane mapping: pencil = lane % W (fastest), t = lane / W.
+// Streaming accesses: data read or written once per pass, not reused by the
+// same kernel, bypass cache allocation.
+template <typename T2> __device__ __forceinline__ T2 nt_load(const T2* p)
+{
+    using E = decltype(p->x);
+    using V = E __attribute__((ext_vector_type(2)));
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+    return T2 { v.x, v.y };
+}
+
+template <typename T2> __device__ __forceinline__ void nt_store(T2* p, T2 v)
+{
+    using E = decltype(v.x);
+    using V = E __attribute__((ext_vector_type(2)));
+    const V w = { v.x, v.y };
+    __builtin_nontemporal_store(w, reinterpret_cast<V*>(p));
+}
+
 template <typename T, int N, int R, int W>
 __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long in_o, long long in_sn, long long in_sw,
                                                       long long in_sb, cx<T>* out, long long out_o, long long out_sn,
@@ -54,7 +72,7 @@ __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long 
 #pragma unroll
     for (int q = 0; q < R; ++q) {
         const int n = t + P * q;
-        v[q] = (live && n < ni) ? src[n * in_sn] : T2 { 0, 0 };
+        v[q] = (live && n < ni) ? nt_load(&src[n * in_sn]) : T2 { 0, 0 };
     }
     __syncthreads();
     reg_fft<T, N, R>(v, lds + w * S::stride, tws, t, fwd != 0);
@@ -62,7 +80,7 @@ __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long 
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             const int n = t + P * s;
-            if (n < no) dst[n * out_sn] = v[s];
+            if (n < no) nt_store(&dst[n * out_sn], v[s]);
         }
     }
 }
@@ -131,7 +149,7 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int x = t + P * q;
-            v[q] = (live && x < kx) ? hrow[x] : T2 { 0, 0 };
+            v[q] = (live && x < kx) ? nt_load(&hrow[x]) : T2 { 0, 0 };
         }
         reg_fft<T, N, R>(v, pl, tws, t, true);
 #pragma unroll
@@ -142,7 +160,7 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
 #pragma unroll
             for (int s = 0; s < R; ++s) {
                 const int x = t + P * s;
-                if (x < lx) orow[x] = v[s];
+                if (x < lx) nt_store(&orow[x], v[s]);
             }
         }
     }
@@ -273,7 +291,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
         T2 tmp[ITER];
 #pragma unroll
         for (int j = 0; j < ITER; ++j)
-            tmp[j] = (k0 + j * KSTEP < NH && col_in) ? p[j * step] : T2 { 0, 0 };
+            tmp[j] = (k0 + j * KSTEP < NH && col_in) ? nt_load(&p[j * step]) : T2 { 0, 0 };
 #pragma unroll
         for (int j = 0; j < ITER; ++j)
             if (k0 + j * KSTEP < NH) stage[(k0 + j * KSTEP) * SROW + c] = tmp[j];
