@@ -15,24 +15,6 @@ template <typename T> __device__ __forceinline__ void load_tw_lds(cx<T>* dst, co
 
 // Strided pencils (consecutive pencils contiguous in memory).  Same arguments
 // as k_c2c.  Lane mapping: pencil = lane % W (fastest), t = lane / W.
-// Streaming accesses: data read or written once per pass, not reused by the
-// same kernel, bypass cache allocation.
-template <typename T2> __device__ __forceinline__ T2 nt_load(const T2* p)
-{
-    using E = decltype(p->x);
-    using V = E __attribute__((ext_vector_type(2)));
-    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
-    return T2 { v.x, v.y };
-}
-
-template <typename T2> __device__ __forceinline__ void nt_store(T2* p, T2 v)
-{
-    using E = decltype(v.x);
-    using V = E __attribute__((ext_vector_type(2)));
-    const V w = { v.x, v.y };
-    __builtin_nontemporal_store(w, reinterpret_cast<V*>(p));
-}
-
 template <typename T, int N, int R, int W>
 __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long in_o, long long in_sn, long long in_sw,
                                                       long long in_sb, cx<T>* out, long long out_o, long long out_sn,

@@ -20,6 +20,25 @@ template <> struct Cx<double> {
 };
 template <typename T> using cx = typename Cx<T>::type;
 
+// Streaming accesses: data read or written once per pass, not reused by the
+// same kernel, bypass cache allocation.
+template <typename T2> __device__ __forceinline__ T2 nt_load(const T2* p)
+{
+    using E = decltype(p->x);
+    using V = E __attribute__((ext_vector_type(2)));
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+    return T2 { v.x, v.y };
+}
+
+template <typename T2> __device__ __forceinline__ void nt_store(T2* p, T2 v)
+{
+    using E = decltype(v.x);
+    using V = E __attribute__((ext_vector_type(2)));
+    const V w = { v.x, v.y };
+    __builtin_nontemporal_store(w, reinterpret_cast<V*>(p));
+}
+
+
 template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b)
 {
     return { a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x };
