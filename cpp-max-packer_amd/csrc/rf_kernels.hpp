@@ -273,7 +273,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
         T2 tmp[ITER];
 #pragma unroll
         for (int j = 0; j < ITER; ++j)
-            tmp[j] = (k0 + j * KSTEP < NH && col_in) ? nt_load(&p[j * step]) : T2 { 0, 0 };
+            tmp[j] = (k0 + j * KSTEP < NH && col_in) ? p[j * step] : T2 { 0, 0 };
 #pragma unroll
         for (int j = 0; j < ITER; ++j)
             if (k0 + j * KSTEP < NH) stage[(k0 + j * KSTEP) * SROW + c] = tmp[j];
