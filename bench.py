@@ -59,6 +59,9 @@ def kernel_bytes(name, n, kernels, ext, elem):
         return sum(nh * (ny + ey - k["shape"][1] + 1) * (ex - k["shape"][0] + 1) * elem for k in fit)
     if name == "c2r_z_argmin":
         return sum(nh * (ey - k["shape"][1] + 1) * (ex - k["shape"][0] + 1) * elem for k in fit)
+    if name == "yz_argmin":
+        # fused inverse y + z: every kz line of every legal x column, read once
+        return sum(nh * ny * (ex - k["shape"][0] + 1) * elem for k in fit)
     return 0
 
 
@@ -113,6 +116,8 @@ def main():
     ap.add_argument("--no-deviation", action="store_true", help="skip the FP32-vs-FP64 deviation step")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal packing comparison")
     ap.add_argument("--no-incremental", action="store_true", help="skip the incremental-step timing")
+    ap.add_argument("--separate-passes", action="store_true",
+                    help="run the inverse y and inverse z passes separately (no fused pass)")
     ap.add_argument("--learned-seeds", default="9000", help="held-out part seeds packed with and without proposals")
     ap.add_argument("--top-o", type=int, default=8)
     ap.add_argument("--top-r", type=int, default=512)
@@ -169,6 +174,8 @@ def main():
         return
     s = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=args.precision or
                    (32 if w.precision == "fp32" else 0), capture_graph=not args.no_graph)
+    if args.separate_passes:
+        s.set_fused_passes(False)
     s.set_environment(env)
     for i, k in enumerate(kernels):
         s.set_kernel(i, k["cells"], k["shape"], k["first"])
@@ -294,6 +301,7 @@ def main():
         "config": {"workload": w.name, "grid": list(w.env_shape), "fft_grid": list(n), "orientations": len(kernels),
                    "pitch_mm": w.pitch, "precision": f"fp{prec}", "error_bound": r["error_bound"],
                    "graph": not args.no_graph, "parallelism": "single GPU", "kernel_spectra": "resident",
+                   "fused_inverse_yz": s.fused_passes(),
                    "autotuned_tiles": tiles},
         "ms_per_step_kernel_spectra_recomputed": ms_step_recompute,
         "incremental": incremental,

@@ -6,6 +6,8 @@
 #include "fft_kernels.hpp"
 #include "sp_regfft.hpp"
 
+#include <type_traits>
+
 namespace sphip {
 
 template <typename T> __device__ __forceinline__ void load_tw_lds(cx<T>* dst, const cx<T>* tw, int n)
@@ -99,7 +101,10 @@ __global__ void __launch_bounds__(W*(N / R)) rf_rows(cx<T>* data, int rows, long
 // for every orientation finishes the kernel's x transform from its partially
 // transformed row H[kz][y][x < kx], multiplies by the conjugate, runs the
 // inverse x transform and stores x < lx into out[o][kz][y][x].
-template <typename T, int N, int R, int W>
+// XT: the output is stored column-major instead, out[o][x][kz][y] (x stride
+// gridDim.y * ny), for the fused inverse y and z pass; lanes then take the
+// pencil fastest so that every store covers runs of W consecutive y.
+template <typename T, int N, int R, int W, bool XT = false>
 __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px,
                                                        const cx<T>* __restrict__ SB, int ny, cx<T>* __restrict__ out,
                                                        long long out_o, const OrientDesc* __restrict__ descs,
@@ -111,7 +116,8 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
     extern __shared__ unsigned char smem_raw[];
     T2* tws = reinterpret_cast<T2*>(smem_raw);
     T2* lds = tws + N;
-    const int t = threadIdx.x % P, w = threadIdx.x / P;
+    const int t = XT ? int(threadIdx.x) / W : int(threadIdx.x) % P;
+    const int w = XT ? int(threadIdx.x) % W : int(threadIdx.x) / P;
     const int kz = blockIdx.y;
     const int y = blockIdx.x * W + w;
     const bool live = y < ny;
@@ -137,7 +143,16 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
 #pragma unroll
         for (int s = 0; s < R; ++s) v[s] = cmulc(sb[s], v[s]);
         reg_fft<T, N, R>(v, pl, tws, t, false);
-        if (live) {
+        if (live && XT) {
+            T2* ocol = out + o * out_o + row;
+            const long long xs = (long long)gridDim.y * ny;
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                const int x = t + P * s;
+                if (x < lx) nt_store(&ocol[x * xs], v[s]);
+            }
+        }
+        else if (live) {
             T2* orow = out + o * out_o + row * N;
 #pragma unroll
             for (int s = 0; s < R; ++s) {
@@ -325,6 +340,193 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
         }
     }
     // wave-level reduction, then one LDS atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long om = __shfl_down(mine, off, 64);
+        mine = om < mine ? om : mine;
+        my_res = fmaxf(my_res, __shfl_down(my_res, off, 64));
+        my_flags |= __shfl_down(my_flags, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (mine != ~0ULL) atomicMin(&best, mine);
+        if (my_res > 0.f) atomicMax(&res_bits, __float_as_uint(my_res));
+        if (my_flags) atomicOr(&flags, my_flags);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (best != ~0ULL) {
+            atomicMin(&result->key, best);
+            if (filt.orient_best) atomicMin(&filt.orient_best[o], best);
+        }
+        if (res_bits) atomicMax(&result->max_residual_bits, res_bits);
+        if (flags) atomicOr(&result->flags, flags);
+    }
+}
+
+
+// Fused inverse y + C2R along z + checks + argmin for N = ny = nz = 256 at
+// radix 16.  One block of 1024 threads owns one x column of one orientation:
+// the column's half spectrum in[o][x][kz][y] (129 x 256 values, written by
+// rf_mul_x<..., true>) is read once, transformed along y in registers, and
+// transformed along z in two chunks of up to 128 legal rows through LDS, so
+// the inverse y output never goes back to memory.
+//  * y pass: 64 groups of 16 threads, two pencils per group (kz = g and
+//    g + 64).  The kz = 0 and kz = 128 planes of a real correlation are
+//    Hermitian in y, so their columns transform together as one complex
+//    pencil (kz0 + i kz128): the real outputs come back as the real and
+//    imaginary parts.  Thread t keeps Y[kz][y = t + 16 s] for s < 16.
+//  * z pass: the rows of a chunk are staged as stage[kz][row] (line 0 keeps
+//    the packed kz0 + i kz128 values); group g takes
+//    rows (2g, 2g + 1) of the chunk as one complex pencil through the
+//    Hermitian extension (as rf_c2r_z_argmin does for column pairs), and the
+//    real and imaginary outputs are the two rows' values at z = t + 16 s.
+// Rounding, the per-value checks, the feasible mask and the key are those of
+// rf_c2r_z_argmin.
+constexpr int kYzThreads = 1024;
+constexpr int kYzStageRow = 132;  // complex values per staged kz line (128 rows + pad)
+
+template <typename T> constexpr std::size_t yz_lds_bytes()
+{
+    constexpr std::size_t fft = std::size_t(64) * RegShape<256, 16>::stride;
+    constexpr std::size_t stage = std::size_t(128) * kYzStageRow;
+    return sizeof(cx<T>) * (256 + (fft > stage ? fft : stage));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restrict__ in, long long in_o,
+                                                            const OrientDesc* __restrict__ descs, T inv_total,
+                                                            StepDeviceResult* __restrict__ result,
+                                                            const unsigned long long* __restrict__ excl, StepFilter filt,
+                                                            const cx<T>* __restrict__ tw)
+{
+    using T2 = cx<T>;
+    constexpr int N = 256, R = 16, P = 16, NH = N / 2 + 1, G = kYzThreads / P;
+    using S = RegShape<N, R>;
+    static_assert(S::P == P && G == 64, "256-point pencils of 16 threads");
+    static_assert(std::is_same_v<T, float>, "the fused pass runs in FP32 (staged pairs are read as float4)");
+    extern __shared__ unsigned char smem_raw[];
+    T2* tws = reinterpret_cast<T2*>(smem_raw);
+    T2* lds = tws + N;
+    __shared__ unsigned long long best;
+    __shared__ unsigned int res_bits, flags;
+    const int o = slot_orientation(filt.act);
+    if (o < 0) return;
+    const OrientDesc d = descs[o];
+    const int x = blockIdx.x;
+    if (d.lx <= 0 || d.ly <= 0 || d.lz <= 0 || x >= d.lx) return;
+    if (threadIdx.x == 0) {
+        best = ~0ULL;
+        res_bits = 0;
+        flags = 0;
+    }
+    const int t = threadIdx.x % P, g = threadIdx.x / P;
+    load_tw_lds<T>(tws, tw, N);
+    const T2* col = in + o * in_o + (long long)x * NH * N;
+    // y pass: both pencils' loads first
+    T2 a[R], b[R];
+    const T2* p0 = col + (long long)g * N + t;
+    const T2* p1 = col + (long long)(g + G) * N + t;
+#pragma unroll
+    for (int q = 0; q < R; ++q) a[q] = nt_load(&p0[P * q]);
+    if (g == 0) {
+        const T2* pn = col + (long long)(N / 2) * N + t;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const T2 c = nt_load(&pn[P * q]);
+            a[q] = T2 { a[q].x - c.y, a[q].y + c.x };
+        }
+    }
+    __syncthreads();
+    T2* pen = lds + g * S::stride;
+    reg_fft_lean<T, N, R>(a, pen, tws, t, false);
+    // the second pencil is loaded only now: hoisting its loads above the first
+    // transform would hold both pencils and that transform's working set at once
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int q = 0; q < R; ++q) b[q] = nt_load(&p1[P * q]);
+    reg_fft_lean<T, N, R>(b, pen, tws, t, false);
+    const T max_out = T(d.mass < (long long)result->env_mass ? d.mass : (long long)result->env_mass);
+    const bool any_excluded = excl[0] != 0;
+    const bool regions_on = filt.act && filt.act[1];
+    unsigned long long mine = ~0ULL;
+    float my_res = 0.f;
+    unsigned int my_flags = 0;
+    T2* stage = lds;
+    unsigned int zero_bits[2] = { 0u, 0u };
+    // chunks of rows [128 c, 128 c + 128); the loop is unrolled so that the
+    // register arrays stay statically indexed
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int row0 = 128 * c;
+        if (row0 >= d.ly) break;  // uniform over the block
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int r = t + P * s;  // row within the chunk: y = row0 + r
+            stage[g * kYzStageRow + r] = a[8 * c + s];
+            stage[(g + G) * kYzStageRow + r] = b[8 * c + s];
+        }
+        __syncthreads();
+        const int y1 = row0 + 2 * g;
+        T2 v[R];
+        // line 0 holds (kz0 + i kz128) of both rows: k = 0 takes the real
+        // parts and k = 128 the imaginary parts (thread 0, q = 0 and 8)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+#pragma unroll
+            for (int q = 4 * h; q < 4 * h + 4; ++q) {
+                const int k = t + P * q;
+                const bool mirror = k > N / 2;
+                const int kk = (k == N / 2) ? 0 : mirror ? N - k : k;
+                const float4 ab = *reinterpret_cast<const float4*>(&stage[kk * kYzStageRow + 2 * g]);
+                const T2 u { ab.x, ab.y }, w { ab.z, ab.w };
+                T2 e = mirror ? T2 { u.x + w.y, w.x - u.y } : T2 { u.x - w.y, u.y + w.x };
+                if (q == 0 && t == 0) e = T2 { u.x, w.x };
+                if (q == 8 && t == 0) e = T2 { u.y, w.y };
+                v[q] = e;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        reg_fft_lean<T, N, R>(v, pen, tws, t, false);
+        // checks over every legal value; zero counts are marked in bit
+        // 2 s + half, which orders them as their keys (z, then y)
+        unsigned int zeros = 0;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            const bool z_in = t + P * s < d.lz;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const bool in = z_in && y1 + half < d.ly;
+                const T val = (half ? v[s].y : v[s].x) * inv_total;
+                const T rv = rint(val);
+                const T res = fabs(val - rv);
+                // branch-free: out-of-range values contribute nothing
+                my_res = fmaxf(my_res, in ? float(res) : 0.f);
+                my_flags |= (in && !(res < T(0.25)) ? 1u : 0u) | (in && (rv < T(0) || rv > max_out) ? 2u : 0u);
+                zeros |= (in && rv == T(0) ? 1u : 0u) << (2 * s + half);
+            }
+        }
+        zero_bits[c] = zeros;
+    }
+    // this thread's lowest feasible key per chunk: the first marked value
+    // that is neither rejected nor outside the allowed regions (scanned after
+    // the transforms, when no spectrum is held any more)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        unsigned int zeros = zero_bits[c];
+        const int y1 = 128 * c + 2 * g;
+        while (zeros) {
+            const int i = __builtin_ctz(zeros);
+            const int z = t + P * (i >> 1), y = y1 + (i & 1);
+            const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
+                                           ((unsigned long long)x << 16) | (unsigned long long)o;
+            if ((!any_excluded || !excluded(excl, key)) && (!regions_on || region_allowed(filt, x, y, z))) {
+                mine = key < mine ? key : mine;
+                break;
+            }
+            zeros &= zeros - 1;
+        }
+    }
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long om = __shfl_down(mine, off, 64);
         mine = om < mine ? om : mine;

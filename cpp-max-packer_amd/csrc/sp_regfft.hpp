@@ -100,4 +100,39 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
     }
 }
 
+// reg_fft for N = R * R (one exchange) with the exchange reads and their
+// twiddles issued four at a time, so that a kernel holding other data in
+// registers keeps the transform's working set small.  Same arithmetic as
+// reg_fft.
+template <typename T, int N, int R>
+__device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tw, int t, bool fwd)
+{
+    using T2 = cx<T>;
+    using S = RegShape<N, R>;
+    static_assert(S::passes == 2, "one exchange");
+    // a fresh copy of t per call, so that the twiddle addresses (t q) are
+    // recomputed here rather than kept live across calls
+    asm volatile("" : "+v"(t));
+    dft_reg<T, R>(v, fwd);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < R; ++s) lds[S::pad(t * R + s)] = v[s];
+    __syncthreads();
+    // second pass: l = R, m = 1, j = t, k = 0
+#pragma unroll
+    for (int q0 = 0; q0 < R; q0 += 4) {
+#pragma unroll
+        for (int q = q0; q < q0 + 4; ++q) {
+            v[q] = lds[S::pad(q * R + t)];
+            if (q > 0) {
+                T2 w = tw[(t * q) & (N - 1)];
+                if (!fwd) w.y = -w.y;
+                v[q] = cmul(v[q], w);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    dft_reg<T, R>(v, fwd);
+}
+
 }  // namespace sphip

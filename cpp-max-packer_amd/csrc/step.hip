@@ -348,6 +348,24 @@ void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>
     if (!ok) throw DeviceError { "no such FFT tile" };
 }
 
+// P1 with the column-major output of the fused inverse y and z pass (FP32,
+// N = 256, radix 16; the tile's width is kept when it is one of 8, 16, 32).
+template <typename T>
+void rf_mul_xt_launch(Tile tile, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px,
+                      const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
+                      const cx<T>* tw)
+{
+    constexpr int N = 256, R = 16;
+    auto go = [&](auto ww) {
+        constexpr int W = decltype(ww)::value;
+        hipLaunchKernelGGL((rf_mul_x<T, N, R, W, true>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)),
+                           (rf_lds<T, N, R>(W)), st, H, h_o, h_px, SB, ny, out, out_o, descs, act, tw);
+    };
+    if (tile.r == R && tile.w == 8) go(std::integral_constant<int, 8> {});
+    else if (tile.r == R && tile.w == 32) go(std::integral_constant<int, 32> {});
+    else go(std::integral_constant<int, 16> {});
+}
+
 template <typename T, typename In>
 void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batches, const In* in, int in_px, int in_py,
                    int ex, int ey, int ez, cx<T>* out, long long out_bs, int out_px, int out_py, int cx_, int cy_,
@@ -1000,6 +1018,12 @@ public:
         const long long h2o = (long long)nh * ny * kmax_[0];
         const long long to = (long long)nh * ny * nx;
         if (pass == kPassMul) {
+            if constexpr (std::is_same_v<T, float>) {
+                if (fused_yz()) {
+                    rf_mul_xt_launch<T>(tile, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twiddles<T>(nx));
+                    return;
+                }
+            }
             rf_mul_launch<T>(nx, tile, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twiddles<T>(nx));
         }
         else if (pass == kPassInvY) {
@@ -1015,6 +1039,47 @@ public:
                              d_offsets_.as<long long>(), excl_.as<unsigned long long>(),
                              keep_raw_ ? raw_.as<double>() : nullptr, filt, twiddles<T>(nz));
         }
+    }
+
+    // The fused inverse y + inverse z pass (rf_yz_argmin) replaces P2 and P3
+    // on FP32 256^3 grids when the step keeps no per-value output (fields or
+    // raw values) and neither pass has a pinned tile.
+    bool fused_yz() const
+    {
+        return fuse_yz_ && use_rf_ && precision_ == 32 && n_[0] == 256 && n_[1] == 256 && n_[2] == 256 &&
+               !fields_needed() && !keep_raw_ && !pinned_[kPassInvY] && !pinned_[kPassC2r] &&
+               yz_lds_bytes<float>() <= max_lds_;
+    }
+
+    bool fused_active()
+    {
+        prepare();
+        return fused_yz();
+    }
+
+    void set_fused_passes(bool on)
+    {
+        if (on != fuse_yz_) {
+            fuse_yz_ = on;
+            tuned_ = false;
+            drop_graphs();
+        }
+    }
+
+    template <typename T> void enqueue_yz(const int* act, int slots)
+    {
+        if constexpr (std::is_same_v<T, float>) {
+            const int nx = n_[0], ny = n_[1], nz = n_[2];
+            const long long to = (long long)(nz / 2 + 1) * ny * nx;
+            const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
+            const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
+                                    orient_best_.as<unsigned long long>() };
+            hipLaunchKernelGGL((rf_yz_argmin<T>), dim3(lmax_[0], 1, slots), dim3(kYzThreads), yz_lds_bytes<T>(),
+                               stream_, t_.as<cx<T>>(), to, d_descs_.as<OrientDesc>(), inv_total,
+                               result_.as<StepDeviceResult>(), excl_.as<unsigned long long>(), filt, twiddles<T>(nz));
+        }
+        else
+            throw std::logic_error("the fused inverse y and z pass runs in FP32");
     }
 
     // One generic (LDS Stockham) pass at tile width W (pencils per block).
@@ -1131,11 +1196,15 @@ public:
                 // the inverse z pass pairs columns into complex transforms by
                 // their place in the tile, so its width sets the arithmetic too
                 if (fixed_radix_ && pass == kPassC2r) continue;
+                // the fused pass replaces P2 and P3 and has no tile; P1 then
+                // writes the column-major layout at radix 16
+                const bool fused = fused_yz();
+                if (fused && pass != kPassMul) continue;
                 const bool rf = pass_uses_rf(pass);
                 char key[160];
-                std::snprintf(key, sizeof key, "p%d%s%s n%d,%d,%d f%d k%d,%d l%d,%d s%d", pass, rf ? "r" : "g",
-                              fixed_radix_ ? "d" : "", nx, ny, nz, precision_, kmax_[0], kmax_[1], lmax_[0], lmax_[1],
-                              nfit_);
+                std::snprintf(key, sizeof key, "p%d%s%s%s n%d,%d,%d f%d k%d,%d l%d,%d s%d", pass, rf ? "r" : "g",
+                              fixed_radix_ ? "d" : "", fused ? "y" : "", nx, ny, nz, precision_, kmax_[0], kmax_[1],
+                              lmax_[0], lmax_[1], nfit_);
                 auto it = ctx_.tiles.find(key);
                 if (it != ctx_.tiles.end()) {
                     tile_[pass] = Tile { it->second.first, it->second.second };
@@ -1148,6 +1217,7 @@ public:
                     if (rf && !tile_fits(pass, t)) continue;
                     // the radix sets the arithmetic; the tile width does not
                     if (rf && fixed_radix_ && t.r != default_tile(pass, n).r) continue;
+                    if (fused && t.r != 16) continue;
                     run_pass<T>(pass, t, act_full_.as<int>(), nfit_);
                     if (hipGetLastError() != hipSuccess) continue;  // launch refused: not a candidate here
                     SP_HIP_CHECK(hipEventRecord(e0, stream_));
@@ -1328,7 +1398,12 @@ public:
             run_pass<T>(kPassMul, tile_[kPassMul], act, slots);
             mark("mul_x");
         }
-        if (slots > 0 && lmax_[0] > 0) {
+        if (slots > 0 && lmax_[0] > 0 && fused_yz()) {
+            // P2 + P3 in one pass over each legal x column
+            enqueue_yz<T>(act, slots);
+            mark("yz_argmin");
+        }
+        else if (slots > 0 && lmax_[0] > 0) {
             // P2: inverse C2C along y for the legal columns, rows y < ly written
             run_pass<T>(kPassInvY, tile_[kPassInvY], act, slots);
             mark("inv_c2c_y");
@@ -1904,6 +1979,7 @@ private:
     StepDeviceResult* host_result_ {};
     long long fields_total_ {};
     bool keep_fields_ {}, keep_raw_ {}, prepared_ {};
+    bool fuse_yz_ { true };
     DeviceBuffer raw_;
     int precision_ { 32 };
     double bound_ {};
@@ -2056,6 +2132,18 @@ int sp_hip_session_set_fast_path(sp_hip_session* s, int on)
 {
     if (!s) return SP_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] { s->s->set_fast_path(on != 0); });
+}
+
+int sp_hip_session_set_fused_passes(sp_hip_session* s, int on)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->set_fused_passes(on != 0); });
+}
+
+int sp_hip_session_get_fused_passes(sp_hip_session* s, int32_t* active)
+{
+    if (!s || !active) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { *active = s->s->fused_active() ? 1 : 0; });
 }
 
 int sp_hip_session_set_active(sp_hip_session* s, const uint32_t* orientations, uint32_t count,

@@ -349,6 +349,17 @@ class Session:
     def set_fast_path(self, on=True):
         self._check(lib().sp_hip_session_set_fast_path(self._h, C.c_int(1 if on else 0)))
 
+    def set_fused_passes(self, on=True):
+        """FP32 256^3 steps without per-value output run the inverse y and
+        inverse z passes as one pass per legal x column (default on)."""
+        self._check(lib().sp_hip_session_set_fused_passes(self._h, C.c_int(1 if on else 0)))
+
+    def fused_passes(self):
+        """Whether the next step runs the fused inverse y + z pass."""
+        a = C.c_int32()
+        self._check(lib().sp_hip_session_get_fused_passes(self._h, C.byref(a)))
+        return bool(a.value)
+
     def stamp(self, orientation, translation):
         t = (C.c_int64 * 3)(*[int(v) for v in translation])
         self._check(lib().sp_hip_session_stamp(self._h, C.c_uint32(orientation), t))

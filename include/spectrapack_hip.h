@@ -193,6 +193,15 @@ int sp_hip_session_profile_step(sp_hip_session* s, int stamp, float* kernel_ms, 
 /* 1 (default): register-resident FFT passes for 256- and 512-point axes;
  * 0: the generic LDS Stockham passes for every axis (same results). */
 int sp_hip_session_set_fast_path(sp_hip_session* s, int on);
+/* 1 (default): on FP32 sessions over a 256^3 FFT grid whose steps keep no
+ * per-value output (no overlap fields, no raw values, the inverse y and z
+ * tiles not pinned), the inverse y and inverse z passes run as one pass per
+ * legal x column that keeps the column's spectrum on chip (the product pass
+ * then writes its output column-major); 0: the two passes run separately.
+ * Same results.  get_fused_passes reports whether the next step fuses them.
+ * No counterpart in the reference (a scheduling choice of this build). */
+int sp_hip_session_set_fused_passes(sp_hip_session* s, int on);
+int sp_hip_session_get_fused_passes(sp_hip_session* s, int32_t* active);
 /* Unrounded legal values (double) of the last step, for measuring the FP32
  * deviation from FP64 on the same occupancy; requires keep_raw. */
 int sp_hip_session_keep_raw(sp_hip_session* s, int keep);

@@ -1,0 +1,114 @@
+"""The fused inverse y + inverse z pass (FP32, 256^3 FFT grid, steps without
+per-value output) against the oracle, and against the separate passes on the
+bench workload with rejections and region masks."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def kernels_with_long_axes(rs, small=1):
+    """Small random kernels plus kernels long in y, x and z: the long y kernel
+    leaves 117 legal rows (the fused pass's single-chunk case), the long x
+    kernel 56 legal columns and the long z kernel 36 legal layers."""
+    out = []
+    for _ in range(small):
+        s = tuple(int(v) for v in rs.integers(1, 5, 3))
+        c = (rs.uniform(0, 1, s[::-1]) < 0.7).astype(np.uint8)
+        c.flat[0] = 1
+        out.append(dict(first=tuple(int(v) for v in rs.integers(-4, 4, 3)), shape=s, cells=c))
+    for s in ((1, 140, 1), (201, 1, 1), (1, 1, 221)):
+        c = np.ones(s[::-1], np.uint8)
+        c[0, 0, 0] = 0
+        out.append(dict(first=(0, 0, 0), shape=s, cells=c))
+    return out
+
+
+def make_session(ctx, env, first, kernels, fused, precision=32):
+    import spectrapack_amd as sp
+    s = sp.Session(ctx, env.shape[::-1], first, len(kernels), precision=precision)
+    s.set_fused_passes(fused)
+    s.set_environment(env)
+    for i, k in enumerate(kernels):
+        s.set_kernel(i, k["cells"], k["shape"], k["first"])
+    return s
+
+
+def test_fused_steps_against_oracle(ctx, oracle):
+    rs = np.random.default_rng(31)
+    env = (rs.uniform(0, 1, (256, 256, 256)) < 0.02).astype(np.uint8)
+    env[:2] = 1
+    first = (-2, 0, -1)
+    kernels = kernels_with_long_axes(rs)
+    s = make_session(ctx, env, first, kernels, True)
+    assert s.fused_passes()
+    excluded = []
+    for step in range(3):
+        print(f"[fused] oracle step={step}", flush=True)
+        o, t, _ = oracle.place_step(env, first, kernels, direct=True, excluded=excluded)
+        if step == 1:
+            # reject the oracle's choice: both must take the next one
+            s.exclude(o, t)
+            excluded.append((o, tuple(t)))
+            o, t, _ = oracle.place_step(env, first, kernels, direct=True, excluded=excluded)
+        r = s.step(stamp=True)
+        assert r["orientation"] == o and tuple(r["translation"]) == tuple(t), (step, r, o, t)
+        assert r["max_residual"] < 0.05
+        env = oracle.stamp(env, first, kernels[o], t)
+    assert np.array_equal(s.environment(), env)
+
+
+def test_fused_off_and_fields_use_separate_passes(ctx):
+    rs = np.random.default_rng(5)
+    env = (rs.uniform(0, 1, (256, 256, 256)) < 0.01).astype(np.uint8)
+    kernels = kernels_with_long_axes(rs, small=3)[:3]
+    s = make_session(ctx, env, (0, 0, 0), kernels, False)
+    assert not s.fused_passes()
+    s.set_fused_passes(True)
+    assert s.fused_passes()
+    s.keep_fields(True)
+    assert not s.fused_passes()  # per-value output needs the separate passes
+    s.keep_fields(False)
+    assert s.fused_passes()
+    # 252 cells is still a 256-point grid; 250 is a grid of its own length
+    s2 = make_session(ctx, env[:, :252, :250], (0, 0, 0), kernels, True)
+    assert not s2.fused_passes()
+    s3 = make_session(ctx, env[:, :252, :], (0, 0, 0), kernels, True)
+    assert s3.fused_passes()
+    s64 = make_session(ctx, env, (0, 0, 0), kernels, True, precision=64)
+    assert not s64.fused_passes()
+    for x in (s, s2, s3, s64):
+        x.close()
+
+
+def test_fused_equals_separate_on_bracket_workload(ctx):
+    """All 96 orientations of the bench workload: fused and separate steps
+    choose the same placements, also with a rejected translation and with an
+    orientation subset restricted to a region mask."""
+    import spectrapack_amd as sp
+    from spectrapack_amd import workloads as W
+    w = W.bracket_in_sphere()
+    env = ctx.mesh_container(w.container.vertices, w.container.triangles, w.env_first, w.env_shape, w.pitch)
+    kernels = ctx.voxelize_objects(w.part.vertices, w.part.triangles, w.catalog, w.pitch)
+    sess = {f: make_session(ctx, env, w.env_first, kernels, f) for f in (True, False)}
+    assert sess[True].fused_passes() and not sess[False].fused_passes()
+    for step in range(5):
+        print(f"[fused] bracket step={step}", flush=True)
+        if step == 2:
+            r = sess[True].step(stamp=False)
+            for s in sess.values():
+                s.exclude(r["orientation"], r["translation"])
+        if step == 3:
+            mask = np.zeros((4, 4, 4), np.uint8)
+            mask[1:3, :, 2:] = 1
+            for s in sess.values():
+                s.set_active(list(range(10, 60, 3)), mask)
+        if step == 4:
+            for s in sess.values():
+                s.set_active()
+        ra = sess[True].step(stamp=True)
+        rb = sess[False].step(stamp=True)
+        assert ra["numeric_flags"] == rb["numeric_flags"] == 0
+        assert ra["orientation"] == rb["orientation"] and ra["translation"] == rb["translation"], (step, ra, rb)
+        assert abs(ra["max_residual"] - rb["max_residual"]) < 0.01 and ra["max_residual"] < 0.05
+    assert np.array_equal(sess[True].environment(), sess[False].environment())
