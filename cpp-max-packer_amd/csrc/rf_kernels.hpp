@@ -103,12 +103,15 @@ __global__ void __launch_bounds__(W*(N / R)) rf_rows(cx<T>* data, int rows, long
 // inverse x transform and stores x < lx into out[o][kz][y][x].
 // XT: the output is stored column-major instead, out[o][x][kz][y] (x stride
 // gridDim.y * ny), for the fused inverse y and z pass; lanes then take the
-// pencil fastest so that every store covers runs of W consecutive y.
+// pencil fastest so that every store covers runs of W consecutive y.  The
+// environment rows are multiplied by sb_scale once per block (the fused pass
+// takes its 1/(nx ny nz) here, an exact power of two on its grids).
 template <typename T, int N, int R, int W, bool XT = false>
 __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px,
                                                        const cx<T>* __restrict__ SB, int ny, cx<T>* __restrict__ out,
                                                        long long out_o, const OrientDesc* __restrict__ descs,
-                                                       const int* __restrict__ act, const cx<T>* __restrict__ tw)
+                                                       const int* __restrict__ act, const cx<T>* __restrict__ tw,
+                                                       T sb_scale)
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
@@ -126,6 +129,10 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
     T2 sb[R];
 #pragma unroll
     for (int s = 0; s < R; ++s) sb[s] = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
+    if (XT) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) sb[s] = T2 { sb[s].x * sb_scale, sb[s].y * sb_scale };
+    }
     T2* pl = lds + w * S::stride;
     const int count = act[0];
     for (int i = 0; i < count; ++i) {
@@ -366,21 +373,28 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
 // Fused inverse y + C2R along z + checks + argmin for N = ny = nz = 256 at
 // radix 16.  One block of 1024 threads owns one x column of one orientation:
 // the column's half spectrum in[o][x][kz][y] (129 x 256 values, written by
-// rf_mul_x<..., true>) is read once, transformed along y in registers, and
-// transformed along z in two chunks of up to 128 legal rows through LDS, so
-// the inverse y output never goes back to memory.
+// rf_mul_x<..., true> and already scaled by 1/(nx ny nz)) is read once,
+// transformed along y in registers, and transformed along z in two chunks of
+// up to 128 legal rows through LDS, so the inverse y output never goes back
+// to memory.
 //  * y pass: 64 groups of 16 threads, two pencils per group (kz = g and
 //    g + 64).  The kz = 0 and kz = 128 planes of a real correlation are
 //    Hermitian in y, so their columns transform together as one complex
 //    pencil (kz0 + i kz128): the real outputs come back as the real and
 //    imaginary parts.  Thread t keeps Y[kz][y = t + 16 s] for s < 16.
 //  * z pass: the rows of a chunk are staged as stage[kz][row] (line 0 keeps
-//    the packed kz0 + i kz128 values); group g takes
-//    rows (2g, 2g + 1) of the chunk as one complex pencil through the
-//    Hermitian extension (as rf_c2r_z_argmin does for column pairs), and the
-//    real and imaginary outputs are the two rows' values at z = t + 16 s.
-// Rounding, the per-value checks, the feasible mask and the key are those of
-// rf_c2r_z_argmin.
+//    the packed kz0 + i kz128 values); group g takes rows (2g, 2g + 1) of the
+//    chunk as one complex pencil through the Hermitian extension (as
+//    rf_c2r_z_argmin does for column pairs), and the real and imaginary
+//    outputs are the two rows' values at z = t + 16 s.
+// The whole column is in registers between the two passes (64 VGPRs of the
+// 128 a thread has at 1024 threads), so the transforms use reg_fft_lean
+// (exchange twiddles from a [q][t] table, reads four at a time).  The checks
+// are those of rf_c2r_z_argmin over the legal values, kept as running
+// extremes: the largest residual (NaN-propagating, so a NaN raises the
+// residual flag as a per-value test does), the smallest and largest rounded
+// count, and a bit per zero count; the lowest feasible key is taken from the
+// zero bits after the transforms.
 constexpr int kYzThreads = 1024;
 constexpr int kYzStageRow = 132;  // complex values per staged kz line (128 rows + pad)
 
@@ -393,7 +407,7 @@ template <typename T> constexpr std::size_t yz_lds_bytes()
 
 template <typename T>
 __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restrict__ in, long long in_o,
-                                                            const OrientDesc* __restrict__ descs, T inv_total,
+                                                            const OrientDesc* __restrict__ descs,
                                                             StepDeviceResult* __restrict__ result,
                                                             const unsigned long long* __restrict__ excl, StepFilter filt,
                                                             const cx<T>* __restrict__ tw)
@@ -419,7 +433,12 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
         flags = 0;
     }
     const int t = threadIdx.x % P, g = threadIdx.x / P;
-    load_tw_lds<T>(tws, tw, N);
+    if (threadIdx.x < N) {
+        // inverse exchange twiddles as [q][t]: conj(w^(t q))
+        const int q = threadIdx.x / R, tt = threadIdx.x % R;
+        const T2 w = tw[(tt * q) & (N - 1)];
+        tws[threadIdx.x] = T2 { w.x, -w.y };
+    }
     const T2* col = in + o * in_o + (long long)x * NH * N;
     // y pass: both pencils' loads first
     T2 a[R], b[R];
@@ -448,9 +467,7 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
     const T max_out = T(d.mass < (long long)result->env_mass ? d.mass : (long long)result->env_mass);
     const bool any_excluded = excl[0] != 0;
     const bool regions_on = filt.act && filt.act[1];
-    unsigned long long mine = ~0ULL;
-    float my_res = 0.f;
-    unsigned int my_flags = 0;
+    T r1 = T(0), lo = T(0), hi = T(0);
     T2* stage = lds;
     unsigned int zero_bits[2] = { 0u, 0u };
     // chunks of rows [128 c, 128 c + 128); the loop is unrolled so that the
@@ -488,26 +505,28 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
             __builtin_amdgcn_sched_barrier(0);
         }
         reg_fft_lean<T, N, R>(v, pen, tws, t, false);
-        // checks over every legal value; zero counts are marked in bit
-        // 2 s + half, which orders them as their keys (z, then y)
+        // checks over every legal value, kept as running extremes (the
+        // residual NaN-propagating, so a NaN raises the residual flag as a
+        // per-value test would); zero counts are marked in bit 2 s + half,
+        // which orders them as their keys (z, then y)
         unsigned int zeros = 0;
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             const bool z_in = t + P * s < d.lz;
-#pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const bool in = z_in && y1 + half < d.ly;
-                const T val = (half ? v[s].y : v[s].x) * inv_total;
-                const T rv = rint(val);
-                const T res = fabs(val - rv);
-                // branch-free: out-of-range values contribute nothing
-                my_res = fmaxf(my_res, in ? float(res) : 0.f);
-                my_flags |= (in && !(res < T(0.25)) ? 1u : 0u) | (in && (rv < T(0) || rv > max_out) ? 2u : 0u);
-                zeros |= (in && rv == T(0) ? 1u : 0u) << (2 * s + half);
-            }
+            const bool ia = z_in && y1 < d.ly, ib = z_in && y1 + 1 < d.ly;
+            const T va = v[s].x, vb = v[s].y;
+            const T ra = rint(va), rb = rint(vb);
+            r1 = __builtin_elementwise_maximum(
+                r1, __builtin_elementwise_maximum(ia ? fabsf(va - ra) : T(0), ib ? fabsf(vb - rb) : T(0)));
+            lo = fminf(lo, fminf(ia ? ra : T(0), ib ? rb : T(0)));
+            hi = fmaxf(hi, fmaxf(ia ? ra : T(0), ib ? rb : T(0)));
+            zeros |= (ra == T(0) && ia ? 1u << (2 * s) : 0u) | (rb == T(0) && ib ? 2u << (2 * s) : 0u);
         }
         zero_bits[c] = zeros;
     }
+    unsigned int my_flags = (!(r1 < T(0.25)) ? 1u : 0u) | (lo < T(0) || hi > max_out ? 2u : 0u);
+    float my_res = float(r1);
+    unsigned long long mine = ~0ULL;
     // this thread's lowest feasible key per chunk: the first marked value
     // that is neither rejected nor outside the allowed regions (scanned after
     // the transforms, when no spectrum is held any more)
@@ -530,12 +549,12 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long om = __shfl_down(mine, off, 64);
         mine = om < mine ? om : mine;
-        my_res = fmaxf(my_res, __shfl_down(my_res, off, 64));
+        my_res = __builtin_elementwise_maximum(my_res, __shfl_down(my_res, off, 64));
         my_flags |= __shfl_down(my_flags, off, 64);
     }
     if ((threadIdx.x & 63) == 0) {
         if (mine != ~0ULL) atomicMin(&best, mine);
-        if (my_res > 0.f) atomicMax(&res_bits, __float_as_uint(my_res));
+        if (!(my_res <= 0.f)) atomicMax(&res_bits, __float_as_uint(my_res));
         if (my_flags) atomicOr(&flags, my_flags);
     }
     __syncthreads();

@@ -100,35 +100,34 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
     }
 }
 
-// reg_fft for N = R * R (one exchange) with the exchange reads and their
-// twiddles issued four at a time, so that a kernel holding other data in
-// registers keeps the transform's working set small.  Same arithmetic as
-// reg_fft.
+// reg_fft for N = R * R (one exchange) with the exchange twiddles taken from
+// a [q][t] table, tq[q * R + t] = w^(t q) in the transform's direction, so
+// that every twiddle read is one base address plus a constant, and with the
+// exchange reads and their twiddles issued four at a time (a kernel holding
+// other data in registers keeps the transform's working set small).  Same
+// arithmetic as reg_fft.
 template <typename T, int N, int R>
-__device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tw, int t, bool fwd)
+__device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd)
 {
-    using T2 = cx<T>;
     using S = RegShape<N, R>;
-    static_assert(S::passes == 2, "one exchange");
-    // a fresh copy of t per call, so that the twiddle addresses (t q) are
-    // recomputed here rather than kept live across calls
+    static_assert(S::passes == 2 && R == 16, "one exchange of 16 x 16");
+    // a fresh copy of t per call: the twiddles are then read again here
+    // rather than kept in registers from an earlier call
     asm volatile("" : "+v"(t));
+    __builtin_assume(t >= 0 && t < R);
     dft_reg<T, R>(v, fwd);
     __syncthreads();
+    // pad(16 t + s) = 17 t + s and pad(16 q + t) = 17 q + t for s, t < 16
 #pragma unroll
-    for (int s = 0; s < R; ++s) lds[S::pad(t * R + s)] = v[s];
+    for (int s = 0; s < R; ++s) lds[(R + 1) * t + s] = v[s];
     __syncthreads();
     // second pass: l = R, m = 1, j = t, k = 0
 #pragma unroll
     for (int q0 = 0; q0 < R; q0 += 4) {
 #pragma unroll
         for (int q = q0; q < q0 + 4; ++q) {
-            v[q] = lds[S::pad(q * R + t)];
-            if (q > 0) {
-                T2 w = tw[(t * q) & (N - 1)];
-                if (!fwd) w.y = -w.y;
-                v[q] = cmul(v[q], w);
-            }
+            v[q] = lds[(R + 1) * q + t];
+            if (q > 0) v[q] = cmul(v[q], tq[q * R + t]);
         }
         __builtin_amdgcn_sched_barrier(0);
     }

@@ -343,7 +343,7 @@ void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>
         using TT = decltype(tt);
         constexpr int N = TT::n, R = TT::r, W = TT::w;
         hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st,
-                           H, h_o, h_px, SB, ny, out, out_o, descs, act, tw);
+                           H, h_o, h_px, SB, ny, out, out_o, descs, act, tw, T(1));
     });
     if (!ok) throw DeviceError { "no such FFT tile" };
 }
@@ -353,13 +353,13 @@ void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>
 template <typename T>
 void rf_mul_xt_launch(Tile tile, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px,
                       const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
-                      const cx<T>* tw)
+                      const cx<T>* tw, T sb_scale)
 {
     constexpr int N = 256, R = 16;
     auto go = [&](auto ww) {
         constexpr int W = decltype(ww)::value;
         hipLaunchKernelGGL((rf_mul_x<T, N, R, W, true>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)),
-                           (rf_lds<T, N, R>(W)), st, H, h_o, h_px, SB, ny, out, out_o, descs, act, tw);
+                           (rf_lds<T, N, R>(W)), st, H, h_o, h_px, SB, ny, out, out_o, descs, act, tw, sb_scale);
     };
     if (tile.r == R && tile.w == 8) go(std::integral_constant<int, 8> {});
     else if (tile.r == R && tile.w == 32) go(std::integral_constant<int, 32> {});
@@ -1020,7 +1020,9 @@ public:
         if (pass == kPassMul) {
             if constexpr (std::is_same_v<T, float>) {
                 if (fused_yz()) {
-                    rf_mul_xt_launch<T>(tile, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twiddles<T>(nx));
+                    // the fused pass's 1/(nx ny nz) = 2^-24, applied exactly to the environment rows
+                    rf_mul_xt_launch<T>(tile, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twiddles<T>(nx),
+                                        T(1) / (T(nx) * T(ny) * T(nz)));
                     return;
                 }
             }
@@ -1071,12 +1073,11 @@ public:
         if constexpr (std::is_same_v<T, float>) {
             const int nx = n_[0], ny = n_[1], nz = n_[2];
             const long long to = (long long)(nz / 2 + 1) * ny * nx;
-            const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
             const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
                                     orient_best_.as<unsigned long long>() };
             hipLaunchKernelGGL((rf_yz_argmin<T>), dim3(lmax_[0], 1, slots), dim3(kYzThreads), yz_lds_bytes<T>(),
-                               stream_, t_.as<cx<T>>(), to, d_descs_.as<OrientDesc>(), inv_total,
-                               result_.as<StepDeviceResult>(), excl_.as<unsigned long long>(), filt, twiddles<T>(nz));
+                               stream_, t_.as<cx<T>>(), to, d_descs_.as<OrientDesc>(), result_.as<StepDeviceResult>(),
+                               excl_.as<unsigned long long>(), filt, twiddles<T>(nz));
         }
         else
             throw std::logic_error("the fused inverse y and z pass runs in FP32");
