@@ -73,6 +73,9 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
+    // t < P lets the compiler drop the wrap of every twiddle index and fold
+    // the padded LDS offsets into constants
+    __builtin_assume(t >= 0 && t < S::P);
     int l = 1;
 #pragma unroll
     for (int p = 0; p < S::passes; ++p) {
