@@ -396,11 +396,12 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
 // count, and a bit per zero count; the lowest feasible key is taken from the
 // zero bits after the transforms.
 constexpr int kYzThreads = 1024;
-constexpr int kYzStageRow = 132;  // complex values per staged kz line (128 rows + pad)
+constexpr int kYzPencil = RegShape<256, 16>::stride;  // LDS values per exchange pencil
+constexpr int kYzStageRow = 132;                       // LDS values per staged kz line (128 rows + pad)
 
 template <typename T> constexpr std::size_t yz_lds_bytes()
 {
-    constexpr std::size_t fft = std::size_t(64) * RegShape<256, 16>::stride;
+    constexpr std::size_t fft = std::size_t(64) * kYzPencil;
     constexpr std::size_t stage = std::size_t(128) * kYzStageRow;
     return sizeof(cx<T>) * (256 + (fft > stage ? fft : stage));
 }
@@ -455,7 +456,7 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
         }
     }
     __syncthreads();
-    T2* pen = lds + g * S::stride;
+    T2* pen = lds + g * kYzPencil;
     reg_fft_lean<T, N, R>(a, pen, tws, t, false);
     // the second pencil is loaded only now: hoisting its loads above the first
     // transform would hold both pencils and that transform's working set at once
