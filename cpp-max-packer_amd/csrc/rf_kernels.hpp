@@ -146,7 +146,14 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
             const int x = t + P * q;
             v[q] = (live && x < kx) ? nt_load(&hrow[x]) : T2 { 0, 0 };
         }
-        reg_fft<T, N, R>(v, pl, tws, t, true);
+        // the kernel row has kx <= h_px nonzero values: at radix 16 with up to
+        // three leading groups of 16, the first pass leaves the zeros out
+        // (uniform over the block)
+        const int lead = (kx + P - 1) / P;
+        if (R == 16 && lead == 1) reg_fft<T, N, R, 1>(v, pl, tws, t, true);
+        else if (R == 16 && lead == 2) reg_fft<T, N, R, 2>(v, pl, tws, t, true);
+        else if (R == 16 && lead == 3) reg_fft<T, N, R, 3>(v, pl, tws, t, true);
+        else reg_fft<T, N, R>(v, pl, tws, t, true);
 #pragma unroll
         for (int s = 0; s < R; ++s) v[s] = cmulc(sb[s], v[s]);
         reg_fft<T, N, R>(v, pl, tws, t, false);

@@ -49,6 +49,51 @@ template <typename T> __device__ __forceinline__ void dft16(cx<T>* v, bool fwd)
     }
 }
 
+// dft16 of an input whose v[q] are zero for q >= NQ (NQ <= 3): the same 4 x 4
+// decomposition with the zero terms left out.  The first radix-4 stage of
+// column n2 then has the single input v[n2]; the second adds the NQ nonzero
+// rows.  Every nonzero term is computed as in dft16, so the outputs equal
+// dft16's.
+template <typename T, int NQ> __device__ __forceinline__ void dft16_lead(cx<T>* v, bool fwd)
+{
+    static_assert(NQ >= 1 && NQ <= 3, "leading nonzero inputs");
+    using T2 = cx<T>;
+    const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173), h = T(0.70710678118654752440);
+    const T sg = fwd ? T(-1) : T(1);
+    auto tw = [&](T2 a, T c, T s) { return T2 { a.x * c - a.y * sg * s, a.x * sg * s + a.y * c }; };
+    // y[n2][k1] = v[n2] w16^(n2 k1) for the nonzero rows n2 < NQ
+    T2 y0 = v[0];
+    T2 y1[4], y2[4];
+    if constexpr (NQ >= 2) {
+        y1[0] = v[1];
+        y1[1] = tw(v[1], c1, s1);  // m=1
+        y1[2] = tw(v[1], h, h);    // m=2
+        y1[3] = tw(v[1], s1, c1);  // m=3
+    }
+    if constexpr (NQ >= 3) {
+        y2[0] = v[2];
+        y2[1] = tw(v[2], h, h);    // m=2
+        y2[2] = mul_mi(v[2], fwd); // m=4
+        y2[3] = tw(v[2], -h, h);   // m=6
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        // Dft<T, 4> of (y0, y1[k1], y2[k1], 0)
+        if constexpr (NQ == 1) {
+            v[k1] = v[k1 + 4] = v[k1 + 8] = v[k1 + 12] = y0;
+        }
+        else {
+            const T2 a = NQ >= 3 ? cadd(y0, y2[k1]) : y0, b = NQ >= 3 ? csub(y0, y2[k1]) : y0;
+            const T2 c = y1[k1];
+            const T2 d = mul_mi(y1[k1], fwd);
+            v[k1] = cadd(a, c);
+            v[k1 + 4] = cadd(b, d);
+            v[k1 + 8] = csub(a, c);
+            v[k1 + 12] = csub(b, d);
+        }
+    }
+}
+
 template <typename T, int R> __device__ __forceinline__ void dft_reg(cx<T>* v, bool fwd)
 {
     if constexpr (R == 16) dft16<T>(v, fwd);
@@ -68,7 +113,9 @@ template <int N, int R> struct RegShape {
 // Runs the transform for one pencil.  v: in = x[t + P*q], out = X[t + P*s].
 // lds: this pencil's region (RegShape::stride elements); tw: exp(-2 pi i k/N)
 // table (LDS).  Every thread of the block must call it (barriers inside).
-template <typename T, int N, int R>
+// LEAD > 0 (radix 16): x[t + P*q] is zero for q >= LEAD, and the first pass
+// leaves those terms out (dft16_lead).
+template <typename T, int N, int R, int LEAD = 0>
 __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, int t, bool fwd)
 {
     using T2 = cx<T>;
@@ -93,7 +140,12 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
                 v[q] = cmul(v[q], w);
             }
         }
-        dft_reg<T, R>(v, fwd);
+        if constexpr (LEAD > 0 && R == 16) {
+            if (p == 0) dft16_lead<T, LEAD>(v, fwd);
+            else dft_reg<T, R>(v, fwd);
+        }
+        else
+            dft_reg<T, R>(v, fwd);
         if (p + 1 < S::passes) {
             __syncthreads();
 #pragma unroll
