@@ -112,7 +112,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=1)
-    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--profile-steps", type=int, default=4)
     ap.add_argument("--no-deviation", action="store_true", help="skip the FP32-vs-FP64 deviation step")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal packing comparison")
     ap.add_argument("--no-incremental", action="store_true", help="skip the incremental-step timing")
@@ -262,7 +262,9 @@ def main():
     if not args.no_ranked:
         ranked = ranked_timing(sp, ctx, w, env, kernels, prec, args)
 
-    # per-kernel breakdown (no graph, events between kernels)
+    # per-kernel breakdown (no graph, events between kernels), after one
+    # untimed profile step
+    s.profile_step(stamp=True)
     prof = {}
     for _ in range(args.profile_steps):
         for name, ms in s.profile_step(stamp=True):
