@@ -83,8 +83,9 @@ def test_fused_off_and_fields_use_separate_passes(ctx):
 
 def test_fused_equals_separate_on_bracket_workload(ctx):
     """All 96 orientations of the bench workload: fused and separate steps
-    choose the same placements, also with a rejected translation and with an
-    orientation subset restricted to a region mask."""
+    find the same lowest feasible corner for every orientation and choose the
+    same placements, also with a rejected translation and with an orientation
+    subset restricted to a region mask."""
     import spectrapack_amd as sp
     from spectrapack_amd import workloads as W
     w = W.bracket_in_sphere()
@@ -111,4 +112,34 @@ def test_fused_equals_separate_on_bracket_workload(ctx):
         assert ra["numeric_flags"] == rb["numeric_flags"] == 0
         assert ra["orientation"] == rb["orientation"] and ra["translation"] == rb["translation"], (step, ra, rb)
         assert abs(ra["max_residual"] - rb["max_residual"]) < 0.01 and ra["max_residual"] < 0.05
+        assert sess[True].orientation_minima() == sess[False].orientation_minima(), step
     assert np.array_equal(sess[True].environment(), sess[False].environment())
+
+
+def test_fused_orientation_minima_edge_extents(ctx):
+    """Every orientation's lowest feasible corner equals the separate passes'
+    for legal extents at the fused pass's edges on a 256 x 256 x 253 window
+    (a 256^3 FFT grid): 128 and 129 legal rows (one chunk, and a second chunk
+    of one row), 4 legal layers, a single legal column, a kernel that does not
+    fit, and a small random kernel; over three occupancies, one of them with a
+    rejected corner."""
+    rs = np.random.default_rng(77)
+    kernels = []
+    for sh in ((1, 129, 1), (2, 128, 1), (1, 1, 250), (256, 1, 1), (1, 1, 256), (3, 2, 2)):
+        c = (rs.uniform(0, 1, sh[::-1]) < 0.8).astype(np.uint8)
+        c.flat[-1] = 1
+        kernels.append(dict(first=(0, 0, 0), shape=sh, cells=c))
+    for trial, density in enumerate((0.002, 0.01, 0.03)):
+        env = (rs.uniform(0, 1, (253, 256, 256)) < density).astype(np.uint8)
+        minima = {}
+        for fused in (True, False):
+            s = make_session(ctx, env, (0, 0, 0), kernels, fused)
+            assert s.fused_passes() == fused
+            r = s.step(stamp=False)
+            if trial == 2:
+                s.exclude(r["orientation"], r["translation"])
+                r = s.step(stamp=False)
+            minima[fused] = (s.orientation_minima(), r["orientation"], r["translation"], r["numeric_flags"])
+            s.close()
+        print(f"[fused] edge extents trial={trial} minima={minima[True][0]}", flush=True)
+        assert minima[True] == minima[False], trial
