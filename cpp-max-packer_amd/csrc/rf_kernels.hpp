@@ -464,14 +464,16 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
     }
     __syncthreads();
     T2* pen = lds + g * kYzPencil;
-    reg_fft_lean<T, N, R>(a, pen, tws, t, false);
+    static_assert(64 % P == 0, "a pencil's threads are lanes of one wave");
+    // phase A's exchanges touch only each group's own region
+    reg_fft_lean<T, N, R, false>(a, pen, tws, t, false);
     // the second pencil is loaded only now: hoisting its loads above the first
     // transform would hold both pencils and that transform's working set at once
     __builtin_amdgcn_sched_barrier(0);
     asm volatile("" ::: "memory");
 #pragma unroll
     for (int q = 0; q < R; ++q) b[q] = nt_load(&p1[P * q]);
-    reg_fft_lean<T, N, R>(b, pen, tws, t, false);
+    reg_fft_lean<T, N, R, false>(b, pen, tws, t, false);
     const T max_out = T(d.mass < (long long)result->env_mass ? d.mass : (long long)result->env_mass);
     const bool any_excluded = excl[0] != 0;
     const bool regions_on = filt.act && filt.act[1];

@@ -161,7 +161,12 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
 // exchange reads and their twiddles issued four at a time (a kernel holding
 // other data in registers keeps the transform's working set small).  Same
 // arithmetic as reg_fft.
-template <typename T, int N, int R>
+// The pencil's 16 threads are lanes of one wave (t = lane % 16 within a group
+// of 16 consecutive lanes), and LDS instructions of a wave complete in order,
+// so the exchange itself needs no block barrier.  SYNC: the pencil's region
+// may still be read by other waves (it aliases a buffer they use), so the
+// block synchronises before the first write.
+template <typename T, int N, int R, bool SYNC = true>
 __device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd)
 {
     using S = RegShape<N, R>;
@@ -171,11 +176,12 @@ __device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* 
     asm volatile("" : "+v"(t));
     __builtin_assume(t >= 0 && t < R);
     dft_reg<T, R>(v, fwd);
-    __syncthreads();
+    if (SYNC) __syncthreads();
+    else __builtin_amdgcn_wave_barrier();
     // pad(16 t + s) = 17 t + s and pad(16 q + t) = 17 q + t for s, t < 16
 #pragma unroll
     for (int s = 0; s < R; ++s) lds[(R + 1) * t + s] = v[s];
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
     // second pass: l = R, m = 1, j = t, k = 0
 #pragma unroll
     for (int q0 = 0; q0 < R; q0 += 4) {
