@@ -126,6 +126,7 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
     const bool live = y < ny;
     const long long row = (long long)kz * ny + y;
     load_tw_lds<T>(tws, tw, N);
+    __syncthreads();
     T2 sb[R];
 #pragma unroll
     for (int s = 0; s < R; ++s) sb[s] = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
@@ -149,14 +150,17 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
         // the kernel row has kx <= h_px nonzero values: at radix 16 with up to
         // three leading groups of 16, the first pass leaves the zeros out
         // (uniform over the block)
+        // row-major form: a pencil is P consecutive lanes of one wave, so its
+        // exchanges are wave-local (the table was synchronised before the loop)
+        constexpr bool WV = !XT && 64 % P == 0;
         const int lead = (kx + P - 1) / P;
-        if (R == 16 && lead == 1) reg_fft<T, N, R, 1>(v, pl, tws, t, true);
-        else if (R == 16 && lead == 2) reg_fft<T, N, R, 2>(v, pl, tws, t, true);
-        else if (R == 16 && lead == 3) reg_fft<T, N, R, 3>(v, pl, tws, t, true);
-        else reg_fft<T, N, R>(v, pl, tws, t, true);
+        if (R == 16 && lead == 1) reg_fft<T, N, R, 1, WV>(v, pl, tws, t, true);
+        else if (R == 16 && lead == 2) reg_fft<T, N, R, 2, WV>(v, pl, tws, t, true);
+        else if (R == 16 && lead == 3) reg_fft<T, N, R, 3, WV>(v, pl, tws, t, true);
+        else reg_fft<T, N, R, 0, WV>(v, pl, tws, t, true);
 #pragma unroll
         for (int s = 0; s < R; ++s) v[s] = cmulc(sb[s], v[s]);
-        reg_fft<T, N, R>(v, pl, tws, t, false);
+        reg_fft<T, N, R, 0, WV>(v, pl, tws, t, false);
         if (live && XT) {
             T2* ocol = out + o * out_o + row;
             const long long xs = (long long)gridDim.y * ny;

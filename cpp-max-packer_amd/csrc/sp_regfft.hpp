@@ -114,12 +114,21 @@ template <int N, int R> struct RegShape {
 // lds: this pencil's region (RegShape::stride elements); tw: exp(-2 pi i k/N)
 // table (LDS).  Every thread of the block must call it (barriers inside).
 // LEAD > 0 (radix 16): x[t + P*q] is zero for q >= LEAD, and the first pass
-// leaves those terms out (dft16_lead).
-template <typename T, int N, int R, int LEAD = 0>
+// leaves those terms out (dft16_lead).  WAVE: the pencil's P threads are P
+// consecutive lanes of one wave (t = lane % P, P dividing 64) and nothing else
+// uses its LDS region; LDS instructions of a wave complete in order, so the
+// exchanges need no block barrier.  The caller synchronises the block once
+// after filling the twiddle table.
+template <typename T, int N, int R, int LEAD = 0, bool WAVE = false>
 __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, int t, bool fwd)
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
+    static_assert(!WAVE || 64 % S::P == 0, "a wave-local pencil must lie within one wave");
+    auto sync = [] {
+        if constexpr (WAVE) __builtin_amdgcn_wave_barrier();
+        else __syncthreads();
+    };
     // t < P lets the compiler drop the wrap of every twiddle index and fold
     // the padded LDS offsets into constants
     __builtin_assume(t >= 0 && t < S::P);
@@ -130,7 +139,7 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
         const int j = t % l, k = t / l;
         if (p > 0) {
             // gather this pass's inputs: src[(q*m + k)*l + j]
-            __syncthreads();
+            sync();
 #pragma unroll
             for (int q = 0; q < R; ++q) v[q] = lds[S::pad((q * m + k) * l + j)];
 #pragma unroll
@@ -147,7 +156,7 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
         else
             dft_reg<T, R>(v, fwd);
         if (p + 1 < S::passes) {
-            __syncthreads();
+            sync();
 #pragma unroll
             for (int s = 0; s < R; ++s) lds[S::pad((k * R + s) * l + j)] = v[s];
         }
