@@ -926,7 +926,11 @@ public:
         const std::size_t nh = std::size_t(n_[2] / 2 + 1);
         sb_.allocate(elem * nh * n_[1] * n_[0]);
         h1_.allocate(elem * O_ * nh * kmax_[1] * kmax_[0]);
-        h2_.allocate(elem * O_ * nh * n_[1] * kmax_[0]);
+        // H2 rows start on 128-byte lines (the product pass reads one per
+        // orientation and spectral row)
+        const int per_line = int(128 / elem);
+        h2p_ = (kmax_[0] + per_line - 1) / per_line * per_line;
+        h2_.allocate(elem * O_ * nh * n_[1] * h2p_);
         t_.allocate(elem * O_ * nh * n_[1] * n_[0]);
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
         for (int a = 0; a != 3; ++a) {
@@ -976,7 +980,7 @@ public:
         const OrientDesc* dd = d_descs_.as<OrientDesc>();
         const std::size_t e = sizeof(T2);
         const long long h1o = (long long)nh * kmax_[1] * kmax_[0];
-        const long long h2o = (long long)nh * ny * kmax_[0];
+        const long long h2o = (long long)nh * ny * h2p_;
         // G1: kernel R2C along z over its own columns (x < kx, y < ky)
         if (rf_supported(nz) && use_rf_) {
             rf_r2c_launch<T, unsigned char>(nz, stream_, kmax_[0], kmax_[1], slots, kcells_.as<unsigned char>(), 0, 0, 0,
@@ -992,15 +996,15 @@ public:
         // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
         if (rf_supported(ny) && use_rf_) {
             rf_cols_launch<T>(ny, default_tile(kPassInvY, ny), stream_, kmax_[0], nh, slots, H1, h1o, (long long)kmax_[0], 1LL,
-                              (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)kmax_[0], 1LL,
-                              (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, 1);
+                              (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)h2p_, 1LL,
+                              (long long)ny * h2p_, 0, 0, ny, dd, 1, act, twy, 1);
         }
         else {
             const int W = choose_w(ny, e, 2, false, 16, 1);
             hipLaunchKernelGGL((k_c2c<T>), dim3((kmax_[0] + W - 1) / W, nh, slots), dim3(kThreads),
                                lds_bytes<T>(ny, 2, W), stream_, H1, h1o, (long long)kmax_[0], 1LL,
-                               (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)kmax_[0], 1LL,
-                               (long long)ny * kmax_[0], 0, 0, ny, dd, 1, act, twy, plans_[1], W, 1);
+                               (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)h2p_, 1LL,
+                               (long long)ny * h2p_, 0, 0, ny, dd, 1, act, twy, plans_[1], W, 1);
         }
         mark("kernel_c2c_y");
     }
@@ -1015,18 +1019,18 @@ public:
         T2* H2 = h2_.as<T2>();
         T2* TT = t_.as<T2>();
         const OrientDesc* dd = d_descs_.as<OrientDesc>();
-        const long long h2o = (long long)nh * ny * kmax_[0];
+        const long long h2o = (long long)nh * ny * h2p_;
         const long long to = (long long)nh * ny * nx;
         if (pass == kPassMul) {
             if constexpr (std::is_same_v<T, float>) {
                 if (fused_yz()) {
                     // the fused pass's 1/(nx ny nz) = 2^-24, applied exactly to the environment rows
-                    rf_mul_xt_launch<T>(tile, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twiddles<T>(nx),
+                    rf_mul_xt_launch<T>(tile, stream_, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx),
                                         T(1) / (T(nx) * T(ny) * T(nz)));
                     return;
                 }
             }
-            rf_mul_launch<T>(nx, tile, stream_, ny, nh, H2, h2o, kmax_[0], SB, TT, to, dd, act, twiddles<T>(nx));
+            rf_mul_launch<T>(nx, tile, stream_, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx));
         }
         else if (pass == kPassInvY) {
             rf_cols_launch<T>(ny, tile, stream_, lmax_[0], nh, slots, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
@@ -1093,11 +1097,11 @@ public:
         T2* H2 = h2_.as<T2>();
         T2* TT = t_.as<T2>();
         const OrientDesc* dd = d_descs_.as<OrientDesc>();
-        const long long h2o = (long long)nh * ny * kmax_[0];
+        const long long h2o = (long long)nh * ny * h2p_;
         const long long to = (long long)nh * ny * nx;
         if (pass == kPassMul) {
             hipLaunchKernelGGL((k_mul_x<T>), dim3((ny + W - 1) / W, nh, slots), dim3(kThreads), lds_bytes<T>(nx, 4, W),
-                               stream_, H2, h2o, kmax_[0], ny, SB, nx, ny, TT, to, dd, 0, 0, act, twiddles<T>(nx),
+                               stream_, H2, h2o, h2p_, ny, SB, nx, ny, TT, to, dd, 0, 0, act, twiddles<T>(nx),
                                plans_[0], W);
         }
         else if (pass == kPassInvY) {
@@ -1391,7 +1395,7 @@ public:
                                (long long)nx * ny, ny, nx, nx, nullptr, 0, nullptr, twx, plans_[0], W, 1);
             mark("env_c2c_x");
         }
-        const long long h2o = (long long)nh * ny * kmax_[0];
+        const long long h2o = (long long)nh * ny * h2p_;
         const long long to = (long long)nh * ny * nx;
         if (slots > 0 && !resident_) enqueue_kernel_spectra<T>(act, slots);
         if (slots > 0) {
@@ -2011,6 +2015,7 @@ private:
     float tile_ms_[kPasses] {};
     bool pinned_[kPasses] {};
     std::size_t max_lds_ { 65536 };
+    int h2p_ { 0 };  // H2 row pitch (values)
     std::vector<hipEvent_t> prof_events_;
     std::vector<const char*> prof_names_;
     // occupancy revision (set_environment and every stamp) and ranking state
