@@ -54,7 +54,9 @@ def kernel_bytes(name, n, kernels, ext, elem):
     if name == "kernel_c2c_y":
         return sum(nh * k["shape"][1] * k["shape"][0] * elem + nh * ny * k["shape"][0] * elem for k in fit)
     if name == "mul_x":
-        return sum(nh * ny * (k["shape"][0] + nx + (ex - k["shape"][0] + 1)) * elem for k in fit)
+        # each orientation's kernel rows (Kx values) in and legal columns (Lx) out;
+        # the environment spectrum once per launch (orientations are batched)
+        return nh * ny * nx * elem + sum(nh * ny * (k["shape"][0] + (ex - k["shape"][0] + 1)) * elem for k in fit)
     if name == "inv_c2c_y":
         return sum(nh * (ny + ey - k["shape"][1] + 1) * (ex - k["shape"][0] + 1) * elem for k in fit)
     if name == "c2r_z_argmin":
