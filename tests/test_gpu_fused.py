@@ -143,3 +143,29 @@ def test_fused_orientation_minima_edge_extents(ctx):
             s.close()
         print(f"[fused] edge extents trial={trial} minima={minima[True][0]}", flush=True)
         assert minima[True] == minima[False], trial
+
+
+def test_fused_after_ineffective_incremental_window(ctx):
+    """Incremental steps whose update window would not be under half the grid
+    (a 128^3 kernel window) run as full steps without overlap fields, which
+    is the fused pass on a 256^3 FP32 grid: the same placements and
+    per-orientation minima as plain steps on the separate passes."""
+    rs = np.random.default_rng(9)
+    env = (rs.uniform(0, 1, (256, 256, 256)) < 0.002).astype(np.uint8)
+    big = np.zeros((128, 128, 128), np.uint8)
+    big[0, 0, 0] = big[-1, -1, -1] = big[0, -1, 0] = big[-1, 0, -1] = 1
+    kernels = [dict(first=(0, 0, 0), shape=(128, 128, 128), cells=big),
+               dict(first=(1, 0, 0), shape=(2, 2, 2), cells=np.ones((2, 2, 2), np.uint8))]
+    inc = make_session(ctx, env, (0, 0, 0), kernels, True)
+    inc.set_incremental(True)
+    ref = make_session(ctx, env, (0, 0, 0), kernels, False)
+    for step in range(3):
+        print(f"[fused] ineffective incremental step={step}", flush=True)
+        ra = inc.step(stamp=True)
+        rb = ref.step(stamp=True)
+        assert (ra["orientation"], ra["translation"]) == (rb["orientation"], rb["translation"]), (step, ra, rb)
+        assert inc.orientation_minima() == ref.orientation_minima(), step
+        assert inc.fused_passes()
+    assert np.array_equal(inc.environment(), ref.environment())
+    inc.close()
+    ref.close()
