@@ -23,6 +23,9 @@ sys.path.insert(0, os.path.join(ROOT, "cpp-max-packer_amd"))
 
 METRIC = "placement steps/s at 256³×96 orientations; parts packed vs plain search"
 PEAK_HBM_GBS = 8000.0
+# MI355X peak engine clock; a CU issues at most one VALU wave-instruction per
+# clock (four SIMDs, each taking four clocks per 64-wide instruction)
+PEAK_CLOCK_GHZ = 2.4
 
 
 def log(msg):
@@ -289,6 +292,21 @@ def main():
             tj = json.load(f)
         if tj.get("precision") == prec and tj.get("kernel") == dominant:
             traffic = tj.get("hbm_bytes_per_launch")
+    # issue side of the same kernel: VALU wave-instructions per launch measured
+    # with rocprofv3 (SQ_INSTS_VALU) on this workload, against the chip's issue
+    # rate of one VALU wave-instruction per CU per clock
+    issue = None
+    ifile = os.path.join(ROOT, "profiles", f"valu_{args.workload}.json")
+    if os.path.exists(ifile):
+        with open(ifile) as f:
+            ij = json.load(f)
+        n_valu = ij.get("valu_wave_instructions_per_launch", {}).get(dominant)
+        if ij.get("precision") == prec and n_valu:
+            cus = torch.cuda.get_device_properties(0).multi_processor_count
+            issue_ms = n_valu / (cus * PEAK_CLOCK_GHZ * 1e9) * 1e3
+            issue = {"bound": "valu", "kernel": dominant, "valu_wave_instructions": n_valu, "cus": cus,
+                     "clock_ghz": PEAK_CLOCK_GHZ, "issue_ms": issue_ms, "avg_launch_ms": dom_ms,
+                     "frac": issue_ms / dom_ms, "source": os.path.relpath(ifile, ROOT)}
     out = {
         "metric": METRIC,
         "value": 1000.0 / ms_step,
@@ -313,6 +331,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
+        "issue_roofline": issue,
         "step_roofline": {"model_bytes_per_step": step_bytes, "achieved": step_achieved, "peak": PEAK_HBM_GBS,
                           "unit": "GB/s", "frac": step_achieved / PEAK_HBM_GBS},
         "kernel_ms": prof,

@@ -523,20 +523,27 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
         // residual NaN-propagating, so a NaN raises the residual flag as a
         // per-value test would); zero counts are marked in bit 2 s + half,
         // which orders them as their keys (z, then y)
+        // A value outside the legal range is replaced by 0 before rounding
+        // (residual 0, count 0: neutral for the extremes); its zero bit is
+        // cleared afterwards with the lane's legal-bit mask.
         unsigned int zeros = 0;
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             const bool z_in = t + P * s < d.lz;
-            const bool ia = z_in && y1 < d.ly, ib = z_in && y1 + 1 < d.ly;
-            const T va = v[s].x, vb = v[s].y;
+            const T va = z_in && y1 < d.ly ? v[s].x : T(0), vb = z_in && y1 + 1 < d.ly ? v[s].y : T(0);
             const T ra = rint(va), rb = rint(vb);
-            r1 = __builtin_elementwise_maximum(
-                r1, __builtin_elementwise_maximum(ia ? fabsf(va - ra) : T(0), ib ? fabsf(vb - rb) : T(0)));
-            lo = fminf(lo, fminf(ia ? ra : T(0), ib ? rb : T(0)));
-            hi = fmaxf(hi, fmaxf(ia ? ra : T(0), ib ? rb : T(0)));
-            zeros |= (ra == T(0) && ia ? 1u << (2 * s) : 0u) | (rb == T(0) && ib ? 2u << (2 * s) : 0u);
+            r1 = __builtin_elementwise_maximum(r1, __builtin_elementwise_maximum(fabsf(va - ra), fabsf(vb - rb)));
+            lo = fminf(lo, fminf(ra, rb));
+            hi = fmaxf(hi, fmaxf(ra, rb));
+            zeros |= (ra == T(0) ? 1u << (2 * s) : 0u) | (rb == T(0) ? 2u << (2 * s) : 0u);
         }
-        zero_bits[c] = zeros;
+        {
+            // legal bits: 2 s + half with t + 16 s < lz and y1 + half < ly
+            const int ns = min(max((d.lz - t + P - 1) / P, 0), R);
+            const unsigned int zmask = ns >= R ? ~0u : (1u << (2 * ns)) - 1u;
+            const unsigned int ymask = (y1 < d.ly ? 0x55555555u : 0u) | (y1 + 1 < d.ly ? 0xAAAAAAAAu : 0u);
+            zero_bits[c] = zeros & zmask & ymask;
+        }
     }
     unsigned int my_flags = (!(r1 < T(0.25)) ? 1u : 0u) | (lo < T(0) || hi > max_out ? 2u : 0u);
     float my_res = float(r1);
