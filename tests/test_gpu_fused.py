@@ -169,3 +169,25 @@ def test_fused_after_ineffective_incremental_window(ctx):
     assert np.array_equal(inc.environment(), ref.environment())
     inc.close()
     ref.close()
+
+
+def test_fused_product_pass_tiles(ctx):
+    """The column-major product pass at each radix-16 tile width (8, 16, 32)
+    feeds the fused pass the same spectrum: every orientation's lowest
+    feasible corner equals the separate passes' on two occupancies."""
+    rs = np.random.default_rng(13)
+    kernels = kernels_with_long_axes(rs, small=3)
+    for trial, density in enumerate((0.003, 0.02)):
+        env = (rs.uniform(0, 1, (256, 256, 256)) < density).astype(np.uint8)
+        ref = make_session(ctx, env, (0, 0, 0), kernels, False)
+        ref.step(stamp=False)
+        want = ref.orientation_minima()
+        ref.close()
+        s = make_session(ctx, env, (0, 0, 0), kernels, True)
+        for width in (8, 16, 32):
+            s.set_tile(0, 16, width)
+            assert s.fused_passes()
+            print(f"[fused] product tile width={width} trial={trial}", flush=True)
+            s.step(stamp=False)
+            assert s.orientation_minima() == want, (trial, width)
+        s.close()
