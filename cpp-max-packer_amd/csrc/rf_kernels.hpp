@@ -96,22 +96,29 @@ __global__ void __launch_bounds__(W*(N / R)) rf_rows(cx<T>* data, int rows, long
     }
 }
 
-// The product pass, batched over orientations: a block owns W spectral rows
-// (kz, y0..y0+W) of the environment spectrum, keeps them in registers, and
-// for every orientation finishes the kernel's x transform from its partially
-// transformed row H[kz][y][x < kx], multiplies by the conjugate, runs the
-// inverse x transform and stores x < lx into out[o][kz][y][x].
+// The product pass, batched over orientations: a block takes a tile of W
+// spectral rows (kz, y0..y0+W) of the environment spectrum, keeps them in
+// registers, and for each of its orientations finishes the kernel's x
+// transform from its partially transformed row H[kz][y][x < kx], multiplies
+// by the conjugate, runs the inverse x transform and stores x < lx into
+// out[o][kz][y][x].
 // XT: the output is stored column-major instead, out[o][x][kz][y] (x stride
-// gridDim.y * ny), for the fused inverse y and z pass; lanes then take the
-// pencil fastest so that every store covers runs of W consecutive y.  The
-// environment rows are multiplied by sb_scale once per block (the fused pass
-// takes its 1/(nx ny nz) here, an exact power of two on its grids).
+// nh * ny), for the fused inverse y and z pass; lanes then take the pencil
+// fastest so that every store covers runs of W consecutive y.  The
+// environment rows are multiplied by sb_scale whenever they are loaded (the
+// fused pass takes its 1/(nx ny nz) here, an exact power of two on its grids).
+// The work is (row tile, orientation) units, tile-major: nh * ceil(ny / W)
+// tiles of W rows times the active orientations.  The grid is 1-D and block
+// b takes units [b U / G, (b + 1) U / G), loading the environment rows again
+// whenever its range enters a new tile.  A grid of as many blocks as the
+// device holds at once then ends together, where one block per tile would
+// leave a last partial round (nh = n/2 + 1 kz rows do not divide into whole
+// rounds of resident blocks).
 template <typename T, int N, int R, int W, bool XT = false>
-__global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px,
-                                                       const cx<T>* __restrict__ SB, int ny, cx<T>* __restrict__ out,
-                                                       long long out_o, const OrientDesc* __restrict__ descs,
-                                                       const int* __restrict__ act, const cx<T>* __restrict__ tw,
-                                                       T sb_scale)
+__global__ void __launch_bounds__(W*(N / R))
+rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __restrict__ SB, int ny, int nh,
+         cx<T>* __restrict__ out, long long out_o, const OrientDesc* __restrict__ descs, const int* __restrict__ act,
+         const cx<T>* __restrict__ tw, T sb_scale)
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
@@ -121,61 +128,75 @@ __global__ void __launch_bounds__(W*(N / R)) rf_mul_x(const cx<T>* __restrict__ 
     T2* lds = tws + N;
     const int t = XT ? int(threadIdx.x) / W : int(threadIdx.x) % P;
     const int w = XT ? int(threadIdx.x) % W : int(threadIdx.x) / P;
-    const int kz = blockIdx.y;
-    const int y = blockIdx.x * W + w;
-    const bool live = y < ny;
-    const long long row = (long long)kz * ny + y;
+    const int count = act[0];
+    const int ytiles = (ny + W - 1) / W;
+    const long long units = (long long)ytiles * nh * count;
+    const long long u0 = units * blockIdx.x / gridDim.x, u1 = units * (blockIdx.x + 1) / gridDim.x;
+    if (u0 >= u1) return;  // uniform across the block
     load_tw_lds<T>(tws, tw, N);
     __syncthreads();
-    T2 sb[R];
-#pragma unroll
-    for (int s = 0; s < R; ++s) sb[s] = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
-    if (XT) {
-#pragma unroll
-        for (int s = 0; s < R; ++s) sb[s] = T2 { sb[s].x * sb_scale, sb[s].y * sb_scale };
-    }
     T2* pl = lds + w * S::stride;
-    const int count = act[0];
-    for (int i = 0; i < count; ++i) {
-        const int o = act[kActHeader + i];
-        const int kx = descs[o].kx, lx = descs[o].lx;
-        if (lx <= 0) continue;  // uniform across the block
-        const T2* hrow = H + o * h_o + row * h_px;
-        T2 v[R];
+    for (long long u = u0; u < u1;) {
+        // the part of this block's range in row tile `tile`: orientations [i0, i1)
+        const int tile = int(u / count);
+        const long long base = (long long)tile * count;
+        const int i0 = int(u - base), i1 = int(min((long long)count, u1 - base));
+        u = base + i1;
+        const int kz = tile / ytiles, y = (tile % ytiles) * W + w;
+        const bool live = y < ny;
+        const long long row = (long long)kz * ny + y;
+        T2 sb[R];
 #pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int x = t + P * q;
-            v[q] = (live && x < kx) ? nt_load(&hrow[x]) : T2 { 0, 0 };
+        for (int s = 0; s < R; ++s) sb[s] = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
+        if (XT) {
+#pragma unroll
+            for (int s = 0; s < R; ++s) sb[s] = T2 { sb[s].x * sb_scale, sb[s].y * sb_scale };
         }
-        // the kernel row has kx <= h_px nonzero values: at radix 16 with up to
-        // three leading groups of 16, the first pass leaves the zeros out
-        // (uniform over the block)
-        // row-major form: a pencil is P consecutive lanes of one wave, so its
-        // exchanges are wave-local (the table was synchronised before the loop)
-        constexpr bool WV = !XT && 64 % P == 0;
-        const int lead = (kx + P - 1) / P;
-        if (R == 16 && lead == 1) reg_fft<T, N, R, 1, WV>(v, pl, tws, t, true);
-        else if (R == 16 && lead == 2) reg_fft<T, N, R, 2, WV>(v, pl, tws, t, true);
-        else if (R == 16 && lead == 3) reg_fft<T, N, R, 3, WV>(v, pl, tws, t, true);
-        else reg_fft<T, N, R, 0, WV>(v, pl, tws, t, true);
+        for (int i = i0; i < i1; ++i) {
+            const int o = act[kActHeader + i];
+            const int kx = descs[o].kx, lx = descs[o].lx;
+            if (lx <= 0) continue;  // uniform across the block
+            const T2* hrow = H + o * h_o + row * h_px;
+            T2 v[R];
 #pragma unroll
-        for (int s = 0; s < R; ++s) v[s] = cmulc(sb[s], v[s]);
-        reg_fft<T, N, R, 0, WV>(v, pl, tws, t, false);
-        if (live && XT) {
-            T2* ocol = out + o * out_o + row;
-            const long long xs = (long long)gridDim.y * ny;
-#pragma unroll
-            for (int s = 0; s < R; ++s) {
-                const int x = t + P * s;
-                if (x < lx) nt_store(&ocol[x * xs], v[s]);
+            for (int q = 0; q < R; ++q) {
+                const int x = t + P * q;
+                v[q] = (live && x < kx) ? nt_load(&hrow[x]) : T2 { 0, 0 };
             }
-        }
-        else if (live) {
-            T2* orow = out + o * out_o + row * N;
+            // the kernel row has kx <= h_px nonzero values: at radix 16 with up to
+            // three leading groups of 16, the first pass leaves the zeros out
+            // (uniform over the block)
+            // row-major form: a pencil is P consecutive lanes of one wave, so its
+            // exchanges are wave-local (the table was synchronised before the loop)
+            constexpr bool WV = !XT && 64 % P == 0;
+            const int lead = (kx + P - 1) / P;
+            if (R == 16 && lead == 1) reg_fft<T, N, R, 1, WV>(v, pl, tws, t, true);
+            else if (R == 16 && lead == 2) reg_fft<T, N, R, 2, WV>(v, pl, tws, t, true);
+            else if (R == 16 && lead == 3) reg_fft<T, N, R, 3, WV>(v, pl, tws, t, true);
+            else reg_fft<T, N, R, 0, WV>(v, pl, tws, t, true);
 #pragma unroll
-            for (int s = 0; s < R; ++s) {
-                const int x = t + P * s;
-                if (x < lx) nt_store(&orow[x], v[s]);
+            for (int s = 0; s < R; ++s) v[s] = cmulc(sb[s], v[s]);
+            reg_fft<T, N, R, 0, WV>(v, pl, tws, t, false);
+            if (live && XT) {
+                T2* ocol = out + o * out_o + row;
+                // a fresh copy of the column stride per orientation: the store
+                // addresses are then formed here instead of being kept in
+                // registers across the loop
+                long long xs = (long long)nh * ny;
+                asm volatile("" : "+s"(xs));
+#pragma unroll
+                for (int s = 0; s < R; ++s) {
+                    const int x = t + P * s;
+                    if (x < lx) nt_store(&ocol[x * xs], v[s]);
+                }
+            }
+            else if (live) {
+                T2* orow = out + o * out_o + row * N;
+#pragma unroll
+                for (int s = 0; s < R; ++s) {
+                    const int x = t + P * s;
+                    if (x < lx) nt_store(&orow[x], v[s]);
+                }
             }
         }
     }

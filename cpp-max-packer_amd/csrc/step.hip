@@ -335,15 +335,35 @@ void rf_rows_launch(int n, hipStream_t st, cx<T>* data, int rows, long long row_
     }
 }
 
+// Grid of rf_mul_x: one block per row tile (all orientations each) when the
+// tiles fill whole rounds of resident blocks to within 2%, which keeps the
+// blocks in flight on neighbouring rows; otherwise as many blocks as the
+// device holds at once (its CUs times the blocks per CU that registers and
+// LDS allow), each with an even share of the tiles times orientations.
+inline int mul_grid(const void* fn, int threads, std::size_t lds, int cus, long long tiles, int slots)
+{
+    int per = 0;
+    SP_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds));
+    const long long cap = (long long)std::max(per, 1) * std::max(cus, 1);
+    const long long rounds = (tiles + cap - 1) / cap;
+    if ((rounds * cap - tiles) * 50 <= rounds * cap) return int(tiles);
+    return int(std::max(1LL, std::min(cap, tiles * slots)));
+}
+
+// P1 over `slots` orientation slots (act[0] of them active).
 template <typename T>
-void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px,
-                   const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act, const cx<T>* tw)
+void rf_mul_launch(int n, Tile tile, hipStream_t st, int cus, int slots, int ny, int nh, const cx<T>* H, long long h_o,
+                   int h_px, const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
+                   const cx<T>* tw)
 {
     const bool ok = dispatch_tile(MulTiles {}, n, tile, [&](auto tt) {
         using TT = decltype(tt);
         constexpr int N = TT::n, R = TT::r, W = TT::w;
-        hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st,
-                           H, h_o, h_px, SB, ny, out, out_o, descs, act, tw, T(1));
+        const std::size_t lds = rf_lds<T, N, R>(W);
+        const int grid = mul_grid(reinterpret_cast<const void*>(&rf_mul_x<T, N, R, W>), W * (N / R), lds, cus,
+                                  (long long)((ny + W - 1) / W) * nh, slots);
+        hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3(grid), dim3(W * (N / R)), lds, st, H, h_o, h_px, SB, ny, nh, out,
+                           out_o, descs, act, tw, T(1));
     });
     if (!ok) throw DeviceError { "no such FFT tile" };
 }
@@ -351,15 +371,18 @@ void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>
 // P1 with the column-major output of the fused inverse y and z pass (FP32,
 // N = 256, radix 16; the tile's width is kept when it is one of 8, 16, 32).
 template <typename T>
-void rf_mul_xt_launch(Tile tile, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px,
-                      const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
+void rf_mul_xt_launch(Tile tile, hipStream_t st, int cus, int slots, int ny, int nh, const cx<T>* H, long long h_o,
+                      int h_px, const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
                       const cx<T>* tw, T sb_scale)
 {
     constexpr int N = 256, R = 16;
     auto go = [&](auto ww) {
         constexpr int W = decltype(ww)::value;
-        hipLaunchKernelGGL((rf_mul_x<T, N, R, W, true>), dim3((ny + W - 1) / W, nh), dim3(W * (N / R)),
-                           (rf_lds<T, N, R>(W)), st, H, h_o, h_px, SB, ny, out, out_o, descs, act, tw, sb_scale);
+        const std::size_t lds = rf_lds<T, N, R>(W);
+        const int grid = mul_grid(reinterpret_cast<const void*>(&rf_mul_x<T, N, R, W, true>), W * (N / R), lds, cus,
+                                  (long long)((ny + W - 1) / W) * nh, slots);
+        hipLaunchKernelGGL((rf_mul_x<T, N, R, W, true>), dim3(grid), dim3(W * (N / R)), lds, st, H, h_o, h_px, SB, ny,
+                           nh, out, out_o, descs, act, tw, sb_scale);
     };
     if (tile.r == R && tile.w == 8) go(std::integral_constant<int, 8> {});
     else if (tile.r == R && tile.w == 32) go(std::integral_constant<int, 32> {});
@@ -433,6 +456,7 @@ public:
         int lds = 0;
         SP_HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx_.device));
         max_lds_ = std::size_t(lds);
+        SP_HIP_CHECK(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, ctx_.device));
         env_.allocate(std::size_t(ext_[0]) * ext_[1] * ext_[2]);
         SP_HIP_CHECK(hipMemsetAsync(env_.get(), 0, env_.bytes(), stream_));
         result_.allocate(sizeof(StepDeviceResult));
@@ -1025,12 +1049,12 @@ public:
             if constexpr (std::is_same_v<T, float>) {
                 if (fused_yz()) {
                     // the fused pass's 1/(nx ny nz) = 2^-24, applied exactly to the environment rows
-                    rf_mul_xt_launch<T>(tile, stream_, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx),
+                    rf_mul_xt_launch<T>(tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx),
                                         T(1) / (T(nx) * T(ny) * T(nz)));
                     return;
                 }
             }
-            rf_mul_launch<T>(nx, tile, stream_, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx));
+            rf_mul_launch<T>(nx, tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx));
         }
         else if (pass == kPassInvY) {
             rf_cols_launch<T>(ny, tile, stream_, lmax_[0], nh, slots, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
@@ -2015,6 +2039,7 @@ private:
     float tile_ms_[kPasses] {};
     bool pinned_[kPasses] {};
     std::size_t max_lds_ { 65536 };
+    int cus_ { 1 };  // compute units of the session's device
     int h2p_ { 0 };  // H2 row pitch (values)
     std::vector<hipEvent_t> prof_events_;
     std::vector<const char*> prof_names_;
