@@ -191,3 +191,29 @@ def test_fused_product_pass_tiles(ctx):
             s.step(stamp=False)
             assert s.orientation_minima() == want, (trial, width)
         s.close()
+
+
+def test_product_pass_shares_break_inside_row_tiles(ctx):
+    """The product pass shares row tiles x active orientations out evenly over
+    a grid of resident blocks, so a block's share can begin and end inside a
+    row tile.  With 1, 2, 5 and 7 active orientations (shares breaking at
+    different orientations) every active orientation keeps the lowest feasible
+    corner it has when all orientations run, and the others report none, on
+    the fused and on the separate passes at 256^3."""
+    rs = np.random.default_rng(41)
+    kernels = kernels_with_long_axes(rs, small=6)
+    env = (rs.uniform(0, 1, (256, 256, 256)) < 0.01).astype(np.uint8)
+    for fused in (True, False):
+        s = make_session(ctx, env, (0, 0, 0), kernels, fused)
+        assert s.fused_passes() == fused
+        s.step(stamp=False)
+        full = s.orientation_minima()
+        assert sum(m is not None for m in full) >= 6
+        for active in ([4], [0, 8], [1, 2, 3, 6, 7], [8, 7, 5, 4, 2, 1, 0]):
+            print(f"[fused] product shares fused={fused} active={active}", flush=True)
+            s.set_active(active)
+            s.step(stamp=False)
+            got = s.orientation_minima()
+            for o in range(len(kernels)):
+                assert got[o] == (full[o] if o in active else None), (fused, active, o)
+        s.close()
