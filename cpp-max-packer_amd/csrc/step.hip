@@ -1049,8 +1049,8 @@ public:
             if constexpr (std::is_same_v<T, float>) {
                 if (fused_yz()) {
                     // the fused pass's 1/(nx ny nz) = 2^-24, applied exactly to the environment rows
-                    rf_mul_xt_launch<T>(tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx),
-                                        T(1) / (T(nx) * T(ny) * T(nz)));
+                    rf_mul_xt_launch<T>(tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act,
+                                        twiddles<T>(nx), T(1) / (T(nx) * T(ny) * T(nz)));
                     return;
                 }
             }
