@@ -110,10 +110,11 @@ __global__ void __launch_bounds__(W*(N / R)) rf_rows(cx<T>* data, int rows, long
 // The work is (row tile, orientation) units, tile-major: nh * ceil(ny / W)
 // tiles of W rows times the active orientations.  The grid is 1-D and block
 // b takes units [b U / G, (b + 1) U / G), loading the environment rows again
-// whenever its range enters a new tile.  A grid of as many blocks as the
-// device holds at once then ends together, where one block per tile would
-// leave a last partial round (nh = n/2 + 1 kz rows do not divide into whole
-// rounds of resident blocks).
+// whenever its range enters a new tile; a grid of one block per tile gives
+// each block one tile with all orientations.  A grid of as many blocks as the
+// device holds at once ends together, where one block per tile can leave a
+// last partial round (nh = n/2 + 1 kz rows do not divide into whole rounds of
+// resident blocks).
 template <typename T, int N, int R, int W, bool XT = false>
 __global__ void __launch_bounds__(W*(N / R))
 rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __restrict__ SB, int ny, int nh,

@@ -335,11 +335,14 @@ void rf_rows_launch(int n, hipStream_t st, cx<T>* data, int rows, long long row_
     }
 }
 
-// Grid of rf_mul_x: one block per row tile (all orientations each) when the
-// tiles fill whole rounds of resident blocks to within 2%, which keeps the
-// blocks in flight on neighbouring rows; otherwise as many blocks as the
-// device holds at once (its CUs times the blocks per CU that registers and
-// LDS allow), each with an even share of the tiles times orientations.
+// Grid of the column-major product pass: one block per row tile (all
+// orientations each) when the tiles fill whole rounds of resident blocks to
+// within 2%, which keeps the blocks in flight on neighbouring rows; otherwise
+// as many blocks as the device holds at once (its CUs times the blocks per CU
+// that registers and LDS allow), each with an even share of the tiles times
+// orientations.  The row-major form always takes one block per tile: its row
+// stores of neighbouring blocks then meet, which measured faster than evenly
+// shared blocks.
 inline int mul_grid(const void* fn, int threads, std::size_t lds, int cus, long long tiles, int slots)
 {
     int per = 0;
@@ -350,20 +353,17 @@ inline int mul_grid(const void* fn, int threads, std::size_t lds, int cus, long 
     return int(std::max(1LL, std::min(cap, tiles * slots)));
 }
 
-// P1 over `slots` orientation slots (act[0] of them active).
+// P1 over the active orientations, one block per row tile.
 template <typename T>
-void rf_mul_launch(int n, Tile tile, hipStream_t st, int cus, int slots, int ny, int nh, const cx<T>* H, long long h_o,
-                   int h_px, const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
-                   const cx<T>* tw)
+void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px,
+                   const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act, const cx<T>* tw)
 {
     const bool ok = dispatch_tile(MulTiles {}, n, tile, [&](auto tt) {
         using TT = decltype(tt);
         constexpr int N = TT::n, R = TT::r, W = TT::w;
-        const std::size_t lds = rf_lds<T, N, R>(W);
-        const int grid = mul_grid(reinterpret_cast<const void*>(&rf_mul_x<T, N, R, W>), W * (N / R), lds, cus,
-                                  (long long)((ny + W - 1) / W) * nh, slots);
-        hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3(grid), dim3(W * (N / R)), lds, st, H, h_o, h_px, SB, ny, nh, out,
-                           out_o, descs, act, tw, T(1));
+        const int tiles = (ny + W - 1) / W * nh;
+        hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3(tiles), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st, H, h_o, h_px,
+                           SB, ny, nh, out, out_o, descs, act, tw, T(1));
     });
     if (!ok) throw DeviceError { "no such FFT tile" };
 }
@@ -1054,7 +1054,7 @@ public:
                     return;
                 }
             }
-            rf_mul_launch<T>(nx, tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx));
+            rf_mul_launch<T>(nx, tile, stream_, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx));
         }
         else if (pass == kPassInvY) {
             rf_cols_launch<T>(ny, tile, stream_, lmax_[0], nh, slots, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
