@@ -85,6 +85,8 @@ int good_length(int n);
 
 // Per-device, per-precision twiddle tables exp(-2 pi i k/n), k in [0, n).
 template <typename T> const cx<T>* twiddles(int n);
+// The same values on the host (n entries).
+template <typename T> void fill_twiddles(int n, cx<T>* out);
 
 struct Context {
     int device {};

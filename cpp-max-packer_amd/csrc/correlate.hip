@@ -9,6 +9,7 @@
 // runs on the device: finite/range/residual per value, total mass, and eight
 // deterministic direct probes.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <limits>
 
@@ -21,7 +22,6 @@ namespace {
 constexpr int kThreads = 256;
 constexpr std::size_t kLdsBudget = 64 * 1024;
 constexpr std::uint64_t kExactInteger = 1ULL << 53;
-constexpr std::uint64_t kMetadataBytes = 256;
 
 int choose_w(int n, std::size_t elem, int buffers, bool pairs, int cap, int min_w)
 {
@@ -89,6 +89,13 @@ struct Readout {
     unsigned int flags;              // 1 nonfinite, 2 residual, 4 range
     unsigned int pad;
 };
+
+// Clears `words` 32-bit words (the check accumulators of a call), so that
+// memory reused from an earlier call starts at zero.
+__global__ void k_zero_words(unsigned int* p, int words)
+{
+    for (int i = threadIdx.x; i < words; i += blockDim.x) p[i] = 0u;
+}
 
 // Values of the full-linear output (shape ox, oy, oz) from the circular field
 // R (shape px, py, pz) at slot (x - (K - 1)) mod P'; rounding and the
@@ -231,18 +238,131 @@ int geometry(const sp_correlation_spec& s, Geometry& g, std::uint64_t& env_count
     return SP_OK;
 }
 
-// Device working bytes of one correlation at precision `bytes` per real.
-std::uint64_t device_bytes(const Geometry& g, std::uint64_t env_count, std::uint64_t ker_count, std::size_t real,
-                           std::size_t env_elem)
+constexpr std::uint64_t kAlign = 256;
+constexpr int kMaxProbes = 8;
+inline std::uint64_t align_up(std::uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// Byte offsets of one correlation's device buffers from one base.  The
+// inputs and their validation counters come first, because their size does
+// not depend on the precision, which is chosen after validation.  Then the two
+// spectra, the real field, the output values, the check accumulators, the
+// probes and, for allocation-free calls only, the three twiddle tables.
+struct Layout {
+    std::uint64_t env, ker, stats, inputs_end;
+    std::uint64_t sb, sk, r, vals, readout, probe, probe_idx, total;
+    std::uint64_t tw[3], total_with_twiddles;
+};
+
+// Sizes of the input part; false when they are not representable.
+bool input_layout(std::uint64_t env_count, std::uint64_t ker_count, std::size_t env_elem, Layout& L)
+{
+    std::uint64_t env_bytes;
+    if (!checked_mul(env_count, env_elem, env_bytes) || env_bytes > (1ULL << 62) || ker_count > (1ULL << 62))
+        return false;
+    L = Layout {};
+    std::uint64_t o = 0;
+    L.env = o;
+    o += align_up(env_bytes);
+    L.ker = o;
+    o += align_up(ker_count);
+    L.stats = o;
+    o += align_up(sizeof(Stats));
+    L.inputs_end = o;
+    return true;
+}
+
+// The rest, once the FFT grid (each axis at most kMaxFftLength) and the
+// precision (`real` bytes per real value) are known.
+void compute_layout(const Geometry& g, std::size_t real, Layout& L)
 {
     const std::uint64_t nh = std::uint64_t(g.fft[2] / 2 + 1);
     const std::uint64_t spec = nh * std::uint64_t(g.fft[1]) * std::uint64_t(g.fft[0]) * 2 * real;
-    return 2 * spec + g.fft_cells * real + g.padded_count * sizeof(double) + env_count * env_elem + ker_count + 4096;
+    std::uint64_t o = L.inputs_end;
+    L.sb = o;
+    o += align_up(spec);
+    L.sk = o;
+    o += align_up(spec);
+    L.r = o;
+    o += align_up(g.fft_cells * real);
+    L.vals = o;
+    o += align_up(g.padded_count * sizeof(double));
+    L.readout = o;
+    o += align_up(sizeof(Readout));
+    L.probe = o;
+    o += align_up(kMaxProbes * sizeof(double));
+    L.probe_idx = o;
+    o += align_up(kMaxProbes * sizeof(long long));
+    L.total = o;
+    for (int a = 0; a != 3; ++a) {
+        L.tw[a] = o;
+        o += align_up(std::uint64_t(g.fft[a]) * 2 * real);
+    }
+    L.total_with_twiddles = o;
+}
+
+// Device bytes one correlation takes at `real` bytes per real value.
+std::uint64_t device_bytes(const Geometry& g, std::uint64_t env_count, std::uint64_t ker_count, std::size_t real,
+                           std::size_t env_elem)
+{
+    Layout L;
+    if (!input_layout(env_count, ker_count, env_elem, L)) return std::numeric_limits<std::uint64_t>::max();
+    compute_layout(g, real, L);
+    return L.total;
+}
+
+// correlation.cpp:220-235: pocketfft's workspace bound for a one-axis call
+// over the longest padded axis L, 64 complex<double> values per point plus
+// a fixed 64 KiB.
+bool pocketfft_workspace(std::uint64_t longest_axis, std::uint64_t& bytes)
+{
+    std::uint64_t values;
+    return checked_mul(longest_axis, 64, values) && checked_mul(values, 16, bytes) &&
+           checked_add(bytes, 64ULL << 10, bytes);
+}
+
+// estimate_correlation_cpu's working bytes (correlation.cpp:596-641): two
+// complex<double> buffers and the double output per padded cell, one
+// pocketfft workspace per FFT thread, the execution owner's reserve (0: the
+// device context owns the device buffers), the caller's metadata objects and
+// the probe index array of eight size_t.  False: not representable
+// (CORRELATION_SHAPE_OVERFLOW).
+bool reference_working_bytes(const Geometry& g, std::uint32_t fft_threads, std::uint64_t metadata, std::uint64_t& out)
+{
+    std::uint64_t ws;
+    const std::uint64_t longest = std::max({ g.padded[0], g.padded[1], g.padded[2] });
+    return checked_mul(g.padded_count, 2 * 16 + 8, out) && pocketfft_workspace(longest, ws) &&
+           checked_mul(ws, fft_threads, ws) && checked_add(out, ws, out) && checked_add(out, metadata, out) &&
+           checked_add(out, kMaxProbes * sizeof(std::uint64_t), out) &&
+           g.padded_count <= std::uint64_t(std::numeric_limits<std::ptrdiff_t>::max()) / 16;
+}
+
+template <typename T> struct Buffers {
+    cx<T>* SB;
+    cx<T>* SK;
+    T* R;
+    double* vals;
+    Readout* rd;
+    double* probe;
+    long long* probe_idx;
+    const cx<T>* tw[3];
+};
+
+template <typename T> Buffers<T> buffers_at(unsigned char* base, const Layout& L, std::uint64_t origin)
+{
+    Buffers<T> b {};
+    b.SB = reinterpret_cast<cx<T>*>(base + (L.sb - origin));
+    b.SK = reinterpret_cast<cx<T>*>(base + (L.sk - origin));
+    b.R = reinterpret_cast<T*>(base + (L.r - origin));
+    b.vals = reinterpret_cast<double*>(base + (L.vals - origin));
+    b.rd = reinterpret_cast<Readout*>(base + (L.readout - origin));
+    b.probe = reinterpret_cast<double*>(base + (L.probe - origin));
+    b.probe_idx = reinterpret_cast<long long*>(base + (L.probe_idx - origin));
+    return b;
 }
 
 template <typename T, typename E>
 int run(sphip::Context& C, const sp_correlation_spec& s, const Geometry& g, const E* denv, const unsigned char* dker,
-        bool binary, double max_out, double expected_mass, std::uint64_t expected_int_mass,
+        const Buffers<T>& B, bool binary, double max_out, double expected_mass, std::uint64_t expected_int_mass,
         const sp_correlation_limits& lim, double* host_values, sp_numeric_report& rep, sp_correlation_stats& st)
 {
     using T2 = cx<T>;
@@ -250,21 +370,16 @@ int run(sphip::Context& C, const sp_correlation_spec& s, const Geometry& g, cons
     const int nh = nz / 2 + 1;
     FftPlan px, py, pz;
     if (!make_plan(nx, px) || !make_plan(ny, py) || !make_plan(nz, pz)) return SP_FFT_LENGTH_UNSUPPORTED;
-    const T2* twx = twiddles<T>(nx);
-    const T2* twy = twiddles<T>(ny);
-    const T2* twz = twiddles<T>(nz);
-    const std::size_t spec = std::size_t(nh) * ny * nx;
-    C.scratch[2].ensure(spec * sizeof(T2));
-    C.scratch[3].ensure(spec * sizeof(T2));
-    C.scratch[4].ensure(g.fft_cells * sizeof(T));
-    C.scratch[5].ensure(g.padded_count * sizeof(double) + sizeof(Readout) + 64 * sizeof(double) + 64 * sizeof(long long));
-    T2* SB = C.scratch[2].as<T2>();
-    T2* SK = C.scratch[3].as<T2>();
-    T* R = C.scratch[4].as<T>();
-    double* dvals = C.scratch[5].as<double>();
-    Readout* rd = reinterpret_cast<Readout*>(dvals + g.padded_count);
-    double* dprobe = reinterpret_cast<double*>(rd + 1);
-    long long* dprobe_idx = reinterpret_cast<long long*>(dprobe + 64);
+    const T2* twx = B.tw[0];
+    const T2* twy = B.tw[1];
+    const T2* twz = B.tw[2];
+    T2* SB = B.SB;
+    T2* SK = B.SK;
+    T* R = B.R;
+    double* dvals = B.vals;
+    Readout* rd = B.rd;
+    double* dprobe = B.probe;
+    long long* dprobe_idx = B.probe_idx;
     hipStream_t st_ = C.stream;
     const std::size_t e = sizeof(T2);
     // forward transforms of both fields
@@ -308,9 +423,9 @@ int run(sphip::Context& C, const sp_correlation_spec& s, const Geometry& g, cons
         hipLaunchKernelGGL((k_c2r_z_store<T>), dim3((nx + W3 - 1) / W3, ny, 1), dim3(kThreads), lds_bytes<T>(nz, 2, W3 / 2),
                            st_, SK, nx, ny, scale, R, twz, pz, W3);
     }
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st_, reinterpret_cast<unsigned int*>(rd),
+                       int(sizeof(Readout) / sizeof(unsigned int)));
     SP_HIP_CHECK(hipGetLastError());
-    Readout zero {};
-    SP_HIP_CHECK(hipMemcpyAsync(rd, &zero, sizeof zero, hipMemcpyHostToDevice, st_));
     const double prox_tol = 1e-10 * std::max(1.0, max_out);
     hipLaunchKernelGGL((k_readout<T>), dim3(1024), dim3(kThreads), 0, st_, R, nx, ny, nz, int(g.padded[0]),
                        int(g.padded[1]), int(g.padded[2]), int(s.kernel_shape[0]), int(s.kernel_shape[1]),
@@ -333,13 +448,16 @@ int run(sphip::Context& C, const sp_correlation_spec& s, const Geometry& g, cons
     // deterministic probes (correlation.cpp:549-577)
     const std::uint64_t n = g.padded_count;
     const std::uint64_t max_index = 0xffffffffULL - (r.argmax & 0xffffffffULL);
-    std::vector<long long> probes;
+    std::array<long long, kMaxProbes> probes {};
+    int nprobe = 0;
     for (std::uint64_t c : { std::uint64_t(0), n - 1, n / 2, max_index, n / 4, n / 3, (n / 3) * 2, (n / 4) * 3 })
-        if (std::find(probes.begin(), probes.end(), (long long)c) == probes.end() && probes.size() < 8)
-            probes.push_back((long long)c);
+        if (std::find(probes.begin(), probes.begin() + nprobe, (long long)c) == probes.begin() + nprobe &&
+            nprobe < kMaxProbes)
+            probes[nprobe++] = (long long)c;
     // direct-term budget: each probe visits its in-range kernel cells
     std::uint64_t terms = 0;
-    for (long long p : probes) {
+    for (int i = 0; i != nprobe; ++i) {
+        const long long p = probes[i];
         std::uint64_t cnt = 1;
         const long long o[3] = { p % g.padded[0], (p / g.padded[0]) % g.padded[1],
                                  p / (static_cast<long long>(g.padded[0]) * g.padded[1]) };
@@ -356,18 +474,18 @@ int run(sphip::Context& C, const sp_correlation_spec& s, const Geometry& g, cons
         terms += cnt;
     }
     st.direct_terms = terms;
-    SP_HIP_CHECK(hipMemcpyAsync(dprobe_idx, probes.data(), probes.size() * sizeof(long long), hipMemcpyHostToDevice, st_));
-    hipLaunchKernelGGL((k_probe<E>), dim3(unsigned(probes.size())), dim3(kThreads), 0, st_, denv, int(s.environment_shape[0]),
+    SP_HIP_CHECK(hipMemcpyAsync(dprobe_idx, probes.data(), nprobe * sizeof(long long), hipMemcpyHostToDevice, st_));
+    hipLaunchKernelGGL((k_probe<E>), dim3(unsigned(nprobe)), dim3(kThreads), 0, st_, denv, int(s.environment_shape[0]),
                        int(s.environment_shape[1]), int(s.environment_shape[2]), dker, int(s.kernel_shape[0]),
                        int(s.kernel_shape[1]), int(s.kernel_shape[2]), int(g.padded[0]), int(g.padded[1]), dprobe_idx,
                        dprobe);
-    std::vector<double> expected(probes.size()), got(probes.size());
-    SP_HIP_CHECK(hipMemcpyAsync(expected.data(), dprobe, probes.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
-    for (std::size_t i = 0; i != probes.size(); ++i)
+    std::array<double, kMaxProbes> expected {}, got {};
+    SP_HIP_CHECK(hipMemcpyAsync(expected.data(), dprobe, nprobe * sizeof(double), hipMemcpyDeviceToHost, st_));
+    for (int i = 0; i != nprobe; ++i)
         SP_HIP_CHECK(hipMemcpyAsync(&got[i], dvals + probes[i], sizeof(double), hipMemcpyDeviceToHost, st_));
     SP_HIP_CHECK(hipStreamSynchronize(st_));
     double max_probe = 0;
-    for (std::size_t i = 0; i != probes.size(); ++i) {
+    for (int i = 0; i != nprobe; ++i) {
         const double err = std::abs(got[i] - expected[i]);
         max_probe = std::max(max_probe, err);
         const double tol = binary ? 0.25 : 1e-10 * std::max(1.0, std::abs(expected[i]));
@@ -387,13 +505,13 @@ int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, st
 {
     constexpr bool binary = std::is_same_v<E, std::uint8_t>;
     if (!ctx || !spec || !out_first || !out_shape || !report || !stats) return SP_INVALID_ARGUMENT;
-    sp_correlation_limits lim { 512ULL << 20, 0, 16777216, 200000000 };
+    sp_correlation_limits lim { 512ULL << 20, 0, 16777216, 200000000, 0 };
     if (limits) lim = *limits;
     *stats = sp_correlation_stats {};
     *report = sp_numeric_report {};
     auto& C = ctx->c;
     std::uint64_t meta;
-    if (!checked_add(lim.reserved_bytes, kMetadataBytes, meta) || meta > lim.max_working_bytes)
+    if (!checked_add(lim.reserved_bytes, lim.metadata_bytes, meta) || meta > lim.max_working_bytes)
         return SP_CORRELATION_MEMORY_LIMIT;
     Geometry g {};
     std::uint64_t env_count, ker_count;
@@ -401,15 +519,17 @@ int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, st
     if (env_n != env_count || ker_n != ker_count || (env_n && !env) || (ker_n && !ker)) return SP_CORRELATION_INVALID_FIELD;
     try {
         use_device(C);
-        C.scratch[0].ensure(env_count * sizeof(E));
-        C.scratch[1].ensure(ker_count + sizeof(Stats) + 64);
-        E* denv = C.scratch[0].as<E>();
-        unsigned char* dker = C.scratch[1].as<unsigned char>();
-        Stats* dst = reinterpret_cast<Stats*>(dker + ((ker_count + 63) / 64) * 64);
+        Layout L;
+        if (!input_layout(env_count, ker_count, sizeof(E), L)) return SP_CORRELATION_SHAPE_OVERFLOW;
+        C.scratch[0].ensure(L.inputs_end);
+        unsigned char* in_base = C.scratch[0].as<unsigned char>();
+        E* denv = reinterpret_cast<E*>(in_base + L.env);
+        unsigned char* dker = in_base + L.ker;
+        Stats* dst = reinterpret_cast<Stats*>(in_base + L.stats);
         SP_HIP_CHECK(hipMemcpyAsync(denv, env, env_count * sizeof(E), hipMemcpyHostToDevice, C.stream));
         SP_HIP_CHECK(hipMemcpyAsync(dker, ker, ker_count, hipMemcpyHostToDevice, C.stream));
-        Stats zero {};
-        SP_HIP_CHECK(hipMemcpyAsync(dst, &zero, sizeof zero, hipMemcpyHostToDevice, C.stream));
+        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, C.stream, reinterpret_cast<unsigned int*>(dst),
+                           int(sizeof(Stats) / sizeof(unsigned int)));
         hipLaunchKernelGGL((k_validate<E>), dim3(512), dim3(kThreads), 0, C.stream, denv, (long long)env_count, dker,
                            (long long)ker_count, dst);
         Stats h {};
@@ -434,15 +554,20 @@ int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, st
         }
         stats->padded_cells = g.padded_count;
         if (g.padded_count > lim.max_padded_cells) return SP_CORRELATION_PADDED_CELL_LIMIT;
+        // correlation.cpp:367-385: the caller's reserve plus the estimate's
+        // working bytes must fit max_working_bytes
+        std::uint64_t working = 0;
+        if (!reference_working_bytes(g, 1, lim.metadata_bytes, working)) return SP_CORRELATION_SHAPE_OVERFLOW;
+        if (!checked_add(lim.reserved_bytes, working, working) || working > lim.max_working_bytes)
+            return SP_CORRELATION_MEMORY_LIMIT;
+        stats->working_bytes_peak = working;
         if (!g.fft[0] || !g.fft[1] || !g.fft[2]) return SP_FFT_LENGTH_UNSUPPORTED;
         int prec = binary ? precision : 64;
         const double bound = fp32_error_bound(g.fft_cells, binary ? h.ones_env : 1, h.ones_kernel);
         if (prec == SP_PRECISION_AUTO) prec = bound > 0.4 ? 64 : 32;
-        const std::uint64_t dev = device_bytes(g, env_count, ker_count, prec == 64 ? 8 : 4, sizeof(E));
-        std::uint64_t working = lim.reserved_bytes;
-        if (!checked_add(working, dev, working) || working > lim.max_working_bytes) return SP_CORRELATION_MEMORY_LIMIT;
-        if (C.bytes_cap && dev > C.bytes_cap) return SP_CORRELATION_MEMORY_LIMIT;
-        stats->working_bytes_peak = working;
+        const std::size_t real = prec == 64 ? 8 : 4;
+        compute_layout(g, real, L);
+        if (C.bytes_cap && L.total > C.bytes_cap) return SP_CORRELATION_MEMORY_LIMIT;
         for (int a = 0; a != 3; ++a) {
             out_first[a] = g.first[a];
             out_shape[a] = g.padded[a];
@@ -451,11 +576,16 @@ int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, st
         report->precision = prec;
         report->error_bound = prec == 32 ? bound : 0.0;
         if (capacity < g.padded_count || !out_values) return SP_CAPACITY;
-        if (prec == 64)
-            return run<double, E>(C, *spec, g, denv, dker, binary, max_out, expected_mass, int_mass, lim, out_values,
-                                  *report, *stats);
-        return run<float, E>(C, *spec, g, denv, dker, binary, max_out, expected_mass, int_mass, lim, out_values, *report,
-                             *stats);
+        C.scratch[1].ensure(L.total - L.inputs_end);
+        unsigned char* base = C.scratch[1].as<unsigned char>();
+        auto go = [&](auto tag) {
+            using T = decltype(tag);
+            Buffers<T> B = buffers_at<T>(base, L, L.inputs_end);
+            for (int a = 0; a != 3; ++a) B.tw[a] = twiddles<T>(g.fft[a]);
+            return run<T, E>(C, *spec, g, denv, dker, B, binary, max_out, expected_mass, int_mass, lim, out_values,
+                             *report, *stats);
+        };
+        return prec == 64 ? go(double {}) : go(float {});
     }
     catch (const DeviceError& e) {
         C.last_error = e.what;
@@ -467,20 +597,27 @@ int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, st
 }
 
 }  // namespace
+
 }  // namespace sphip
 
 extern "C" {
 
-int sp_hip_estimate(const sp_correlation_spec* spec, sp_correlation_estimate* out)
+int sp_hip_estimate(const sp_correlation_spec* spec, uint32_t fft_threads, uint64_t metadata_bytes,
+                    sp_correlation_estimate* out)
 {
     if (!spec || !out) return SP_INVALID_ARGUMENT;
+    *out = sp_correlation_estimate {};
+    if (fft_threads < 1 || fft_threads > 8) return SP_CPU_THREAD_COUNT_UNSUPPORTED;
     sphip::Geometry g {};
     std::uint64_t ec, kc;
     if (int rc = sphip::geometry(*spec, g, ec, kc)) return rc;
     for (int a = 0; a != 3; ++a) out->padded_shape[a] = g.padded[a];
     out->padded_cells = g.padded_count;
-    if (!g.fft[0] || !g.fft[1] || !g.fft[2]) return SP_FFT_LENGTH_UNSUPPORTED;
-    out->working_bytes = sphip::device_bytes(g, ec, kc, 8, 1);
+    if (!sphip::reference_working_bytes(g, fft_threads, metadata_bytes, out->working_bytes)) {
+        out->working_bytes = 0;
+        return SP_CORRELATION_SHAPE_OVERFLOW;
+    }
+    if (g.fft[0] && g.fft[1] && g.fft[2]) out->device_bytes = sphip::device_bytes(g, ec, kc, 8, 1);
     return SP_OK;
 }
 

@@ -23,10 +23,11 @@ struct OrientDesc {
 // Result of a step (device).  key packs (uz, uy, ux, orientation) in 16 bits each.
 struct StepDeviceResult {
     unsigned long long key;
-    unsigned int flags;       // bit0 residual >= 0.25, bit1 range
+    unsigned int flags;       // bit0 residual >= 0.25, bit1 range, bit2 the chosen corner's direct recheck
     unsigned int max_residual_bits;
     unsigned long long env_mass;
-    unsigned long long pad;
+    unsigned int run_flags;   // union of the flags of earlier steps since the host last read the result
+    unsigned int pad;
 };
 
 // Active orientations of a step: act[0] = count, act[1] = regions on,

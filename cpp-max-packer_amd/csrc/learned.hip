@@ -1182,7 +1182,7 @@ int sp_hip_proposal_step(sp_hip_proposal* p, sp_hip_session* s, const float desc
         p->p->step(s, descriptor, top_orientations, top_regions, stamp != 0, fallback != 0, *out, fb);
     });
     if (used_fallback) *used_fallback = fb;
-    if (rc == SP_OK && (out->numeric_flags & 3u)) {
+    if (rc == SP_OK && ((out->numeric_flags | out->run_flags) & 7u)) {
         p->ctx->c.last_error = "step numeric check failed";
         return SP_CORRELATION_NUMERIC;
     }

@@ -67,12 +67,7 @@ template <typename T> const cx<T>* twiddles(int n)
     auto it = cache.find(key);
     if (it != cache.end()) return it->second.as<cx<T>>();
     std::vector<cx<T>> host(n);
-    const long double pi = 3.141592653589793238462643383279502884L;
-    for (int k = 0; k < n; ++k) {
-        const long double a = 2.0L * pi * static_cast<long double>(k) / static_cast<long double>(n);
-        host[k].x = T(std::cos(a));
-        host[k].y = T(-std::sin(a));
-    }
+    fill_twiddles<T>(n, host.data());
     DeviceBuffer buf(sizeof(cx<T>) * n);
     SP_HIP_CHECK(hipMemcpy(buf.get(), host.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
     const cx<T>* p = buf.as<cx<T>>();
@@ -82,6 +77,19 @@ template <typename T> const cx<T>* twiddles(int n)
 
 template const cx<float>* twiddles<float>(int);
 template const cx<double>* twiddles<double>(int);
+
+template <typename T> void fill_twiddles(int n, cx<T>* out)
+{
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < n; ++k) {
+        const long double a = 2.0L * pi * static_cast<long double>(k) / static_cast<long double>(n);
+        out[k].x = T(std::cos(a));
+        out[k].y = T(-std::sin(a));
+    }
+}
+
+template void fill_twiddles<float>(int, cx<float>*);
+template void fill_twiddles<double>(int, cx<double>*);
 
 void use_device(const Context& c) { SP_HIP_CHECK(hipSetDevice(c.device)); }
 

@@ -48,6 +48,7 @@ __global__ void k_reset_result(StepDeviceResult* r, unsigned long long* orient_b
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         r->key = ~0ULL;
+        r->run_flags |= r->flags;
         r->flags = 0;
         r->max_residual_bits = 0;
     }
@@ -60,15 +61,42 @@ __global__ void k_set_key(StepDeviceResult* r, unsigned long long key)
     if (threadIdx.x == 0 && blockIdx.x == 0) r->key = key;
 }
 
+// Direct recheck of the step's choice, the one translation the step acts on
+// (the reference's deterministic direct probes, correlation.cpp:549-577, ask
+// the same of eight outputs): every cell of the chosen kernel at the chosen
+// corner must be free in the occupancy the choice was made on.  Any overlap
+// sets flag bit 2, and the stamp that follows is skipped.
+__global__ void k_verify_choice(const unsigned char* __restrict__ env, int ex, int ey, int ez,
+                                const unsigned char* __restrict__ kcells, const OrientDesc* __restrict__ descs,
+                                StepDeviceResult* r)
+{
+    const unsigned long long key = r->key;
+    if (key == ~0ULL) return;
+    const int o = int(key & 0xffffULL);
+    const int ux = int((key >> 16) & 0xffffULL), uy = int((key >> 32) & 0xffffULL), uz = int(key >> 48);
+    const OrientDesc d = descs[o];
+    const long long n = (long long)d.kx * d.ky * d.kz;
+    unsigned int hit = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (!kcells[d.cells_offset + i]) continue;
+        const int gx = ux + int(i % d.kx), gy = uy + int((i / d.kx) % d.ky), gz = uz + int(i / ((long long)d.kx * d.ky));
+        // a corner outside the window, or a kernel cell beyond it, is an overlap
+        if (gx >= ex || gy >= ey || gz >= ez || env[gx + (long long)ex * (gy + (long long)ey * gz)]) hit = 1;
+    }
+    if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(&r->flags, 4u);
+}
+
 // ORs the chosen kernel into the occupancy (gap_fill.cpp:98-121 for the chosen
-// pose) and adds the newly set cells to the environment mass.
+// pose) and adds the newly set cells to the environment mass.  A step whose
+// checks failed (any flag) places nothing, as a failed correlation places
+// nothing in the reference.
 __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsigned char* __restrict__ kcells,
                         const OrientDesc* __restrict__ descs, StepDeviceResult* r, unsigned char* collect = nullptr,
                         int cx = 0, int cy = 0, int mx = 0, int my = 0, int mz = 0,
                         unsigned long long* collect_mass = nullptr, int shift = 0)
 {
     const unsigned long long key = r->key;
-    if (key == ~0ULL) return;
+    if (key == ~0ULL || r->flags != 0) return;
     const int o = int(key & 0xffffULL);
     const int ux = int((key >> 16) & 0xffffULL), uy = int((key >> 32) & 0xffffULL), uz = int(key >> 48);
     const OrientDesc d = descs[o];
@@ -229,6 +257,7 @@ __global__ void k_choose(StepDeviceResult* __restrict__ res, const unsigned long
     if (threadIdx.x == 0) {
         res->key = m;
         if (take_flags) {
+            res->run_flags |= res->flags;
             res->flags = upd->flags;
             res->max_residual_bits = upd->max_residual_bits;
         }
@@ -656,6 +685,17 @@ public:
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
+    // The direct recheck of the chosen corner (not for the circular update
+    // window or a real-valued proximity session, whose result key is unused).
+    void enqueue_verify()
+    {
+        if (circular_ || real_env_) return;
+        hipLaunchKernelGGL(k_verify_choice, dim3(16), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0],
+                           ext_[1], ext_[2], kcells_.as<unsigned char>(), d_descs_.as<OrientDesc>(),
+                           result_.as<StepDeviceResult>());
+        mark("verify");
+    }
+
     // Stamps the result's key: the kernel, or with a pair clearance its
     // dilated stamp kernel at the corner minus the halo; optionally collects
     // the newly set cells (at stamp-kernel coordinates) and counts them.
@@ -829,6 +869,7 @@ public:
         hipLaunchKernelGGL(k_choose, dim3(1), dim3(256), 0, stream_, result_.as<StepDeviceResult>(),
                            orient_best_.as<unsigned long long>(), O_, upd_->result_.as<StepDeviceResult>(),
                            fresh ? 0 : 1);
+        if (!fresh) enqueue_verify();
         if (stamp) enqueue_collecting_stamp();
         SP_HIP_CHECK(hipGetLastError());
     }
@@ -1440,6 +1481,7 @@ public:
             run_pass<T>(kPassC2r, tile_[kPassC2r], act, slots);
             mark("c2r_z_argmin");
         }
+        enqueue_verify();
         if (stamp) {
             enqueue_stamp(nullptr, 0, 0, nullptr);
             mark("stamp");
@@ -1612,10 +1654,16 @@ public:
         SP_HIP_CHECK(hipMemcpyAsync(host_result_, result_.get(), sizeof(StepDeviceResult), hipMemcpyDeviceToHost, stream_));
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
         const StepDeviceResult r = *host_result_;
+        if (r.run_flags) {
+            // reported once: later reads start a new run
+            SP_HIP_CHECK(hipMemsetAsync(&result_.as<StepDeviceResult>()->run_flags, 0, sizeof(unsigned int), stream_));
+            SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        }
         out = sp_step_result {};
         out.precision = precision_;
         out.error_bound = bound_;
         out.numeric_flags = r.flags;
+        out.run_flags = r.run_flags;
         float res;
         std::memcpy(&res, &r.max_residual_bits, sizeof res);
         out.max_residual = res;
@@ -2152,10 +2200,16 @@ int sp_hip_session_get_environment(sp_hip_session* s, uint8_t* cells, uint64_t c
     return guarded(s->ctx, [&] { s->s->get_environment(cells, count); });
 }
 
-int sp_hip_session_set_kernel(sp_hip_session* s, uint32_t o, const uint8_t* cells, const uint32_t shape[3],
-                              const int64_t first[3])
+int sp_hip_session_set_kernel(sp_hip_session* s, uint32_t o, const uint8_t* cells, uint64_t count,
+                              const uint32_t shape[3], const int64_t first[3])
 {
     if (!s || !cells || !shape || !first) return SP_INVALID_ARGUMENT;
+    std::uint64_t n = 1;
+    for (int a = 0; a != 3; ++a) {
+        if (shape[a] == 0 || shape[a] > 65535) return fail(s->ctx, SP_CORRELATION_SHAPE_OVERFLOW, "kernel shape");
+        n *= shape[a];
+    }
+    if (count != n) return fail(s->ctx, SP_CORRELATION_INVALID_FIELD, "kernel cell count does not match its shape");
     return guarded(s->ctx, [&] { s->s->set_kernel(o, cells, shape, first); });
 }
 
@@ -2283,7 +2337,8 @@ int sp_hip_session_step(sp_hip_session* s, int stamp, sp_step_result* out)
         s->s->launch(stamp != 0);
         s->s->read_result(*out);
     });
-    if (rc == SP_OK && (out->numeric_flags & 3u)) return fail(s->ctx, SP_CORRELATION_NUMERIC, "step numeric check failed");
+    if (rc == SP_OK && ((out->numeric_flags | out->run_flags) & 7u))
+        return fail(s->ctx, SP_CORRELATION_NUMERIC, "step numeric check failed");
     return rc;
 }
 
@@ -2298,7 +2353,10 @@ int sp_hip_session_run(sp_hip_session* s, uint32_t steps, int stamp)
 int sp_hip_session_last_result(sp_hip_session* s, sp_step_result* out)
 {
     if (!s || !out) return SP_INVALID_ARGUMENT;
-    return guarded(s->ctx, [&] { s->s->read_result(*out); });
+    const int rc = guarded(s->ctx, [&] { s->s->read_result(*out); });
+    if (rc == SP_OK && ((out->numeric_flags | out->run_flags) & 7u))
+        return fail(s->ctx, SP_CORRELATION_NUMERIC, "a step's numeric or direct check failed");
+    return rc;
 }
 
 int sp_hip_session_overlap_field(sp_hip_session* s, uint32_t o, int32_t* out, uint64_t count, uint32_t legal[3])

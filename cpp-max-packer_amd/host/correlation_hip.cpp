@@ -75,6 +75,11 @@ FftExecution* thread_execution(CorrelationFailure& error)
 }
 
 
+// The reference counts these objects of its caller in every admission
+// (correlation.cpp:67-72); the mirror's types have the reference's layout.
+constexpr std::uint64_t kMetadataBytes = 2 * sizeof(CorrelationOutcome) + sizeof(std::vector<double>) +
+                                         sizeof(runtime::OperationControl);
+
 sp_correlation_spec to_c(const CorrelationSpec& s)
 {
     sp_correlation_spec c {};
@@ -99,10 +104,10 @@ CorrelationOutcome correlate(const CorrelationSpec& spec, std::span<const E> env
     if (exec->closed()) return failure(SP_CORRELATION_EXECUTION_CLOSED);
     const sp_correlation_spec c = to_c(spec);
     const sp_correlation_limits l { limits.max_working_bytes, limits.reserved_bytes, limits.max_padded_cells,
-                                    limits.max_direct_terms };
+                                    limits.max_direct_terms, kMetadataBytes };
     sp_correlation_estimate est {};
     std::uint64_t cells = 0;
-    if (sp_hip_estimate(&c, &est) == SP_OK) cells = est.padded_cells;
+    if (sp_hip_estimate(&c, 1, kMetadataBytes, &est) == SP_OK) cells = est.padded_cells;
     CorrelationResult result;
     sp_numeric_report report {};
     sp_correlation_stats stats {};
@@ -178,8 +183,8 @@ CorrelationEstimateOutcome estimate_correlation_cpu(const CorrelationSpec& spec,
         return CorrelationFailure { "CPU_THREAD_COUNT_UNSUPPORTED", "Unsupported FFT thread count.", {} };
     const sp_correlation_spec c = to_c(spec);
     sp_correlation_estimate e {};
-    const int rc = sp_hip_estimate(&c, &e);
-    if (rc != SP_OK && rc != SP_FFT_LENGTH_UNSUPPORTED) return failure(rc);
+    const int rc = sp_hip_estimate(&c, fft_threads, kMetadataBytes, &e);
+    if (rc != SP_OK) return failure(rc);
     CorrelationEstimate out;
     for (int a = 0; a != 3; ++a) out.padded_shape[a] = e.padded_shape[a];
     out.padded_cells = e.padded_cells;

@@ -1,5 +1,6 @@
 // Test driver of the C++ host mirror (run by tests/test_gpu_host.py).
-//   spectrapack_host_test correlate          — golden vectors and failure variants
+//   spectrapack_host_test correlate          — golden vectors, failure variants and admission
+//   spectrapack_host_test estimate           — the admission estimate alone (no device work)
 //   spectrapack_host_test place <input.txt>  — the placement loop on a scene
 //     read from whitespace-separated numbers; prints one line per proposal
 //     "P orientation tx ty tz verdict" and a final "S steps proposals accepted outcome".
@@ -11,6 +12,8 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <limits>
+#include <utility>
 #include <stop_token>
 #include <string>
 #include <vector>
@@ -29,6 +32,8 @@ static int failures = 0;
             ++failures;                                                                 \
         }                                                                               \
     } while (0)
+
+static void admission_checks();
 
 static int correlate_checks()
 {
@@ -78,8 +83,149 @@ static int correlate_checks()
     const auto p = sc::correlate_proximity_cpu(spec, penv, ker);
     CHECK(std::holds_alternative<sc::CorrelationResult>(p));
     if (const auto* r = std::get_if<sc::CorrelationResult>(&p)) CHECK(std::abs(r->values[5] - 1.25) < 1e-12);
+    admission_checks();
     std::printf("correlate checks: %d failures\n", failures);
     return failures == 0 ? 0 : 1;
+}
+
+// engine/pack_compute/tests/correlation_tests.cpp:200-214: the admission
+// bytes the reference expects for a padded cell count and longest axis.
+static std::uint64_t metadata_bytes()
+{
+    return 2 * sizeof(sc::CorrelationOutcome) + sizeof(std::vector<double>) +
+           sizeof(spectrapack::runtime::OperationControl);
+}
+static std::uint64_t expected_working_bytes(std::uint64_t padded_cells, std::uint64_t longest_axis,
+                                            std::uint64_t reserved_bytes = 0)
+{
+    return reserved_bytes + metadata_bytes() + 2 * padded_cells * 16 + padded_cells * 8 + 64 * longest_axis * 16 +
+           (64ULL << 10) + 8 * sizeof(std::size_t);
+}
+
+static const sc::CorrelationFailure* failed(const sc::CorrelationOutcome& o)
+{
+    return std::get_if<sc::CorrelationFailure>(&o);
+}
+
+// correlation_tests.cpp:184-198 binary_pattern
+static std::vector<std::uint8_t> binary_pattern(sc::Shape3 shape, std::uint32_t salt)
+{
+    std::vector<std::uint8_t> cells(std::size_t(shape[0]) * shape[1] * shape[2]);
+    for (std::uint32_t z = 0; z != shape[2]; ++z)
+        for (std::uint32_t y = 0; y != shape[1]; ++y)
+            for (std::uint32_t x = 0; x != shape[0]; ++x) {
+                const auto hash = 17 * x + 11 * y + 5 * z + 3 * x * y + 7 * y * z + 13 * x * z + salt;
+                cells[x + std::size_t(shape[0]) * (y + std::size_t(shape[1]) * z)] = std::uint8_t((hash % 19) < 7);
+            }
+    cells.front() = 1;
+    cells.back() = 1;
+    return cells;
+}
+
+// The estimate alone (no device work): estimate_correlation_cpu's working
+// bytes (correlation.cpp:596-641) on the reference's own cases.
+static int estimate_checks()
+{
+    const std::uint64_t meta = metadata_bytes();
+    std::printf("metadata bytes %llu\n", (unsigned long long)meta);
+    const sc::CorrelationSpec one { { 1, 1, 1 }, { 1, 1, 1 }, { 0, 0, 0 }, { 0, 0, 0 } };
+    const auto e1 = sc::estimate_correlation_cpu(one, 1);
+    CHECK(std::holds_alternative<sc::CorrelationEstimate>(e1));
+    if (const auto* e = std::get_if<sc::CorrelationEstimate>(&e1)) {
+        // correlation_tests.cpp:593: 66,664 accounted bytes plus the metadata
+        CHECK(e->working_bytes == 66'664 + meta);
+        CHECK(e->working_bytes == expected_working_bytes(1, 1));
+        CHECK(e->padded_cells == 1);
+    }
+    // correlation_tests.cpp:216-239 with lengths 8 + 6 - 1 = 13 and 64 + 64 - 1 = 127 on every axis
+    for (const auto [el, kl] : { std::pair<std::uint32_t, std::uint32_t> { 8, 6 }, { 64, 64 } })
+        for (int axis = 0; axis != 3; ++axis) {
+            sc::Shape3 es { 1, 1, 1 }, ks { 1, 1, 1 };
+            es[axis] = el;
+            ks[axis] = kl;
+            const sc::CorrelationSpec spec { es, ks, { 6, -5, 9 }, { -3, 4, -7 } };
+            const auto est = sc::estimate_correlation_cpu(spec, 1);
+            const std::uint64_t p = el + kl - 1;
+            CHECK(std::holds_alternative<sc::CorrelationEstimate>(est) &&
+                  std::get<sc::CorrelationEstimate>(est).working_bytes == expected_working_bytes(p, p));
+            CHECK(std::holds_alternative<sc::CorrelationEstimate>(est) &&
+                  std::get<sc::CorrelationEstimate>(est).padded_shape[axis] == p);
+        }
+    // one pocketfft workspace per FFT thread (correlation.cpp:625-628)
+    const sc::CorrelationSpec threads { { 5, 3, 2 }, { 2, 1, 1 }, {}, {} };
+    const auto t1 = sc::estimate_correlation_cpu(threads, 1);
+    for (const std::uint32_t c : { 2U, 4U, 8U }) {
+        const auto tc = sc::estimate_correlation_cpu(threads, c);
+        CHECK(std::holds_alternative<sc::CorrelationEstimate>(tc) &&
+              std::get<sc::CorrelationEstimate>(tc).working_bytes ==
+                  std::get<sc::CorrelationEstimate>(t1).working_bytes + (c - 1) * (64 * 6 * 16 + (64ULL << 10)));
+    }
+    for (const std::uint32_t c : { 0U, 9U }) {
+        const auto bad = sc::estimate_correlation_cpu(threads, c);
+        CHECK(std::holds_alternative<sc::CorrelationFailure>(bad) &&
+              std::get<sc::CorrelationFailure>(bad).code == "CPU_THREAD_COUNT_UNSUPPORTED");
+    }
+    // representability (correlation.cpp:601-619)
+    const sc::CorrelationSpec zero { { 0, 1, 1 }, { 1, 1, 1 }, {}, {} };
+    const auto z = sc::estimate_correlation_cpu(zero, 1);
+    CHECK(std::holds_alternative<sc::CorrelationFailure>(z) &&
+          std::get<sc::CorrelationFailure>(z).code == "CORRELATION_SHAPE_OVERFLOW");
+    const sc::CorrelationSpec wide { { 0xffffffffu, 1, 1 }, { 2, 1, 1 }, {}, {} };
+    const auto w = sc::estimate_correlation_cpu(wide, 1);
+    CHECK(std::holds_alternative<sc::CorrelationFailure>(w) &&
+          std::get<sc::CorrelationFailure>(w).code == "CORRELATION_SHAPE_OVERFLOW");
+    const sc::CorrelationSpec origin { { 1, 1, 1 }, { 2, 1, 1 }, { std::numeric_limits<std::int64_t>::min(), 0, 0 }, {} };
+    const auto o = sc::estimate_correlation_cpu(origin, 1);
+    CHECK(std::holds_alternative<sc::CorrelationFailure>(o) &&
+          std::get<sc::CorrelationFailure>(o).code == "CORRELATION_INDEX_OVERFLOW");
+    std::printf("estimate checks: %d failures\n", failures);
+    return failures == 0 ? 0 : 1;
+}
+
+// correlation_tests.cpp:575-605 (AT-12 preflight) and :216-239
+// (qualify_padded_axis): the admission a correlation applies.
+static void admission_checks()
+{
+    const std::uint64_t meta = metadata_bytes();
+    const sc::CorrelationSpec spec { { 1, 1, 1 }, { 1, 1, 1 }, { 0, 0, 0 }, { 0, 0, 0 } };
+    const std::vector<std::uint8_t> one { 1 };
+    sc::CorrelationLimits limits;
+    limits.max_working_bytes = 1024;
+    CHECK(failed(sc::correlate_binary_cpu(spec, one, one, limits)) &&
+          failed(sc::correlate_binary_cpu(spec, one, one, limits))->code == "CORRELATION_MEMORY_LIMIT");
+    limits.max_working_bytes = 512ULL << 20;
+    limits.reserved_bytes = 1234;
+    const auto ok = sc::correlate_binary_cpu(spec, one, one, limits);
+    CHECK(std::holds_alternative<sc::CorrelationResult>(ok));
+    const std::uint64_t accounted = 66'664 + meta;
+    if (const auto* r = std::get_if<sc::CorrelationResult>(&ok))
+        CHECK(r->stats.working_bytes_peak == limits.reserved_bytes + accounted);
+    limits.max_working_bytes = limits.reserved_bytes + accounted - 1;
+    const auto over = sc::correlate_binary_cpu(spec, one, one, limits);
+    CHECK(failed(over) && failed(over)->code == "CORRELATION_MEMORY_LIMIT");
+    limits.max_working_bytes = std::numeric_limits<std::uint64_t>::max();
+    limits.reserved_bytes = std::numeric_limits<std::uint64_t>::max();
+    const auto wrap = sc::correlate_binary_cpu(spec, one, one, limits);
+    CHECK(failed(wrap) && failed(wrap)->code == "CORRELATION_MEMORY_LIMIT");
+    for (const auto [el, kl] : { std::pair<std::uint32_t, std::uint32_t> { 8, 6 }, { 64, 64 } })
+        for (int axis = 0; axis != 3; ++axis) {
+            sc::Shape3 es { 1, 1, 1 }, ks { 1, 1, 1 };
+            es[axis] = el;
+            ks[axis] = kl;
+            const sc::CorrelationSpec s { es, ks, { 6, -5, 9 }, { -3, 4, -7 } };
+            const auto env = binary_pattern(es, 3);
+            const auto ker = binary_pattern(ks, 11);
+            sc::CorrelationLimits lim;
+            lim.reserved_bytes = 321;
+            const std::uint64_t p = el + kl - 1;
+            lim.max_working_bytes = expected_working_bytes(p, p, lim.reserved_bytes);
+            const auto r = sc::correlate_binary_cpu(s, env, ker, lim);
+            CHECK(std::holds_alternative<sc::CorrelationResult>(r) &&
+                  std::get<sc::CorrelationResult>(r).stats.working_bytes_peak == lim.max_working_bytes);
+            --lim.max_working_bytes;
+            const auto rej = sc::correlate_binary_cpu(s, env, ker, lim);
+            CHECK(failed(rej) && failed(rej)->code == "CORRELATION_MEMORY_LIMIT");
+        }
 }
 
 static bool read_scene(std::istream& in, sg::Environment& env, std::vector<sg::OrientedKernel>& kernels)
@@ -210,6 +356,7 @@ static int place(const char* path)
 int main(int argc, char** argv)
 {
     if (argc >= 2 && std::string(argv[1]) == "correlate") return correlate_checks();
+    if (argc >= 2 && std::string(argv[1]) == "estimate") return estimate_checks();
     if (argc >= 3 && std::string(argv[1]) == "place") return place(argv[2]);
     if (argc >= 3 && std::string(argv[1]) == "rank") return rank(argv[2]);
     std::printf("usage: %s correlate | place <input> | rank <input>\n", argv[0]);

@@ -55,7 +55,7 @@ struct OpenSession {
         for (std::uint32_t o = 0; o != kernels.size(); ++o) {
             const auto& k = kernels[o];
             const std::int64_t first[3] = { k.first[0], k.first[1], k.first[2] };
-            if (sp_hip_session_set_kernel(s, o, k.cells.data(), k.shape.data(), first) != SP_OK) {
+            if (sp_hip_session_set_kernel(s, o, k.cells.data(), k.cells.size(), k.shape.data(), first) != SP_OK) {
                 failure = "field";
                 return;
             }

@@ -30,7 +30,7 @@ class Spec(C.Structure):
 
 class Limits(C.Structure):
     _fields_ = [("max_working_bytes", C.c_uint64), ("reserved_bytes", C.c_uint64),
-                ("max_padded_cells", C.c_uint64), ("max_direct_terms", C.c_uint64)]
+                ("max_padded_cells", C.c_uint64), ("max_direct_terms", C.c_uint64), ("metadata_bytes", C.c_uint64)]
 
 
 class Stats(C.Structure):
@@ -44,7 +44,8 @@ class Report(C.Structure):
 
 
 class Estimate(C.Structure):
-    _fields_ = [("padded_shape", C.c_uint32 * 3), ("padded_cells", C.c_uint64), ("working_bytes", C.c_uint64)]
+    _fields_ = [("padded_shape", C.c_uint32 * 3), ("padded_cells", C.c_uint64), ("working_bytes", C.c_uint64),
+                ("device_bytes", C.c_uint64)]
 
 
 class SessionDesc(C.Structure):
@@ -53,7 +54,7 @@ class SessionDesc(C.Structure):
 
 
 class StepResult(C.Structure):
-    _fields_ = [("orientation", C.c_int32), ("reserved", C.c_int32), ("translation", C.c_int64 * 3),
+    _fields_ = [("orientation", C.c_int32), ("run_flags", C.c_uint32), ("translation", C.c_int64 * 3),
                 ("key", C.c_int64 * 3), ("max_residual", C.c_double), ("numeric_flags", C.c_uint32),
                 ("precision", C.c_int32), ("error_bound", C.c_double)]
 
@@ -157,23 +158,21 @@ class Context:
     def _fail(self, rc, stats=None):
         raise Failure(rc, status_code(rc), self.last_error(), stats)
 
-    def estimate(self, spec):
-        s = _spec(spec)
-        e = Estimate()
-        rc = lib().sp_hip_estimate(C.byref(s), C.byref(e))
-        if rc:
-            self._fail(rc)
-        return dict(padded_shape=tuple(e.padded_shape), padded_cells=e.padded_cells, working_bytes=e.working_bytes)
+    def estimate(self, spec, fft_threads=1, metadata_bytes=0):
+        return estimate(spec, fft_threads, metadata_bytes)
 
     def correlate(self, spec, environment, kernel, limits=None, precision=SP_PRECISION_AUTO, proximity=False):
         """correlate_binary_cpu / correlate_proximity_cpu on the device.
 
-        Returns dict(translation_first, shape, values, numeric, stats); raises
-        Failure with the reference's code string otherwise."""
+        limits: (max_working_bytes, reserved_bytes, max_padded_cells,
+        max_direct_terms[, metadata_bytes]).  Returns dict(translation_first,
+        shape, values, numeric, stats); raises Failure with the reference's
+        code string otherwise."""
         s = _spec(spec)
         env = np.ascontiguousarray(environment, dtype=np.float64 if proximity else np.uint8).ravel()
         ker = np.ascontiguousarray(kernel, dtype=np.uint8).ravel()
-        lim = Limits(*(limits if limits is not None else (512 << 20, 0, 16777216, 200000000)))
+        lim_t = tuple(limits) if limits is not None else (512 << 20, 0, 16777216, 200000000)
+        lim = Limits(*(lim_t + (0,) * (5 - len(lim_t))))
         es = np.array(spec[0], dtype=np.int64)
         ks = np.array(spec[1], dtype=np.int64)
         n = int(np.prod(es + ks - 1)) if (es > 0).all() and (ks > 0).all() else 0
@@ -188,10 +187,9 @@ class Context:
                                                   _p(out, C.c_double), C.c_uint64(out.size), first, shape, C.byref(rep),
                                                   C.byref(st))
         else:
-            rc = lib().sp_hip_correlate_binary(self._h, C.byref(s), _p(env, C.c_uint8), C.c_uint64(env.size),
-                                               _p(ker, C.c_uint8), C.c_uint64(ker.size), C.byref(lim),
-                                               C.c_int(precision), _p(out, C.c_double), C.c_uint64(out.size), first,
-                                               shape, C.byref(rep), C.byref(st))
+            rc = lib().sp_hip_correlate_binary(self._h, C.byref(s), _p(env, C.c_uint8), C.c_uint64(env.size), _p(ker, C.c_uint8),
+                    C.c_uint64(ker.size), C.byref(lim), C.c_int(precision), _p(out, C.c_double), C.c_uint64(out.size),
+                    first, shape, C.byref(rep), C.byref(st))
         stats = dict(padded_cells=st.padded_cells, working_bytes_peak=st.working_bytes_peak,
                      direct_terms=st.direct_terms)
         if rc:
@@ -263,6 +261,19 @@ class Context:
         return cells.reshape(int(shape[2]), int(shape[1]), int(shape[0]))
 
 
+def estimate(spec, fft_threads=1, metadata_bytes=0):
+    """estimate_correlation_cpu (correlation.cpp:596-641) through the C ABI;
+    needs no GPU.  working_bytes counts `metadata_bytes` for the caller's
+    result objects; device_bytes is what one FP64 correlation takes here."""
+    s = _spec(spec)
+    e = Estimate()
+    rc = lib().sp_hip_estimate(C.byref(s), C.c_uint32(fft_threads), C.c_uint64(metadata_bytes), C.byref(e))
+    if rc:
+        raise Failure(rc, status_code(rc))
+    return dict(padded_shape=tuple(e.padded_shape), padded_cells=e.padded_cells, working_bytes=e.working_bytes,
+                device_bytes=e.device_bytes)
+
+
 def _spec(spec) -> Spec:
     es, ks, ef, kf = spec
     return Spec((C.c_uint32 * 3)(*[int(v) for v in es]), (C.c_uint32 * 3)(*[int(v) for v in ks]),
@@ -314,10 +325,14 @@ class Session:
 
     def set_kernel(self, o, cells, shape, first):
         """cells [z, y, x] uint8; shape and first as (x, y, z)."""
-        a = np.ascontiguousarray(cells, dtype=np.uint8).ravel()
+        c = np.asarray(cells)
+        if c.ndim == 3 and tuple(c.shape) != tuple(int(v) for v in shape)[::-1]:
+            raise ValueError(f"kernel cells have shape {c.shape}, expected [z, y, x] = {tuple(shape)[::-1]}")
+        a = np.ascontiguousarray(c, dtype=np.uint8).ravel()
         sh = (C.c_uint32 * 3)(*[int(v) for v in shape])
         fi = (C.c_int64 * 3)(*[int(v) for v in first])
-        self._check(lib().sp_hip_session_set_kernel(self._h, C.c_uint32(o), _p(a, C.c_uint8), sh, fi))
+        self._check(lib().sp_hip_session_set_kernel(self._h, C.c_uint32(o), _p(a, C.c_uint8), C.c_uint64(a.size),
+                                                    sh, fi))
         self.kernel_shapes[o] = tuple(int(v) for v in shape)
 
     def set_autotune(self, on=True):
@@ -410,7 +425,8 @@ class Session:
     @staticmethod
     def _result(r: StepResult):
         return dict(orientation=r.orientation, translation=tuple(r.translation), key=tuple(r.key),
-                    max_residual=r.max_residual, numeric_flags=r.numeric_flags, precision=r.precision,
+                    max_residual=r.max_residual, numeric_flags=r.numeric_flags, run_flags=r.run_flags,
+                    precision=r.precision,
                     error_bound=r.error_bound)
 
     def step(self, stamp=True):

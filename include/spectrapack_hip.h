@@ -77,12 +77,16 @@ typedef struct {
     int64_t kernel_first[3];
 } sp_correlation_spec;
 
-/* correlation.hpp:29-36 */
+/* correlation.hpp:29-36, plus the bytes of the caller's result objects that
+ * the reference counts in its admission (correlation.cpp:67-72: two
+ * CorrelationOutcome, one vector<double>, one OperationControl; the C++
+ * mirror fills it from its own types, C callers state their own). */
 typedef struct {
     uint64_t max_working_bytes;
     uint64_t reserved_bytes;
     uint64_t max_padded_cells;
     uint64_t max_direct_terms;
+    uint64_t metadata_bytes;
 } sp_correlation_limits;
 
 /* correlation.hpp:38-44 */
@@ -102,18 +106,29 @@ typedef struct {
     uint32_t fft_shape[3];
 } sp_numeric_report;
 
-/* correlation.hpp:100-103 */
+/* correlation.hpp:100-103, plus the device bytes of this build. */
 typedef struct {
     uint32_t padded_shape[3];
     uint64_t padded_cells;
-    uint64_t working_bytes;
+    uint64_t working_bytes; /* the reference's admission figure (see sp_hip_estimate) */
+    uint64_t device_bytes;  /* device memory one FP64 correlation of this spec takes here;
+                               0 when a padded axis exceeds the device FFT's maximum */
 } sp_correlation_estimate;
 
 enum sp_precision { SP_PRECISION_AUTO = 0, SP_PRECISION_FP32 = 32, SP_PRECISION_FP64 = 64 };
 
-/* correlation.cpp:596-641 estimate_correlation_cpu: padded shape E+K-1 and the
- * device bytes the GPU correlation needs. */
-int sp_hip_estimate(const sp_correlation_spec* spec, sp_correlation_estimate* out);
+/* correlation.cpp:596-641 estimate_correlation_cpu: padded shape P = E+K-1 and
+ * working_bytes = 40 prod(P) + fft_threads (64 * 16 max(P) + 64 KiB)
+ *                 + metadata_bytes + 64
+ * (two complex<double> buffers and the double output per padded cell, the
+ * pinned pocketfft workspace bound per FFT thread, the caller's metadata
+ * objects, the probe index array; the execution owner reserves nothing here).
+ * fft_threads outside 1..8: CPU_THREAD_COUNT_UNSUPPORTED.  A correlation
+ * admits reserved_bytes + working_bytes (one thread) against
+ * max_working_bytes and reports that sum as working_bytes_peak, as the
+ * reference does. */
+int sp_hip_estimate(const sp_correlation_spec* spec, uint32_t fft_threads, uint64_t metadata_bytes,
+                    sp_correlation_estimate* out);
 
 /* correlation.hpp:108-111 correlate_binary_cpu: full-linear correlation of a
  * uint8 environment {0,1} with a uint8 kernel {0,1}.  out_values receives
@@ -151,11 +166,13 @@ typedef struct {
 
 typedef struct {
     int32_t orientation;     /* -1 when no orientation has a free translation */
-    int32_t reserved;
+    uint32_t run_flags;      /* numeric_flags of every earlier step since the result was last read (OR) */
     int64_t translation[3];  /* t: kernel cell i sits at world cell t + kernel_first + i */
     int64_t key[3];          /* (t + kernel_first) as x, y, z */
     double max_residual;     /* max |v - round(v)| over legal translations, all orientations */
-    uint32_t numeric_flags;  /* bit0: residual >= 0.25, bit1: range check failed */
+    uint32_t numeric_flags;  /* bit0: residual >= 0.25, bit1: range check failed, bit2: the chosen
+                                corner's direct recheck found an overlap.  A step with any flag
+                                stamps nothing. */
     int32_t precision;
     double error_bound;      /* a-priori bound of the step's precision */
 } sp_step_result;
@@ -166,10 +183,14 @@ void sp_hip_session_destroy(sp_hip_session* s);
 int sp_hip_session_set_environment(sp_hip_session* s, const uint8_t* cells, uint64_t count);
 int sp_hip_session_get_environment(sp_hip_session* s, uint8_t* cells, uint64_t count);
 /* Kernel o: uint8 {0,1} cells of shape kernel_shape (X-fastest) whose cell 0
- * is world cell kernel_first.  All orientations must be set before a step. */
-int sp_hip_session_set_kernel(sp_hip_session* s, uint32_t orientation, const uint8_t* cells,
+ * is world cell kernel_first; count must equal prod(kernel_shape)
+ * (CORRELATION_INVALID_FIELD otherwise, as a shape/cells mismatch is in
+ * correlation.cpp:310-313).  All orientations must be set before a step. */
+int sp_hip_session_set_kernel(sp_hip_session* s, uint32_t orientation, const uint8_t* cells, uint64_t count,
                               const uint32_t kernel_shape[3], const int64_t kernel_first[3]);
-/* One step; stamp != 0 ORs the chosen kernel into the occupancy on the device. */
+/* One step; stamp != 0 ORs the chosen kernel into the occupancy on the device.
+ * CORRELATION_NUMERIC when this step or an earlier unread one failed a check
+ * (out is filled; the failing step stamped nothing). */
 int sp_hip_session_step(sp_hip_session* s, int stamp, sp_step_result* out);
 /* gap_fill.cpp:512-523 verdicts outside the graph: stamp kernel `orientation` at
  * `translation` (an accepted proposal), or keep that translation blocked for
@@ -177,7 +198,9 @@ int sp_hip_session_step(sp_hip_session* s, int stamp, sp_step_result* out);
 int sp_hip_session_stamp(sp_hip_session* s, uint32_t orientation, const int64_t translation[3]);
 int sp_hip_session_exclude(sp_hip_session* s, uint32_t orientation, const int64_t translation[3]);
 /* Enqueue `steps` steps without reading results back (for throughput timing);
- * sp_hip_session_step_result then reads the last one. */
+ * sp_hip_session_last_result then reads the last one, and returns
+ * CORRELATION_NUMERIC when any step since the previous read failed a check
+ * (run_flags / numeric_flags say which). */
 int sp_hip_session_run(sp_hip_session* s, uint32_t steps, int stamp);
 int sp_hip_session_last_result(sp_hip_session* s, sp_step_result* out);
 /* Legal overlap counts of orientation o from the last step (int32, X-fastest,
