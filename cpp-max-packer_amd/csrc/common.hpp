@@ -88,6 +88,10 @@ template <typename T> const cx<T>* twiddles(int n);
 // The same values on the host (n entries).
 template <typename T> void fill_twiddles(int n, cx<T>* out);
 
+// Address of the allocation-free correlation's static scratch on `device`
+// (the current device), looked up once per device; SP_OK or a status.
+int init_noalloc_scratch(int device);
+
 struct Context {
     int device {};
     hipStream_t stream {};

@@ -8,13 +8,24 @@
 // output at circular slot (x - (K - 1)) mod P'.  Every check of the reference
 // runs on the device: finite/range/residual per value, total mass, and eight
 // deterministic direct probes.
+//
+// sp_hip_correlate_binary_noalloc runs the same kernels on the static device
+// array `scratch` below instead of allocated buffers, for callers that cannot
+// allocate device memory.
 #include <algorithm>
 #include <array>
 #include <cstring>
 #include <limits>
+#include <mutex>
 
 #include "common.hpp"
 #include "fft_kernels.hpp"
+
+// Working memory of the allocation-free correlation: one copy per GPU, held
+// from the library's first use on that GPU until the process ends.  Host code
+// never uses this name as an address; it uses the per-device pointer that
+// hipGetSymbolAddress returns (init_noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[SP_NOALLOC_SCRATCH_BYTES];
 
 namespace sphip {
 namespace {
@@ -497,11 +508,29 @@ int run(sphip::Context& C, const sp_correlation_spec& s, const Geometry& g, cons
     return SP_OK;
 }
 
+// Per-device address of `scratch` (hipGetSymbolAddress at set-up) and a lock
+// that keeps calls of one process on one GPU from overlapping in it.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_init_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] {};
+std::mutex g_scratch_mu[kMaxDevices];
+
+// Twiddles exp(-2 pi i k / n) written into the scratch's table at `dst`, by
+// the same host formula as the cached tables, one axis at a time through a
+// stack buffer.
+template <typename T> void upload_twiddles(hipStream_t st, cx<T>* dst, int n)
+{
+    std::array<cx<T>, kMaxFftLength> host;
+    fill_twiddles<T>(n, host.data());
+    SP_HIP_CHECK(hipMemcpyAsync(dst, host.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice, st));
+    SP_HIP_CHECK(hipStreamSynchronize(st));
+}
+
 template <typename E>
 int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, std::uint64_t env_n, const std::uint8_t* ker,
               std::uint64_t ker_n, const sp_correlation_limits* limits, int precision, double* out_values,
               std::uint64_t capacity, std::int64_t out_first[3], std::uint32_t out_shape[3], sp_numeric_report* report,
-              sp_correlation_stats* stats)
+              sp_correlation_stats* stats, bool noalloc)
 {
     constexpr bool binary = std::is_same_v<E, std::uint8_t>;
     if (!ctx || !spec || !out_first || !out_shape || !report || !stats) return SP_INVALID_ARGUMENT;
@@ -517,12 +546,28 @@ int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, st
     std::uint64_t env_count, ker_count;
     if (int rc = geometry(*spec, g, env_count, ker_count)) return rc;
     if (env_n != env_count || ker_n != ker_count || (env_n && !env) || (ker_n && !ker)) return SP_CORRELATION_INVALID_FIELD;
+    std::unique_lock<std::mutex> hold;
     try {
         use_device(C);
         Layout L;
         if (!input_layout(env_count, ker_count, sizeof(E), L)) return SP_CORRELATION_SHAPE_OVERFLOW;
-        C.scratch[0].ensure(L.inputs_end);
-        unsigned char* in_base = C.scratch[0].as<unsigned char>();
+        unsigned char* in_base;
+        if (noalloc) {
+            if (C.device < 0 || C.device >= kMaxDevices || !g_scratch_ptr[C.device]) {
+                C.last_error = "the allocation-free scratch of this device is not set up";
+                return SP_DEVICE_ERROR;
+            }
+            if (L.inputs_end > SP_NOALLOC_SCRATCH_BYTES) {
+                C.last_error = "inputs exceed the allocation-free scratch";
+                return SP_SCRATCH_LIMIT;
+            }
+            hold = std::unique_lock<std::mutex>(g_scratch_mu[C.device]);
+            in_base = g_scratch_ptr[C.device];
+        }
+        else {
+            C.scratch[0].ensure(L.inputs_end);
+            in_base = C.scratch[0].as<unsigned char>();
+        }
         E* denv = reinterpret_cast<E*>(in_base + L.env);
         unsigned char* dker = in_base + L.ker;
         Stats* dst = reinterpret_cast<Stats*>(in_base + L.stats);
@@ -567,7 +612,15 @@ int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, st
         if (prec == SP_PRECISION_AUTO) prec = bound > 0.4 ? 64 : 32;
         const std::size_t real = prec == 64 ? 8 : 4;
         compute_layout(g, real, L);
-        if (C.bytes_cap && L.total > C.bytes_cap) return SP_CORRELATION_MEMORY_LIMIT;
+        if (noalloc) {
+            if (L.total_with_twiddles > SP_NOALLOC_SCRATCH_BYTES) {
+                C.last_error = "the correlation needs " + std::to_string(L.total_with_twiddles) +
+                               " bytes, more than the allocation-free scratch holds";
+                return SP_SCRATCH_LIMIT;
+            }
+        }
+        else if (C.bytes_cap && L.total > C.bytes_cap)
+            return SP_CORRELATION_MEMORY_LIMIT;
         for (int a = 0; a != 3; ++a) {
             out_first[a] = g.first[a];
             out_shape[a] = g.padded[a];
@@ -576,12 +629,29 @@ int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, st
         report->precision = prec;
         report->error_bound = prec == 32 ? bound : 0.0;
         if (capacity < g.padded_count || !out_values) return SP_CAPACITY;
-        C.scratch[1].ensure(L.total - L.inputs_end);
-        unsigned char* base = C.scratch[1].as<unsigned char>();
+        unsigned char* base;
+        std::uint64_t origin;
+        if (noalloc) {
+            base = in_base;
+            origin = 0;
+        }
+        else {
+            C.scratch[1].ensure(L.total - L.inputs_end);
+            base = C.scratch[1].as<unsigned char>();
+            origin = L.inputs_end;
+        }
         auto go = [&](auto tag) {
             using T = decltype(tag);
-            Buffers<T> B = buffers_at<T>(base, L, L.inputs_end);
-            for (int a = 0; a != 3; ++a) B.tw[a] = twiddles<T>(g.fft[a]);
+            Buffers<T> B = buffers_at<T>(base, L, origin);
+            for (int a = 0; a != 3; ++a) {
+                if (noalloc) {
+                    cx<T>* t = reinterpret_cast<cx<T>*>(base + L.tw[a]);
+                    upload_twiddles<T>(C.stream, t, g.fft[a]);
+                    B.tw[a] = t;
+                }
+                else
+                    B.tw[a] = twiddles<T>(g.fft[a]);
+            }
             return run<T, E>(C, *spec, g, denv, dker, B, binary, max_out, expected_mass, int_mass, lim, out_values,
                              *report, *stats);
         };
@@ -597,6 +667,20 @@ int correlate(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const E* env, st
 }
 
 }  // namespace
+
+// Set-up of the allocation-free scratch for one device (the library's
+// context creation calls it with that device current): its address on that
+// device, once.
+int init_noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return SP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(g_scratch_init_mu);
+    if (g_scratch_ptr[device]) return SP_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SP_DEVICE_ERROR;
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    return SP_OK;
+}
 
 }  // namespace sphip
 
@@ -630,7 +714,21 @@ int sp_hip_correlate_binary(sp_hip_ctx* ctx, const sp_correlation_spec* spec, co
     if (precision != SP_PRECISION_AUTO && precision != SP_PRECISION_FP32 && precision != SP_PRECISION_FP64)
         return SP_INVALID_ARGUMENT;
     return sphip::correlate<std::uint8_t>(ctx, spec, environment, environment_count, kernel, kernel_count, limits,
-                                          precision, out_values, out_capacity, out_first, out_shape, report, stats);
+                                          precision, out_values, out_capacity, out_first, out_shape, report, stats,
+                                          false);
+}
+
+int sp_hip_correlate_binary_noalloc(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const uint8_t* environment,
+                                    uint64_t environment_count, const uint8_t* kernel, uint64_t kernel_count,
+                                    const sp_correlation_limits* limits, int precision, double* out_values,
+                                    uint64_t out_capacity, int64_t out_first[3], uint32_t out_shape[3],
+                                    sp_numeric_report* report, sp_correlation_stats* stats)
+{
+    if (precision != SP_PRECISION_AUTO && precision != SP_PRECISION_FP32 && precision != SP_PRECISION_FP64)
+        return SP_INVALID_ARGUMENT;
+    return sphip::correlate<std::uint8_t>(ctx, spec, environment, environment_count, kernel, kernel_count, limits,
+                                          precision, out_values, out_capacity, out_first, out_shape, report, stats,
+                                          true);
 }
 
 int sp_hip_correlate_proximity(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const double* environment,
@@ -640,7 +738,8 @@ int sp_hip_correlate_proximity(sp_hip_ctx* ctx, const sp_correlation_spec* spec,
                                sp_correlation_stats* stats)
 {
     return sphip::correlate<double>(ctx, spec, environment, environment_count, kernel, kernel_count, limits,
-                                    SP_PRECISION_FP64, out_values, out_capacity, out_first, out_shape, report, stats);
+                                    SP_PRECISION_FP64, out_values, out_capacity, out_first, out_shape, report, stats,
+                                    false);
 }
 
 }  // extern "C"

@@ -151,6 +151,7 @@ const char* sp_hip_status_code(int status)
     case SP_FIELD_INDEX_OVERFLOW: return "FIELD_INDEX_OVERFLOW";
     case SP_FFT_LENGTH_UNSUPPORTED: return "FFT_LENGTH_UNSUPPORTED";
     case SP_CAPACITY: return "CAPACITY";
+    case SP_SCRATCH_LIMIT: return "SCRATCH_LIMIT";
     default: return "UNKNOWN";
     }
 }
@@ -168,6 +169,12 @@ int sp_hip_ctx_create(int device, uint64_t bytes_cap, sp_hip_ctx** out)
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return SP_DEVICE_ERROR;
+    }
+    if (const int rc = sphip::init_noalloc_scratch(device); rc != SP_OK) {
+        ctx->c.last_error = "hipGetSymbolAddress of the allocation-free scratch failed";
+        (void)hipStreamDestroy(ctx->c.stream);
+        delete ctx;
+        return rc;
     }
     *out = ctx;
     return SP_OK;

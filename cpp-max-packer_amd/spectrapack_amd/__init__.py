@@ -161,13 +161,15 @@ class Context:
     def estimate(self, spec, fft_threads=1, metadata_bytes=0):
         return estimate(spec, fft_threads, metadata_bytes)
 
-    def correlate(self, spec, environment, kernel, limits=None, precision=SP_PRECISION_AUTO, proximity=False):
+    def correlate(self, spec, environment, kernel, limits=None, precision=SP_PRECISION_AUTO, proximity=False,
+                  noalloc=False):
         """correlate_binary_cpu / correlate_proximity_cpu on the device.
 
         limits: (max_working_bytes, reserved_bytes, max_padded_cells,
-        max_direct_terms[, metadata_bytes]).  Returns dict(translation_first,
-        shape, values, numeric, stats); raises Failure with the reference's
-        code string otherwise."""
+        max_direct_terms[, metadata_bytes]).  noalloc: the allocation-free
+        entry point (binary only).  Returns dict(translation_first, shape,
+        values, numeric, stats); raises Failure with the reference's code
+        string otherwise."""
         s = _spec(spec)
         env = np.ascontiguousarray(environment, dtype=np.float64 if proximity else np.uint8).ravel()
         ker = np.ascontiguousarray(kernel, dtype=np.uint8).ravel()
@@ -187,7 +189,8 @@ class Context:
                                                   _p(out, C.c_double), C.c_uint64(out.size), first, shape, C.byref(rep),
                                                   C.byref(st))
         else:
-            rc = lib().sp_hip_correlate_binary(self._h, C.byref(s), _p(env, C.c_uint8), C.c_uint64(env.size), _p(ker, C.c_uint8),
+            fn = lib().sp_hip_correlate_binary_noalloc if noalloc else lib().sp_hip_correlate_binary
+            rc = fn(self._h, C.byref(s), _p(env, C.c_uint8), C.c_uint64(env.size), _p(ker, C.c_uint8),
                     C.c_uint64(ker.size), C.byref(lim), C.c_int(precision), _p(out, C.c_double), C.c_uint64(out.size),
                     first, shape, C.byref(rep), C.byref(st))
         stats = dict(padded_cells=st.padded_cells, working_bytes_peak=st.working_bytes_peak,

@@ -56,7 +56,8 @@ enum sp_status {
     SP_FIELD_INPUT = 34,      /* conservative_fields.cpp FIELD_INPUT */
     SP_FIELD_INDEX_OVERFLOW = 35,
     SP_FFT_LENGTH_UNSUPPORTED = 36, /* a padded length above the kernels' maximum */
-    SP_CAPACITY = 37          /* caller buffer too small; required size reported */
+    SP_CAPACITY = 37,         /* caller buffer too small; required size reported */
+    SP_SCRATCH_LIMIT = 38     /* the allocation-free correlation needs more than its static scratch */
 };
 
 /* Returns the reference's code string for a status ("CORRELATION_NUMERIC", ...). */
@@ -139,6 +140,24 @@ int sp_hip_correlate_binary(sp_hip_ctx* ctx, const sp_correlation_spec* spec, co
                             const sp_correlation_limits* limits, int precision, double* out_values,
                             uint64_t out_capacity, int64_t out_first[3], uint32_t out_shape[3],
                             sp_numeric_report* report, sp_correlation_stats* stats);
+
+/* No counterpart in the reference: sp_hip_correlate_binary for callers that
+ * cannot allocate device memory (code recorded into a HIP graph, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, checks and
+ * results; every device buffer of the call, twiddle tables included, lives in
+ * one static device array of SP_NOALLOC_SCRATCH_BYTES per GPU (held from the
+ * library's first use on that GPU until the process ends), and the call
+ * makes no device allocation.  A call that needs more returns
+ * SP_SCRATCH_LIMIT (64 MiB fits, e.g., an FP32 correlation on a 128^3 FFT
+ * grid).  Calls on one GPU share its array: the library serialises the calls
+ * of one process, and work that overlaps it on that GPU from outside these
+ * calls races. */
+#define SP_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int sp_hip_correlate_binary_noalloc(sp_hip_ctx* ctx, const sp_correlation_spec* spec, const uint8_t* environment,
+                                    uint64_t environment_count, const uint8_t* kernel, uint64_t kernel_count,
+                                    const sp_correlation_limits* limits, int precision, double* out_values,
+                                    uint64_t out_capacity, int64_t out_first[3], uint32_t out_shape[3],
+                                    sp_numeric_report* report, sp_correlation_stats* stats);
 
 /* correlation.hpp:113-117 correlate_proximity_cpu: environment values in [0,1]
  * (double), FP64 on the device (the reference pins these to 1e-10 relative). */
