@@ -209,7 +209,25 @@ def test_cube_catalog_closed_form(oracle):
         R = W.rotation_matrix(v)
         assert np.allclose(np.abs(R).sum(axis=0), 1) and np.isclose(np.linalg.det(R), 1)
         assert np.allclose(R, np.round(R))
-    assert np.allclose(q[0], [0, 0, 0, 1]) or True
+    # the 24 canonical quaternions in lexicographic order of their signed
+    # permutation matrices (orientation_cube.hpp:147-218): every matrix differs,
+    # and the order is fixed by the published cube45 digest below, whose first
+    # 24 entries are this catalog
+    assert len({tuple(np.round(W.rotation_matrix(v)).astype(int).ravel()) for v in q}) == 24
+
+
+def catalog_digest(q):
+    """engine/pack_io/src/result_builder.cpp:190-208: SHA-256 of the compact
+    JSON {"version":1,"quaternions_xyzw":[[x,y,z,w],...]} with zeros as 0.0
+    (shortest round-trip doubles, as nlohmann::json dumps them)."""
+    import hashlib
+    import json
+    rows = [[0.0 if v == 0 else float(v) for v in r] for r in np.asarray(q)]
+    text = json.dumps({"version": 1, "quaternions_xyzw": rows}, separators=(",", ":"))
+    return hashlib.sha256(text.encode()).hexdigest()
+
+
+CUBE45_SHA256 = "7434d4e2eb8cd12e6a67af3a5d636a7435a4e50da043777586ff0fbff9eae4ef"  # docs/ALGORITHM.md:19
 
 
 def test_cube45_catalog(oracle):
@@ -217,6 +235,20 @@ def test_cube45_catalog(oracle):
     assert q.shape == (96, 4)
     assert np.allclose(np.linalg.norm(q, axis=1), 1)
     assert len({tuple(np.round(v, 12)) for v in q}) == 96
+    # the published digest pins every bit of every quaternion and their order
+    assert catalog_digest(q) == CUBE45_SHA256
+    # cardinal orientations stay first (docs/ALGORITHM.md:20)
+    assert np.array_equal(q[:24], oracle.cube_catalog())
+
+
+def test_planted_catalog_errors_change_the_digest(oracle):
+    q = oracle.cube45_catalog()
+    swapped = q.copy()
+    swapped[[0, 1]] = swapped[[1, 0]]
+    assert catalog_digest(swapped) != CUBE45_SHA256
+    nudged = q.copy()
+    nudged[30, 0] = np.nextafter(nudged[30, 0], 2.0)
+    assert catalog_digest(nudged) != CUBE45_SHA256
 
 
 def test_custom_catalog_dedup(oracle):
@@ -261,27 +293,72 @@ def test_raster_closed_cells_touching_surface(oracle):
             assert c[i[2], i[1], i[0]] == 1
 
 
-def test_raster_interior_matches_solid(oracle):
-    """Cells wholly inside the star solid are occupied and cells whose closed box misses it are free,
-    judged by exact radial containment of the star-shaped mesh at cell corners."""
+def cuboid_cell_relation(field, first, pitch, half, R, margin=1e-9):
+    """Exact relation of a rotated cuboid (half extents `half`, world = R local)
+    to the closed cells of a kernel window: per cell, whether a separating axis
+    of the 15 SAT axes (3 cell axes, 3 cuboid axes, 9 cross products) splits
+    them by more than `margin`, and whether all eight cell corners lie strictly
+    inside the cuboid.  Returns (disjoint_marked, inside_free): the counts of
+    cells the field marks although they are disjoint from the solid, and of
+    cells the field leaves free although they lie inside it."""
+    kz, ky, kx = field.shape
+    z, y, x = np.meshgrid(np.arange(kz), np.arange(ky), np.arange(kx), indexing="ij")
+    lo = (np.stack([x, y, z], -1) + np.asarray(first)) * pitch
+    centre = lo + pitch / 2
+    axes = [np.eye(3)[a] for a in range(3)] + [R[:, i] for i in range(3)]
+    for a in range(3):
+        for i in range(3):
+            c = np.cross(np.eye(3)[a], R[:, i])
+            if np.linalg.norm(c) > 1e-12:
+                axes.append(c)
+    separated = np.zeros(field.shape, bool)
+    for ax in axes:
+        r_box = sum(half[i] * abs(ax @ R[:, i]) for i in range(3))
+        r_cell = pitch / 2 * np.abs(ax).sum()
+        separated |= np.abs(centre @ ax) > r_box + r_cell + margin * np.linalg.norm(ax)
+    corners = np.stack([lo + pitch * np.array(d) for d in itertools.product((0, 1), repeat=3)], -2)
+    local = corners @ R  # R^T p for every corner
+    inside = np.all(np.abs(local) < np.asarray(half) - margin, axis=(-1, -2))
+    return int((separated & (field != 0)).sum()), int((inside & (field == 0)).sum()), int(separated.sum())
+
+
+CUBOID_POSES = [((7.3, 5.1, 3.7), (0.3, -0.1, 0.2, 0.9), 0.5), ((10.0, 6.0, 4.0), (0.1, 0.7, -0.2, 0.6), 1.0),
+                ((4.2, 9.5, 2.6), (-0.5, 0.4, 0.6, 0.45), 0.7)]
+
+
+@pytest.mark.parametrize("dims,quat,pitch", CUBOID_POSES)
+def test_raster_rotated_cuboid_exact_sat(oracle, dims, quat, pitch):
+    """At rotated poses, every closed cell that the exact 15-axis SAT separates
+    from the cuboid is free, and every cell wholly inside it is occupied
+    (validation_kernel.cpp:2038-2079 marks exactly the cells its interval SAT
+    cannot separate; conservative_fields.cpp:919-1003 fills the interior)."""
     from spectrapack_amd import workloads as W
-    m = W.cuboid(7.3, 5.1, 3.7)
-    q = W.canonical([0.3, -0.1, 0.2, 0.9])
-    f, s, c = oracle.voxelize_object(m.vertices, m.triangles, q, 0.5)
+    m = W.cuboid(*dims)
+    q = W.canonical(list(quat))
+    f, s, c = oracle.voxelize_object(m.vertices, m.triangles, q, pitch)
     R = W.rotation_matrix(q)
-    half = np.array([7.3, 5.1, 3.7]) / 2
-    for z in range(0, s[2]):
-        for y in range(0, s[1]):
-            for x in range(0, s[0]):
-                lo = (np.array([x, y, z]) + f) * 0.5
-                corners = np.array([lo + 0.5 * np.array(d) for d in itertools.product((0, 1), repeat=3)])
-                local = corners @ R  # R^T p
-                inside = np.all(np.abs(local) < half - 1e-9, axis=1)
-                if inside.all():
-                    assert c[z, y, x] == 1
-                elif (np.abs(local) > half + 1e-9).any(axis=1).all() and not inside.any():
-                    # all corners outside is not sufficient for disjointness; skip ambiguous cells
-                    pass
+    marked, free_inside, n_sep = cuboid_cell_relation(c, f, pitch, np.array(dims) / 2, R)
+    assert marked == 0 and free_inside == 0
+    assert n_sep > 0 and c.sum() > 0
+
+
+def test_planted_raster_dilation_is_caught(oracle):
+    """A raster that over-marks by one cell (the oracle's field dilated along
+    x, as a dropped SAT axis would) fails the exact check, and so does a
+    field whose interior was not filled."""
+    from scipy import ndimage
+    from spectrapack_amd import workloads as W
+    dims, quat, pitch = CUBOID_POSES[0]
+    m = W.cuboid(*dims)
+    q = W.canonical(list(quat))
+    f, s, c = oracle.voxelize_object(m.vertices, m.triangles, q, pitch)
+    R = W.rotation_matrix(q)
+    half = np.array(dims) / 2
+    grown = ndimage.binary_dilation(c, structure=np.ones((1, 1, 3), bool)).astype(np.uint8)
+    assert cuboid_cell_relation(grown, f, pitch, half, R)[0] > 0
+    # a fill stage that leaves the interior empty
+    hollow = (c & ~ndimage.binary_erosion(c, structure=np.ones((3, 3, 3), bool), iterations=2)).astype(np.uint8)
+    assert cuboid_cell_relation(hollow, f, pitch, half, R)[1] > 0
 
 
 def test_box_container_closed_form(oracle):
@@ -296,18 +373,71 @@ def test_box_container_closed_form(oracle):
     assert c2[3:63, 3:63, 3:63].sum() == 0 and c2.sum() == 66 ** 3 - 60 ** 3
 
 
-def test_mesh_container_sphere(oracle):
+def sphere_container_violations(c, mesh, first, pitch, rng, margin=1e-9):
+    """Cells of a convex closed mesh container that break the container rule
+    (conservative_fields.cpp:1200-1275, free only if certified inside):
+    (free cells that hold a sampled surface point, blocked cells whose eight
+    corners lie strictly inside every face plane)."""
+    v, t = mesh.vertices, mesh.triangles
+    a, b, cc = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    n = np.cross(b - a, cc - a)
+    ctr = v.mean(axis=0)
+    n *= np.sign(((a - ctr) * n).sum(axis=1))[:, None]  # outward
+    off = (n * a).sum(axis=1)
+    # surface samples: 12 per face
+    u = rng.uniform(0, 1, (len(t), 12, 2))
+    flip = u.sum(-1) > 1
+    u[flip] = 1 - u[flip]
+    pts = (a[:, None] + u[..., :1] * (b - a)[:, None] + u[..., 1:] * (cc - a)[:, None]).reshape(-1, 3)
+    idx = np.floor(pts / pitch).astype(int) - np.asarray(first)
+    surface_free = int((c[idx[:, 2], idx[:, 1], idx[:, 0]] == 0).sum())
+    kz, ky, kx = c.shape
+    z, y, x = np.meshgrid(np.arange(kz), np.arange(ky), np.arange(kx), indexing="ij")
+    lo = (np.stack([x, y, z], -1) + np.asarray(first)) * pitch
+    inside = np.ones(c.shape, bool)
+    for d in itertools.product((0, 1), repeat=3):
+        p = (lo + pitch * np.array(d)).reshape(-1, 3)
+        inside &= (p @ n.T - off < -margin * np.linalg.norm(n, axis=1)).all(axis=1).reshape(c.shape)
+    inside_blocked = int((inside & (c != 0)).sum())
+    return surface_free, inside_blocked, int(inside.sum())
+
+
+@pytest.mark.parametrize("pitch", [1.0, 0.7])
+def test_mesh_container_sphere(oracle, pitch):
+    from spectrapack_amd import workloads as W
+    sph = W.sphere(20.0, n_seg=32, n_lat=15)
+    # the UV sphere is convex: every vertex lies on the inner side of every face plane
+    v, t = sph.vertices, sph.triangles
+    nrm = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    nrm *= np.sign(((v[t[:, 0]] - v.mean(axis=0)) * nrm).sum(axis=1))[:, None]
+    assert ((v @ nrm.T - (nrm * v[t[:, 0]]).sum(axis=1)) < 1e-9 * np.abs(nrm).max()).all()
+    first, shape = W.mesh_window(sph, pitch)
+    c = oracle.mesh_container(sph.vertices, sph.triangles, first, shape, pitch)
+    lo, hi = sph.bounds()
+    ctr = (lo + hi) / 2
+    z, y, x = np.meshgrid(*[np.arange(s) for s in shape[::-1]], indexing="ij")
+    centres = (np.stack([x + first[0] + 0.5, y + first[1] + 0.5, z + first[2] + 0.5], -1)) * pitch - ctr
+    r = np.linalg.norm(centres, axis=-1)
+    assert (c[r > 20 + pitch] == 1).all()
+    surface_free, inside_blocked, n_inside = sphere_container_violations(c, sph, first, pitch,
+                                                                         np.random.default_rng(3))
+    assert surface_free == 0 and inside_blocked == 0
+    assert n_inside > 0.5 * (c == 0).sum()
+
+
+def test_planted_container_errors_are_caught(oracle):
+    """A container whose blocked cells are eroded leaves surface cells free; a
+    dilated one blocks cells wholly inside: each fails the check above."""
+    from scipy import ndimage
     from spectrapack_amd import workloads as W
     sph = W.sphere(20.0, n_seg=32, n_lat=15)
     first, shape = W.mesh_window(sph, 1.0)
     c = oracle.mesh_container(sph.vertices, sph.triangles, first, shape, 1.0)
-    lo, hi = sph.bounds()
-    ctr = (lo + hi) / 2
-    z, y, x = np.meshgrid(*[np.arange(s) for s in shape[::-1]], indexing="ij")
-    centres = np.stack([x + first[0] + 0.5, y + first[1] + 0.5, z + first[2] + 0.5], -1) - ctr
-    r = np.linalg.norm(centres, axis=-1)
-    assert (c[r < 20 * np.cos(np.pi / 15) - 1.0] == 0).all()
-    assert (c[r > 20 + 1.0] == 1).all()
+    eroded = ndimage.binary_erosion(c, structure=ndimage.generate_binary_structure(3, 1),
+                                    border_value=1).astype(np.uint8)
+    assert sphere_container_violations(eroded, sph, first, 1.0, np.random.default_rng(3))[0] > 0
+    dilated = ndimage.binary_dilation(c, structure=ndimage.generate_binary_structure(3, 1)).astype(np.uint8)
+    assert sphere_container_violations(dilated, sph, first, 1.0, np.random.default_rng(3))[1] > 0
 
 
 def test_proximity_edt_brute_force(oracle):
