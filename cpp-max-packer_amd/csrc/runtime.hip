@@ -152,6 +152,8 @@ const char* sp_hip_status_code(int status)
     case SP_FFT_LENGTH_UNSUPPORTED: return "FFT_LENGTH_UNSUPPORTED";
     case SP_CAPACITY: return "CAPACITY";
     case SP_SCRATCH_LIMIT: return "SCRATCH_LIMIT";
+    case SP_FIELD_COPY_ID: return "FIELD_COPY_ID";
+    case SP_FIELD_COUNT_UNDERFLOW: return "FIELD_COUNT_UNDERFLOW";
     default: return "UNKNOWN";
     }
 }

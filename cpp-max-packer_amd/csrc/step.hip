@@ -11,6 +11,7 @@
 #include <functional>
 #include <memory>
 #include <numeric>
+#include <set>
 #include <type_traits>
 
 #include "common.hpp"
@@ -25,6 +26,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr std::size_t kMaxExcluded = 65536;
+constexpr unsigned int kCopyLogCap = 1u << 20;  // copies whose footprint a session keeps (removable copies)
 constexpr std::size_t kLdsBudget = 64 * 1024;
 
 // Largest tile width (power of two, <= cap) whose LDS footprint fits.
@@ -86,17 +88,33 @@ __global__ void k_verify_choice(const unsigned char* __restrict__ env, int ex, i
     if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(&r->flags, 4u);
 }
 
+// Placed copies (conservative_fields.cpp:1666-1887, BlockedField): every
+// stamped copy is appended to a log of its keys (log[0] = copies stamped,
+// log[1 + id] = key of copy id while id < capacity) and adds one to the placed
+// count of every cell of its stamp footprint.  A cell is blocked while the
+// environment mask blocks it or its placed count is nonzero.
+struct CopyTrack {
+    unsigned int* counts;     // placed count per environment cell (null: not tracked)
+    unsigned long long* log;
+    unsigned int log_cap;
+};
+
 // ORs the chosen kernel into the occupancy (gap_fill.cpp:98-121 for the chosen
 // pose) and adds the newly set cells to the environment mass.  A step whose
 // checks failed (any flag) places nothing, as a failed correlation places
 // nothing in the reference.
 __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsigned char* __restrict__ kcells,
-                        const OrientDesc* __restrict__ descs, StepDeviceResult* r, unsigned char* collect = nullptr,
-                        int cx = 0, int cy = 0, int mx = 0, int my = 0, int mz = 0,
+                        const OrientDesc* __restrict__ descs, StepDeviceResult* r, CopyTrack track,
+                        unsigned char* collect = nullptr, int cx = 0, int cy = 0, int mx = 0, int my = 0, int mz = 0,
                         unsigned long long* collect_mass = nullptr, int shift = 0)
 {
     const unsigned long long key = r->key;
     if (key == ~0ULL || r->flags != 0) return;
+    if (track.log && blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned long long id = track.log[0];
+        if (id < track.log_cap) track.log[1 + id] = key;
+        track.log[0] = id + 1;
+    }
     const int o = int(key & 0xffffULL);
     const int ux = int((key >> 16) & 0xffffULL), uy = int((key >> 32) & 0xffffULL), uz = int(key >> 48);
     const OrientDesc d = descs[o];
@@ -107,7 +125,9 @@ __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsign
         const int x = int(i % d.kx), y = int((i / d.kx) % d.ky), z = int(i / ((long long)d.kx * d.ky));
         const int gx = ux - shift + x, gy = uy - shift + y, gz = uz - shift + z;
         if (gx < 0 || gy < 0 || gz < 0 || gx >= ex || gy >= ey || gz >= ez) continue;
-        unsigned char* c = env + gx + (long long)ex * (gy + (long long)ey * gz);
+        const long long idx = gx + (long long)ex * (gy + (long long)ey * gz);
+        if (track.counts) atomicAdd(track.counts + idx, 1u);
+        unsigned char* c = env + idx;
         if (*c == 0) {
             *c = 1;
             ++added;
@@ -118,6 +138,56 @@ __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsign
     if (added) {
         atomicAdd(&r->env_mass, added);
         if (collect_mass) atomicAdd(collect_mass, added);
+    }
+}
+
+// conservative_fields.cpp:1848-1887 BlockedField::remove: subtracts one placed
+// count over the footprint of the copy with `key` (the stamp kernel at the
+// corner minus `shift`); a cell whose count reaches zero and that the mask
+// leaves free becomes free, is collected at stamp-kernel coordinates into
+// the update window (when given) and leaves the environment mass.  A count
+// that was already zero sets bit 0 of *fault (the count invariant).
+__global__ void k_unstamp(unsigned char* env, const unsigned char* __restrict__ mask, int ex, int ey, int ez,
+                          const unsigned char* __restrict__ kcells, const OrientDesc* __restrict__ descs,
+                          unsigned long long key, unsigned int* counts, StepDeviceResult* r, unsigned int* fault,
+                          unsigned char* collect, int cx, int cy, unsigned long long* collect_mass, int shift)
+{
+    const int o = int(key & 0xffffULL);
+    const int ux = int((key >> 16) & 0xffffULL), uy = int((key >> 32) & 0xffffULL), uz = int(key >> 48);
+    const OrientDesc d = descs[o];
+    const long long n = (long long)d.kx * d.ky * d.kz;
+    unsigned long long freed = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (!kcells[d.cells_offset + i]) continue;
+        const int x = int(i % d.kx), y = int((i / d.kx) % d.ky), z = int(i / ((long long)d.kx * d.ky));
+        const int gx = ux - shift + x, gy = uy - shift + y, gz = uz - shift + z;
+        if (gx < 0 || gy < 0 || gz < 0 || gx >= ex || gy >= ey || gz >= ez) continue;
+        const long long idx = gx + (long long)ex * (gy + (long long)ey * gz);
+        const unsigned int old = atomicSub(counts + idx, 1u);
+        if (old == 0u) {
+            atomicAdd(counts + idx, 1u);
+            atomicOr(fault, 1u);
+            continue;
+        }
+        if (old == 1u && mask[idx] == 0 && env[idx] != 0) {
+            env[idx] = 0;
+            ++freed;
+            if (collect) collect[x + (long long)cx * (y + (long long)cy * z)] = 1;
+        }
+    }
+    if (freed) {
+        atomicAdd(&r->env_mass, 0ULL - freed);
+        if (collect_mass) atomicAdd(collect_mass, freed);
+    }
+}
+
+// After a removal every orientation's feasible set may grow: each minimum is
+// searched again from the first corner.
+__global__ void k_rescan_all(unsigned long long* __restrict__ best, int* __restrict__ rescan, int orientations)
+{
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < orientations; o += gridDim.x * blockDim.x) {
+        best[o] = 0ULL;
+        rescan[o] = 1;
     }
 }
 
@@ -168,14 +238,14 @@ __device__ __forceinline__ int unwrap(int s, int n, int k) { return s < n - (k -
 
 __global__ void k_apply_update(int* __restrict__ F, const OrientDesc* __restrict__ md, const long long* __restrict__ moff,
                                const int* __restrict__ U, const OrientDesc* __restrict__ ud,
-                               const long long* __restrict__ uoff, const StepDeviceResult* __restrict__ res, int nxu,
-                               int nyu, int nzu, const unsigned long long* __restrict__ best, int* __restrict__ rescan,
-                               int shift)
+                               const long long* __restrict__ uoff, const unsigned long long* __restrict__ origin,
+                               int nxu, int nyu, int nzu, const unsigned long long* __restrict__ best,
+                               int* __restrict__ rescan, int shift, int sign)
 {
     const int o = blockIdx.y;
     const OrientDesc d = md[o], e = ud[o];
     if (d.lx <= 0 || e.lx <= 0) return;
-    const unsigned long long key = res->key;
+    const unsigned long long key = *origin;
     const int ox = int((key >> 16) & 0xffffULL) - shift, oy = int((key >> 32) & 0xffffULL) - shift,
               oz = int(key >> 48) - shift;
     const long long n = (long long)e.lx * e.ly * e.lz;
@@ -188,7 +258,7 @@ __global__ void k_apply_update(int* __restrict__ F, const OrientDesc* __restrict
         const int tx = ox + unwrap(int(i % e.lx), nxu, d.kx), ty = oy + unwrap(int((i / e.lx) % e.ly), nyu, d.ky),
                   tz = oz + unwrap(int(i / ((long long)e.lx * e.ly)), nzu, d.kz);
         if (tx < 0 || ty < 0 || tz < 0 || tx >= d.lx || ty >= d.ly || tz >= d.lz) continue;
-        f[tx + (long long)d.lx * (ty + (long long)d.ly * tz)] += v;
+        f[tx + (long long)d.lx * (ty + (long long)d.ly * tz)] += sign * v;
         if (pack_key(tx, ty, tz, o) == b) rescan[o] = 1;
     }
 }
@@ -495,7 +565,25 @@ public:
         SP_HIP_CHECK(hipHostMalloc(&host_result_, sizeof(StepDeviceResult), hipHostMallocDefault));
         excl_.allocate(sizeof(unsigned long long) * (1 + kMaxExcluded));
         SP_HIP_CHECK(hipMemsetAsync(excl_.get(), 0, sizeof(unsigned long long), stream_));
+        // sessions of the C ABI track their placed copies; the update window
+        // and the proximity session (which share a stream) never stamp
+        tracks_copies_ = shared_stream == nullptr;
+        if (tracks_copies_) {
+            mask_.allocate(env_.bytes());
+            SP_HIP_CHECK(hipMemsetAsync(mask_.get(), 0, mask_.bytes(), stream_));
+            counts_.allocate(env_.bytes() * sizeof(unsigned int));
+            SP_HIP_CHECK(hipMemsetAsync(counts_.get(), 0, counts_.bytes(), stream_));
+            copylog_.allocate(sizeof(unsigned long long) * (1 + kCopyLogCap));
+            SP_HIP_CHECK(hipMemsetAsync(copylog_.get(), 0, sizeof(unsigned long long), stream_));
+            fault_.allocate(sizeof(unsigned int));
+        }
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    CopyTrack track() const
+    {
+        if (!tracks_copies_) return CopyTrack { nullptr, nullptr, 0u };
+        return CopyTrack { counts_.as<unsigned int>(), copylog_.as<unsigned long long>(), kCopyLogCap };
     }
     ~Session()
     {
@@ -512,6 +600,14 @@ public:
             if (cells[i] > 1) throw std::domain_error("CORRELATION_INVALID_ENVIRONMENT");
         use_device(ctx_);
         SP_HIP_CHECK(hipMemcpyAsync(env_.get(), cells, count, hipMemcpyHostToDevice, stream_));
+        if (tracks_copies_) {
+            // the given occupancy is the fixed mask; placed copies start over
+            SP_HIP_CHECK(hipMemcpyAsync(mask_.get(), env_.get(), count, hipMemcpyDeviceToDevice, stream_));
+            SP_HIP_CHECK(hipMemsetAsync(counts_.get(), 0, counts_.bytes(), stream_));
+            SP_HIP_CHECK(hipMemsetAsync(copylog_.get(), 0, sizeof(unsigned long long), stream_));
+            removed_.clear();
+            frozen_copies_ = 0;
+        }
         std::uint64_t mass = 0;
         for (std::uint64_t i = 0; i != count; ++i) mass += cells[i];
         StepDeviceResult r0 {};
@@ -696,6 +792,119 @@ public:
         mark("verify");
     }
 
+    // ---- placed copies (conservative_fields.cpp:1666-1887) -----------------
+    std::uint64_t copy_count()
+    {
+        if (!tracks_copies_) return 0;
+        unsigned long long n = 0;
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(&n, copylog_.get(), sizeof n, hipMemcpyDeviceToHost, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        return n;
+    }
+
+    struct CopyInfo {
+        unsigned long long key;
+        bool removed;
+    };
+
+    // Every copy stamped since the environment was set, in stamp order
+    // (copy id = position); copies beyond the log's capacity are counted
+    // but not listed.
+    std::vector<CopyInfo> copies()
+    {
+        const std::uint64_t n = copy_count();
+        const std::uint64_t kept = std::min<std::uint64_t>(n, kCopyLogCap);
+        std::vector<unsigned long long> keys(kept);
+        if (kept) {
+            SP_HIP_CHECK(hipMemcpyAsync(keys.data(), copylog_.as<unsigned long long>() + 1, kept * sizeof(unsigned long long),
+                                        hipMemcpyDeviceToHost, stream_));
+            SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        }
+        std::vector<CopyInfo> out(kept);
+        for (std::uint64_t i = 0; i != kept; ++i) out[i] = CopyInfo { keys[i], removed_.count(i) != 0 };
+        return out;
+    }
+
+    void key_pose(unsigned long long key, std::uint32_t& o, std::int64_t t[3]) const
+    {
+        o = std::uint32_t(key & 0xffffULL);
+        const std::int64_t u[3] = { std::int64_t((key >> 16) & 0xffffULL), std::int64_t((key >> 32) & 0xffffULL),
+                                     std::int64_t(key >> 48) };
+        for (int a = 0; a != 3; ++a) t[a] = efirst_[a] + u[a] - kernels_[o].first[a];
+    }
+
+    // BlockedField::remove (conservative_fields.cpp:1848-1887): the copy's
+    // placed counts are subtracted; cells that no copy and no mask block any
+    // longer become free.  Incremental fields subtract the freed cells'
+    // correlation, and every orientation's minimum is searched again.
+    void remove_copy(std::uint64_t id)
+    {
+        if (!tracks_copies_) throw std::invalid_argument("FIELD_COPY_ID");
+        const std::uint64_t n = copy_count();
+        if (id >= n || id >= kCopyLogCap || removed_.count(id)) throw std::invalid_argument("FIELD_COPY_ID");
+        prepare();
+        if (id < frozen_copies_) throw std::invalid_argument("FIELD_COPY_ID");  // stamped under other stamp kernels
+        use_device(ctx_);
+        unsigned long long key = 0;
+        SP_HIP_CHECK(hipMemcpyAsync(&key, copylog_.as<unsigned long long>() + 1 + id, sizeof key, hipMemcpyDeviceToHost,
+                                    stream_));
+        SP_HIP_CHECK(hipMemsetAsync(fault_.get(), 0, sizeof(unsigned int), stream_));
+        const bool inc = incremental_ && inc_ready_ && inc_effective_;
+        if (inc && pending_) apply_pending();
+        const bool dil = halo_ > 0;
+        const unsigned char* kc = dil ? scells_.as<unsigned char>() : kcells_.as<unsigned char>();
+        const OrientDesc* kd = dil ? sdescs_.as<OrientDesc>() : d_descs_.as<OrientDesc>();
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        if (inc) {
+            Session& u = *upd_;
+            SP_HIP_CHECK(hipMemsetAsync(u.env_.get(), 0, u.env_.bytes(), stream_));
+            StepDeviceResult* ur = u.result_.as<StepDeviceResult>();
+            SP_HIP_CHECK(hipMemsetAsync(&ur->env_mass, 0, sizeof(ur->env_mass), stream_));
+            hipLaunchKernelGGL(k_unstamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(),
+                               mask_.as<unsigned char>(), ext_[0], ext_[1], ext_[2], kc, kd, key,
+                               counts_.as<unsigned int>(), result_.as<StepDeviceResult>(), fault_.as<unsigned int>(),
+                               u.env_.as<unsigned char>(), u.ext_[0], u.ext_[1], &ur->env_mass, halo_);
+            // the removed copy's key as the update window's origin
+            rm_key_.ensure(sizeof(unsigned long long));
+            SP_HIP_CHECK(hipMemcpyAsync(rm_key_.get(), &key, sizeof key, hipMemcpyHostToDevice, stream_));
+            u.launch_mode(false, false);
+            hipLaunchKernelGGL(k_apply_update, dim3(64, O_), dim3(kThreads), 0, stream_, fields_.as<int>(),
+                               d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), u.fields_.as<int>(),
+                               u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(),
+                               rm_key_.as<unsigned long long>(), u.n_[0], u.n_[1], u.n_[2],
+                               orient_best_.as<unsigned long long>(), rescan_.as<int>(), halo_, -1);
+            hipLaunchKernelGGL(k_rescan_all, dim3(1), dim3(256), 0, stream_, orient_best_.as<unsigned long long>(),
+                               rescan_.as<int>(), O_);
+            hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
+                               d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(), rescan_.as<int>());
+        }
+        else
+            hipLaunchKernelGGL(k_unstamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(),
+                               mask_.as<unsigned char>(), ext_[0], ext_[1], ext_[2], kc, kd, key,
+                               counts_.as<unsigned int>(), result_.as<StepDeviceResult>(), fault_.as<unsigned int>(),
+                               nullptr, 0, 0, nullptr, halo_);
+        SP_HIP_CHECK(hipGetLastError());
+        unsigned int fault = 0;
+        SP_HIP_CHECK(hipMemcpyAsync(&fault, fault_.get(), sizeof fault, hipMemcpyDeviceToHost, stream_));
+        StepDeviceResult ur {};
+        if (inc) SP_HIP_CHECK(hipMemcpyAsync(&ur, upd_->result_.get(), sizeof ur, hipMemcpyDeviceToHost, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        removed_.insert(id);
+        ++env_rev_;
+        if (fault) throw std::domain_error("FIELD_COUNT_UNDERFLOW");
+        if (ur.flags) throw std::domain_error("CORRELATION_NUMERIC");
+    }
+
+    void placed_counts(std::uint32_t* out, std::uint64_t count)
+    {
+        if (count != std::uint64_t(ext_[0]) * ext_[1] * ext_[2]) throw std::invalid_argument("environment size");
+        if (!tracks_copies_) throw std::invalid_argument("no placed copies");
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(out, counts_.get(), count * sizeof(std::uint32_t), hipMemcpyDeviceToHost, stream_));
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
     // Stamps the result's key: the kernel, or with a pair clearance its
     // dilated stamp kernel at the corner minus the halo; optionally collects
     // the newly set cells (at stamp-kernel coordinates) and counts them.
@@ -705,7 +914,7 @@ public:
         hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
                            ext_[2], dil ? scells_.as<unsigned char>() : kcells_.as<unsigned char>(),
                            dil ? sdescs_.as<OrientDesc>() : d_descs_.as<OrientDesc>(), result_.as<StepDeviceResult>(),
-                           collect, cx, cy, 0, 0, 0, cmass, halo_);
+                           track(), collect, cx, cy, 0, 0, 0, cmass, halo_);
     }
 
     // Pair clearance between placed copies (0: none).
@@ -831,8 +1040,9 @@ public:
         u.launch_mode(false, false);
         hipLaunchKernelGGL(k_apply_update, dim3(64, O_), dim3(kThreads), 0, stream_, fields_.as<int>(),
                            d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), u.fields_.as<int>(),
-                           u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(), result_.as<StepDeviceResult>(),
-                           u.n_[0], u.n_[1], u.n_[2], orient_best_.as<unsigned long long>(), rescan_.as<int>(), halo_);
+                           u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(),
+                           &result_.as<StepDeviceResult>()->key, u.n_[0], u.n_[1], u.n_[2],
+                           orient_best_.as<unsigned long long>(), rescan_.as<int>(), halo_, 1);
         hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
                            d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(), rescan_.as<int>());
         SP_HIP_CHECK(hipGetLastError());
@@ -914,6 +1124,9 @@ public:
     void prepare()
     {
         if (prepared_) return;
+        // copies stamped before the stamp kernels are rebuilt keep footprints
+        // the new kernels may not reproduce: they can no longer be removed
+        if (tracks_copies_) frozen_copies_ = copy_count();
         for (const auto& k : kernels_)
             if (!k.set) throw std::invalid_argument("kernel not set");
         std::uint64_t max_mass = 0;
@@ -2089,6 +2302,10 @@ private:
     std::size_t max_lds_ { 65536 };
     int cus_ { 1 };  // compute units of the session's device
     int h2p_ { 0 };  // H2 row pitch (values)
+    bool tracks_copies_ {};
+    DeviceBuffer mask_, counts_, copylog_, fault_, rm_key_;
+    std::set<std::uint64_t> removed_;
+    std::uint64_t frozen_copies_ {};
     std::vector<hipEvent_t> prof_events_;
     std::vector<const char*> prof_names_;
     // occupancy revision (set_environment and every stamp) and ranking state
@@ -2146,6 +2363,8 @@ template <typename F> int guarded(sp_hip_ctx* ctx, F&& f)
         const std::string w = e.what();
         if (w == "CORRELATION_INVALID_ENVIRONMENT") return fail(ctx, SP_CORRELATION_INVALID_ENVIRONMENT, w);
         if (w == "CORRELATION_INVALID_KERNEL") return fail(ctx, SP_CORRELATION_INVALID_KERNEL, w);
+        if (w == "FIELD_COUNT_UNDERFLOW") return fail(ctx, SP_FIELD_COUNT_UNDERFLOW, w);
+        if (w == "CORRELATION_NUMERIC") return fail(ctx, SP_CORRELATION_NUMERIC, w);
         return fail(ctx, SP_INVALID_ARGUMENT, w);
     }
     catch (const std::length_error& e) {
@@ -2153,6 +2372,10 @@ template <typename F> int guarded(sp_hip_ctx* ctx, F&& f)
     }
     catch (const std::bad_alloc&) {
         return fail(ctx, SP_CORRELATION_ALLOCATION, "host allocation failed");
+    }
+    catch (const std::invalid_argument& e) {
+        if (std::string(e.what()) == "FIELD_COPY_ID") return fail(ctx, SP_FIELD_COPY_ID, "copy id is not present");
+        return fail(ctx, SP_INVALID_ARGUMENT, e.what());
     }
     catch (const std::exception& e) {
         return fail(ctx, SP_INVALID_ARGUMENT, e.what());
@@ -2310,6 +2533,35 @@ int sp_hip_session_stamp(sp_hip_session* s, uint32_t orientation, const int64_t 
 {
     if (!s || !translation) return SP_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] { s->s->stamp_at(orientation, translation); });
+}
+
+int sp_hip_session_copies(sp_hip_session* s, sp_placed_copy* out, uint64_t capacity, uint64_t* count)
+{
+    if (!s || !count || (capacity && !out)) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        const auto all = s->s->copies();
+        *count = all.size();
+        if (all.size() > capacity) throw std::length_error("capacity");
+        for (std::size_t i = 0; i != all.size(); ++i) {
+            sp_placed_copy c {};
+            c.copy_id = i;
+            s->s->key_pose(all[i].key, c.orientation, c.translation);
+            c.removed = all[i].removed ? 1u : 0u;
+            out[i] = c;
+        }
+    });
+}
+
+int sp_hip_session_remove_copy(sp_hip_session* s, uint64_t copy_id)
+{
+    if (!s) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->remove_copy(copy_id); });
+}
+
+int sp_hip_session_placed_counts(sp_hip_session* s, uint32_t* counts, uint64_t count)
+{
+    if (!s || !counts) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->placed_counts(counts, count); });
 }
 
 int sp_hip_session_keep_raw(sp_hip_session* s, int keep)

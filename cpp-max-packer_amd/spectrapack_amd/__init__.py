@@ -79,6 +79,11 @@ class Placement(C.Structure):
                 ("score", C.c_double)]
 
 
+class PlacedCopy(C.Structure):
+    _fields_ = [("copy_id", C.c_uint64), ("translation", C.c_int64 * 3), ("orientation", C.c_uint32),
+                ("removed", C.c_uint32)]
+
+
 VALIDATE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(C.c_int64))
 
 PAGE_SIZE = 32
@@ -385,6 +390,28 @@ class Session:
     def exclude(self, orientation, translation):
         t = (C.c_int64 * 3)(*[int(v) for v in translation])
         self._check(lib().sp_hip_session_exclude(self._h, C.c_uint32(orientation), t))
+
+    def copies(self):
+        """Every copy stamped since the environment was set (BlockedField
+        footprints): [dict(copy_id, orientation, translation, removed)]."""
+        n = C.c_uint64(0)
+        rc = lib().sp_hip_session_copies(self._h, None, C.c_uint64(0), C.byref(n))
+        if rc and rc != 37:
+            self._check(rc)
+        out = (PlacedCopy * max(n.value, 1))()
+        self._check(lib().sp_hip_session_copies(self._h, out, C.c_uint64(n.value), C.byref(n)))
+        return [dict(copy_id=int(out[i].copy_id), orientation=int(out[i].orientation),
+                     translation=tuple(out[i].translation), removed=bool(out[i].removed)) for i in range(n.value)]
+
+    def remove_copy(self, copy_id):
+        """BlockedField::remove of one placed copy (conservative_fields.cpp:1848-1887)."""
+        self._check(lib().sp_hip_session_remove_copy(self._h, C.c_uint64(copy_id)))
+
+    def placed_counts(self):
+        ex, ey, ez = self.env_shape
+        a = np.zeros(ex * ey * ez, np.uint32)
+        self._check(lib().sp_hip_session_placed_counts(self._h, _p(a, C.c_uint32), C.c_uint64(a.size)))
+        return a.reshape(ez, ey, ex)
 
     def keep_raw(self, keep=True):
         self._check(lib().sp_hip_session_keep_raw(self._h, C.c_int(1 if keep else 0)))

@@ -57,7 +57,9 @@ enum sp_status {
     SP_FIELD_INDEX_OVERFLOW = 35,
     SP_FFT_LENGTH_UNSUPPORTED = 36, /* a padded length above the kernels' maximum */
     SP_CAPACITY = 37,         /* caller buffer too small; required size reported */
-    SP_SCRATCH_LIMIT = 38     /* the allocation-free correlation needs more than its static scratch */
+    SP_SCRATCH_LIMIT = 38,    /* the allocation-free correlation needs more than its static scratch */
+    SP_FIELD_COPY_ID = 39,    /* conservative_fields.cpp FIELD_COPY_ID: no such (removable) copy */
+    SP_FIELD_COUNT_UNDERFLOW = 40 /* conservative_fields.cpp FIELD_COUNT_UNDERFLOW */
 };
 
 /* Returns the reference's code string for a status ("CORRELATION_NUMERIC", ...). */
@@ -216,6 +218,29 @@ int sp_hip_session_step(sp_hip_session* s, int stamp, sp_step_result* out);
  * that orientation in every later step (a rejected proposal). */
 int sp_hip_session_stamp(sp_hip_session* s, uint32_t orientation, const int64_t translation[3]);
 int sp_hip_session_exclude(sp_hip_session* s, uint32_t orientation, const int64_t translation[3]);
+/* Placed copies (conservative_fields.hpp:127-164, conservative_fields.cpp:1666-1887
+ * BlockedField add / remove / placed_count).  The occupancy given to
+ * set_environment is the fixed mask.  Every copy the session stamps (a step's
+ * choice, or sp_hip_session_stamp) gets the next copy id (0, 1, ... since the
+ * environment was set) and adds one to the placed count of each cell of its
+ * stamp footprint (its kernel, dilated by the pair clearance).  A cell is
+ * blocked while the mask blocks it or its count is nonzero.  remove_copy
+ * subtracts the copy's counts and frees the cells no copy and no mask block
+ * any longer; the next step sees the new occupancy (incremental sessions
+ * subtract the freed cells' correlation from their fields).  FIELD_COPY_ID:
+ * the id was never stamped, is removed already, lies beyond the first 2^20
+ * copies, or was stamped before the kernels or the pair clearance changed.
+ * copies lists every stamped copy in id order (*count = copies; CAPACITY
+ * when capacity is smaller); placed_counts reads the per-cell counts. */
+typedef struct {
+    uint64_t copy_id;
+    int64_t translation[3];
+    uint32_t orientation;
+    uint32_t removed;
+} sp_placed_copy;
+int sp_hip_session_copies(sp_hip_session* s, sp_placed_copy* out, uint64_t capacity, uint64_t* count);
+int sp_hip_session_remove_copy(sp_hip_session* s, uint64_t copy_id);
+int sp_hip_session_placed_counts(sp_hip_session* s, uint32_t* counts, uint64_t count);
 /* Enqueue `steps` steps without reading results back (for throughput timing);
  * sp_hip_session_last_result then reads the last one, and returns
  * CORRELATION_NUMERIC when any step since the previous read failed a check

@@ -523,3 +523,55 @@ def test_clearance_stencil_closed_form(oracle):
     # clearance exactly on a gap boundary stays inclusive (outward rounding)
     assert (2, 0, 0) in {tuple(v) for v in oracle.clearance_stencil(1.0, 1.0).tolist()}
     assert (3, 0, 0) not in {tuple(v) for v in oracle.clearance_stencil(1.0, 1.0).tolist()}
+
+
+def sol02_blocker(oracle):
+    """conservative_fields_tests.cpp:63-91 (SOL-02): the cuboid [.1, .2]^3 placed at
+    translation (3, 3, 3) in the 7^3 window at 1 mm with a 1.5 mm pair clearance."""
+    from spectrapack_amd import workloads as W
+    import blocked as B
+    box = W.cuboid(0.1, 0.1, 0.1)
+    verts = box.vertices - box.vertices.min(axis=0) + 0.1
+    f, s, c = oracle.voxelize_object(verts, box.triangles, [0, 0, 0, 1], 1.0)
+    kernel = dict(first=f, shape=s, cells=c)
+    mask = oracle.box_container((0, 0, 0), (7, 7, 7), 1.0, (7.0, 7.0, 7.0))
+    stencil = oracle.clearance_stencil(1.0, 1.5)
+    fp = oracle.stamp(np.zeros_like(mask), (0, 0, 0), kernel, (3, 3, 3), stencil)
+    return B, mask, fp, kernel
+
+
+def test_blocked_field_sol02(oracle):
+    """The reference's own SOL-02 sequence: full Euclidean pair clearance and
+    transactional overlapping counts."""
+    B, mask, fp, _ = sol02_blocker(oracle)
+    assert mask.sum() == 0
+    assert fp[3, 3, 5] == 1 and fp[3, 5, 5] == 1 and fp[5, 5, 5] == 0  # offsets (2,0,0), (2,2,0), (2,2,2)
+    b = B.BlockedField(mask)
+    b.add("a", fp)
+    b.add("b", fp)
+    assert b.placed_count((3, 3, 5)) == 2
+    with pytest.raises(B.FieldFailure) as e:
+        b.remove("missing")
+    assert e.value.code == "FIELD_COPY_ID"
+    with pytest.raises(B.FieldFailure):
+        b.add("a", fp)
+    b.remove("a")
+    assert b.placed_count((3, 3, 5)) == 1 and b.blocked()[3, 3, 5] == 1
+    b.remove("b")
+    assert b.placed_count((3, 3, 5)) == 0 and b.blocked().sum() == 0
+
+
+def test_planted_count_error_is_caught(oracle):
+    """A field that clears a copy's cells on removal instead of counting them
+    down leaves (5,3,3) free while copy "b" still covers it."""
+    B, mask, fp, _ = sol02_blocker(oracle)
+
+    class Clearing(B.BlockedField):
+        def remove(self, copy_id):
+            self.counts.flat[self.footprints.pop(copy_id)] = 0
+
+    b = Clearing(mask)
+    b.add("a", fp)
+    b.add("b", fp)
+    b.remove("a")
+    assert b.placed_count((3, 3, 5)) != 1
