@@ -58,7 +58,8 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     # C++20 host mirror of engine/pack_compute over the C ABI (no HIP headers)
-    host_srcs = [os.path.join(HOST, f) for f in ("correlation_hip.cpp", "placement.cpp")
+    host_srcs = [os.path.join(HOST, f) for f in ("correlation_hip.cpp", "placement.cpp", "exact_validation.cpp",
+                                                  "validate_capi.cpp")
                  if os.path.exists(os.path.join(HOST, f))]
     if host_srcs:
         tmp = HOST_LIB + ".tmp"

@@ -4,6 +4,8 @@
 //   spectrapack_host_test place <input.txt>  — the placement loop on a scene
 //     read from whitespace-separated numbers; prints one line per proposal
 //     "P orientation tx ty tz verdict" and a final "S steps proposals accepted outcome".
+//   spectrapack_host_test place-exact <input.txt> — the placement loop with the
+//     exact host validation as its accept verdict
 //   spectrapack_host_test rank <input.txt>   — the ranked loop on a scene (the
 //     same numbers, then origin_z cmin_z cmax_z, one max z per orientation,
 //     capacity and reject_every); prints "R orientation tx ty tz score_bits"
@@ -353,8 +355,61 @@ static int place(const char* path)
     return 0;
 }
 
+// place-exact: the scene of `place`, then capacity, the part mesh (vertex
+// count, vertices, triangle count, triangles) and the box container's
+// dimensions; place_parts with the exact host validation as the accept
+// verdict.  Prints "P orientation tx ty tz verdict code" per proposal, the
+// summary "S steps proposals accepted outcome" and "F code" for the exact
+// validation of all accepted copies together.
+static int place_exact(const char* path)
+{
+    std::ifstream in(path);
+    sg::Environment env;
+    std::vector<sg::OrientedKernel> kernels;
+    if (!read_scene(in, env, kernels)) {
+        std::printf("bad input\n");
+        return 2;
+    }
+    std::uint64_t capacity;
+    std::size_t nv, nt;
+    in >> capacity >> nv;
+    spectrapack::geometry::exact::Mesh part;
+    part.vertices.resize(nv);
+    for (auto& v : part.vertices) in >> v[0] >> v[1] >> v[2];
+    in >> nt;
+    part.triangles.resize(nt);
+    for (auto& t : part.triangles) in >> t[0] >> t[1] >> t[2];
+    spectrapack::geometry::exact::BoxContainer box;
+    in >> box.dims_mm[0] >> box.dims_mm[1] >> box.dims_mm[2];
+    if (!in) {
+        std::printf("bad input\n");
+        return 2;
+    }
+    spectrapack::geometry::exact::Validator validator(part, box);
+    spectrapack::geometry::exact::ValidationReport last;
+    const auto accept = sg::exact_accept(validator, &last);
+    std::vector<spectrapack::geometry::exact::Pose> accepted;
+    const auto stats = sg::place_parts(env, kernels, sg::PlacementRequest { capacity, 0 }, {},
+                                       [&](const sg::Proposal& p) {
+                                           const sg::GapVerdict v = accept(p);
+                                           std::printf("P %u %lld %lld %lld %d %s\n", p.orientation,
+                                                       (long long)p.translation[0], (long long)p.translation[1],
+                                                       (long long)p.translation[2], int(v),
+                                                       std::string(last.code).c_str());
+                                           if (v == sg::GapVerdict::accepted)
+                                               accepted.push_back({ p.translation_mm, p.rotation_xyzw });
+                                           return v;
+                                       });
+    std::printf("S %llu %llu %llu %s\n", (unsigned long long)stats.steps, (unsigned long long)stats.proposals,
+                (unsigned long long)stats.accepted, std::string(stats.outcome).c_str());
+    const auto all = spectrapack::geometry::exact::validate_placements(part, accepted, box);
+    std::printf("F %s %llu\n", std::string(all.code).c_str(), (unsigned long long)all.pair_tests);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc >= 3 && std::string(argv[1]) == "place-exact") return place_exact(argv[2]);
     if (argc >= 2 && std::string(argv[1]) == "correlate") return correlate_checks();
     if (argc >= 2 && std::string(argv[1]) == "estimate") return estimate_checks();
     if (argc >= 3 && std::string(argv[1]) == "place") return place(argv[2]);

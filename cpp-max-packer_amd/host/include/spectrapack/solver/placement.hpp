@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "spectrapack/compute/correlation.hpp"
+#include "spectrapack/geometry/exact_validation.hpp"
 #include "spectrapack/runtime/operation_control.hpp"
 
 struct sp_hip_session;
@@ -76,6 +77,15 @@ PlacementStats place_parts(const Environment& environment, std::span<const Orien
                            const PlacementRequest& request, const runtime::OperationControl& control,
                            const std::function<GapVerdict(const Proposal&)>& accept,
                            compute::FftExecution* execution = nullptr, std::vector<std::uint8_t>* final_occupancy = nullptr);
+
+// The exact host validation as place_parts' accept verdict (the native
+// validation callback of gap_fill.cpp:512-523, validation.cpp:517-634): a
+// proposal is accepted when `validator` certifies its pose (rotation of its
+// orientation, translation pitch * t) against the container and every copy
+// accepted before, and rejected otherwise (invalid or unresolved).  The last
+// report is kept in *last when given.
+std::function<GapVerdict(const Proposal&)> exact_accept(geometry::exact::Validator& validator,
+                                                        geometry::exact::ValidationReport* last = nullptr);
 
 struct RankingRequest {
     std::uint64_t capacity {};

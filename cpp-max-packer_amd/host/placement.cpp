@@ -222,6 +222,19 @@ RankingStats rank_parts(const Environment& env, std::span<const OrientedKernel> 
     return stats;
 }
 
+std::function<GapVerdict(const Proposal&)> exact_accept(geometry::exact::Validator& validator,
+                                                        geometry::exact::ValidationReport* last)
+{
+    return [&validator, last](const Proposal& p) {
+        geometry::exact::Pose pose;
+        pose.translation_mm = p.translation_mm;
+        pose.rotation_xyzw = p.rotation_xyzw;
+        const auto r = validator.try_add(pose);
+        if (last) *last = r;
+        return r.validity == geometry::exact::Validity::valid ? GapVerdict::accepted : GapVerdict::rejected;
+    };
+}
+
 std::vector<OrientedKernel> voxelize_kernels(std::span<const std::array<double, 3>> vertices,
                                              std::span<const std::array<std::uint32_t, 3>> triangles,
                                              std::span<const std::array<double, 4>> rotations, double pitch_mm,

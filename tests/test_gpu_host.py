@@ -135,3 +135,41 @@ def test_cpp_ranked_loop(tmp_path, ctx, reject_every, capacity):
     assert int(summary[1]) == len(want) and int(summary[2]) == calls[0]
     assert summary[3] == ("SPECTRAL_COPY_LIMIT" if len(want) == capacity else "SPECTRAL_PIPELINE_COMPLETE")
     assert int(summary[4]) == int(s.environment().sum())
+
+
+def test_cpp_placement_with_exact_validation(tmp_path, oracle):
+    """place_parts with the exact host validation as its accept verdict
+    (gap_fill.cpp:512-523 with validation.cpp:517-634): an L-bracket at 12
+    seeded rotations in a 40 mm box; every spectral proposal is certified
+    valid, so the loop equals the plain one, and the accepted copies validate
+    together."""
+    from spectrapack_amd import workloads as W
+    dims = (40.0, 40.0, 40.0)
+    first, shape = W.box_window(dims, 1.0)
+    env = oracle.box_container(first, shape, 1.0, dims)
+    part = W.l_bracket(11, 4.0, 12.0)
+    cat = W.so3_catalog(12, seed=5)
+    kernels = []
+    for q in cat:
+        f, s, c = oracle.voxelize_object(part.vertices, part.triangles, q, 1.0)
+        kernels.append(dict(q=q, first=f, shape=s, cells=c))
+    tail = f"40 {len(part.vertices)} " + " ".join(repr(float(v)) for v in part.vertices.ravel()) + \
+        f" {len(part.triangles)} " + " ".join(str(int(v)) for v in part.triangles.ravel()) + \
+        " " + " ".join(repr(float(v)) for v in dims) + "\n"
+    path = tmp_path / "scene.txt"
+    write_scene(path, env, first, 1.0, kernels, 0, 0, 0, tail=tail)
+    print("[host] place-exact bracket in 40 mm box", flush=True)
+    r = subprocess.run([DRIVER, "place-exact", str(path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    props = [l.split() for l in lines if l.startswith("P ")]
+    assert props and all(p[5] == "0" and p[6] == "VALID" for p in props), props[:5]
+    summary = [l for l in lines if l.startswith("S ")][0].split()
+    assert int(summary[3]) == len(props)
+    assert [l for l in lines if l.startswith("F ")][0].split()[1] == "VALID"
+    # the same proposals as the plain oracle loop
+    cur = env.copy()
+    for p in props:
+        o, t, _ = oracle.place_step(cur, first, kernels, direct=True)
+        assert (o, tuple(int(v) for v in t)) == (int(p[1]), (int(p[2]), int(p[3]), int(p[4])))
+        cur = oracle.stamp(cur, first, kernels[o], t)

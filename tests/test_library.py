@@ -8,12 +8,12 @@ import re
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
+def declared_symbols(prefix="sp_hip_"):
     names = set()
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names.update(re.findall(r"\b(sp_hip_\w+)\s*\(", text))
+        names.update(re.findall(r"\b(" + prefix + r"\w+)\s*\(", text))
     return sorted(names)
 
 
@@ -42,3 +42,12 @@ def test_fp32_bound_monotone_and_selects():
     assert sp.fp32_error_bound(n, n, 1000) < sp.fp32_error_bound(n, n, 10000)
     assert sp.fp32_error_bound(n, n, 20000) < 0.4
     assert sp.fp32_error_bound(512 ** 3, 512 ** 3, 200000) > 0.4
+
+
+def test_host_library_exports_the_validation_abi():
+    from spectrapack_amd import validation
+    lib = ctypes.CDLL(validation.LIB_PATH)
+    names = declared_symbols("sp_validat")
+    assert "sp_validate_placements" in names and "sp_validator_try_add" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
