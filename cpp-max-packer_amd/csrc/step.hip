@@ -1831,6 +1831,85 @@ public:
         return v;
     }
 
+    // Back-to-back launches of one pass of the step (as a rocprof kernel trace
+    // sees them): `launches` launches recorded in one graph and replayed
+    // between two events on the session stream, after a step has filled its
+    // inputs.  Every pass rewrites the same output from the same inputs, so
+    // the session's state is unchanged.  Returns ms per launch.
+    float time_pass(const std::string& name, int launches)
+    {
+        if (launches < 1 || launches > 1000) throw std::invalid_argument("launches");
+        prepare();
+        use_device(ctx_);
+        ensure_kernel_spectra();
+        autotune();
+        const int* act = act_full_.as<int>();
+        const int slots = nfit_;
+        if (slots == 0 || lmax_[0] == 0) throw std::invalid_argument("no orientation fits");
+        std::function<void()> one;
+        if (name == "mul_x")
+            one = [&] {
+                if (precision_ == 64) run_pass<double>(kPassMul, tile_[kPassMul], act, slots);
+                else run_pass<float>(kPassMul, tile_[kPassMul], act, slots);
+            };
+        else if (name == "yz_argmin") {
+            if (!fused_yz()) throw std::invalid_argument("the fused pass does not run on this session");
+            one = [&] { enqueue_yz<float>(act, slots); };
+        }
+        else if (name == "inv_c2c_y" || name == "c2r_z_argmin") {
+            if (fused_yz()) throw std::invalid_argument("the separate inverse passes do not run on this session");
+            const int pass = name == "inv_c2c_y" ? kPassInvY : kPassC2r;
+            one = [&, pass] {
+                if (precision_ == 64) run_pass<double>(pass, tile_[pass], act, slots);
+                else run_pass<float>(pass, tile_[pass], act, slots);
+            };
+        }
+        else
+            throw std::invalid_argument("unknown pass");
+        // inputs of every pass: one step without stamping
+        launch_mode(false, false);
+        one();
+        SP_HIP_CHECK(hipGetLastError());
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        SP_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int i = 0; i != launches; ++i) one();
+        }
+        catch (...) {
+            (void)hipStreamEndCapture(stream_, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        SP_HIP_CHECK(hipStreamEndCapture(stream_, &graph));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        float ms = 0.f;
+        try {
+            SP_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            SP_HIP_CHECK(hipGraphLaunch(exec, stream_));  // warm-up replay
+            SP_HIP_CHECK(hipEventCreate(&e0));
+            SP_HIP_CHECK(hipEventCreate(&e1));
+            SP_HIP_CHECK(hipEventRecord(e0, stream_));
+            SP_HIP_CHECK(hipGraphLaunch(exec, stream_));
+            SP_HIP_CHECK(hipEventRecord(e1, stream_));
+            SP_HIP_CHECK(hipEventSynchronize(e1));
+            SP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        }
+        catch (...) {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (exec) (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        return ms / float(launches);
+    }
+
     // One step without the graph, timing each kernel between events.
     std::vector<std::pair<std::string, float>> profile_step(bool stamp)
     {
@@ -2632,6 +2711,12 @@ int sp_hip_session_profile_step(sp_hip_session* s, int stamp, float* kernel_ms, 
             }
         }
     });
+}
+
+int sp_hip_session_time_pass(sp_hip_session* s, const char* pass, uint32_t launches, float* ms)
+{
+    if (!s || !pass || !ms) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { *ms = s->s->time_pass(pass, int(launches)); });
 }
 
 void* sp_hip_session_stream(sp_hip_session* s) { return s ? static_cast<void*>(s->s->stream()) : nullptr; }

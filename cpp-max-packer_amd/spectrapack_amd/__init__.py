@@ -574,6 +574,12 @@ class Session:
         raw = names.raw
         return [(raw[32 * i:32 * i + 32].split(b"\0")[0].decode(), float(ms[i])) for i in range(n.value)]
 
+    def time_pass(self, name, launches=20):
+        """ms per launch of one pass of the step, launched back to back in a graph."""
+        ms = C.c_float()
+        self._check(lib().sp_hip_session_time_pass(self._h, name.encode(), C.c_uint32(launches), C.byref(ms)))
+        return float(ms.value)
+
     def stream(self):
         return lib().sp_hip_session_stream(self._h)
 

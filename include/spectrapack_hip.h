@@ -324,6 +324,13 @@ int sp_hip_session_set_active(sp_hip_session* s, const uint32_t* orientations, u
  * corner relative to environment_first; all ones when the orientation had no
  * feasible translation (or was not searched).  count >= orientations. */
 int sp_hip_session_orientation_minima(sp_hip_session* s, uint64_t* keys, uint32_t count);
+/* Milliseconds per launch of one pass of the step ("mul_x", "yz_argmin",
+ * "inv_c2c_y" or "c2r_z_argmin"), launched back to back: `launches` launches
+ * (1..1000) recorded in one graph and replayed between two events after an
+ * unstamped step has filled the pass's inputs.  Each launch rewrites the same
+ * output from the same inputs, so results and occupancy are unchanged.  No
+ * counterpart in the reference (measurement of this build). */
+int sp_hip_session_time_pass(sp_hip_session* s, const char* pass, uint32_t launches, float* ms);
 /* Stream the session launches on (hipStream_t), for events around its work. */
 void* sp_hip_session_stream(sp_hip_session* s);
 /* Device bytes held by the session. */
