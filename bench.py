@@ -161,6 +161,7 @@ def main():
                     help="run the inverse y and inverse z passes separately (no fused pass)")
     ap.add_argument("--learned-seeds", default="9000,9001,9002",
                     help="held-out part seeds packed with and without proposals")
+    ap.add_argument("--weights", default="", help="proposal weights (.npy) for the learned leg (default: the package's)")
     ap.add_argument("--top-o", type=int, default=8)
     ap.add_argument("--top-r", type=int, default=512)
     ap.add_argument("--only-learned", action="store_true",
@@ -474,11 +475,12 @@ def learned_comparison(ctx, args, plain=True):
     import spectrapack_amd as sp
     from spectrapack_amd import learned as L
     from spectrapack_amd import workloads as W
-    if not os.path.exists(L.DEFAULT_WEIGHTS):
-        return {"skipped": "no trained weights at " + os.path.relpath(L.DEFAULT_WEIGHTS, ROOT)}
+    wpath = args.weights or L.DEFAULT_WEIGHTS
+    if not os.path.exists(wpath):
+        return {"skipped": "no trained weights at " + os.path.relpath(wpath, ROOT)}
     prop = sp.Proposal(ctx, max_batch=1)
-    prop.set_parameters(L.load_weights())
-    meta_path = os.path.splitext(L.DEFAULT_WEIGHTS)[0] + ".json"
+    prop.set_parameters(L.load_weights(wpath))
+    meta_path = os.path.splitext(wpath)[0] + ".json"
     meta = json.load(open(meta_path)) if os.path.exists(meta_path) else {}
     rows = []
     for seed in [int(v) for v in args.learned_seeds.split(",")]:

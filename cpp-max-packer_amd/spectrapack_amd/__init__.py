@@ -137,6 +137,7 @@ class Context:
     """The sp_hip_ctx handle (the role FftExecution plays: correlation.hpp:78-98)."""
 
     def __init__(self, device: int = 0, bytes_cap: int = 0):
+        self.device = int(device)
         self._h = C.c_void_p()
         rc = lib().sp_hip_ctx_create(C.c_int(device), C.c_uint64(bytes_cap), C.byref(self._h))
         if rc:

@@ -167,43 +167,101 @@ class Scene:
         self.session.close()
 
 
-def collect(ctx, proposal: Proposal, seeds, max_steps=100000, log=print, first_slot=0):
+def collect(ctx, proposal: Proposal, seeds, max_steps=100000, log=print, first_slot=0, every=1, incremental=False,
+            threads=1, log_every=1):
     """Plain-search episodes (stamping every choice) recorded as samples in
-    the proposal's bank.  Returns the labels and per-episode statistics."""
-    slot = first_slot
+    the proposal's bank: every `every`-th step is recorded (its input before
+    the step, its choice and every orientation's lowest feasible corner); the
+    steps between run without reading results back.  incremental: the
+    episodes run incremental steps (identical choices).  threads: episodes
+    run concurrently on that many sessions (one stream each).  Returns the
+    labels and per-episode statistics."""
+    import threading
+    lock = threading.Lock()
+    state = dict(slot=first_slot, next=0, done=0, t0=time.perf_counter())
     rec = dict(slot=[], desc=[], target=[], mask=[], region=[], orientation=[], episode=[], tied=[])
-    episodes = []
-    for e, seed in enumerate(seeds):
-        t0 = time.perf_counter()
-        sc = Scene(ctx, W.star_so3_box(int(seed)))
-        s = sc.session
-        steps = 0
-        while steps < max_steps:
-            if slot >= proposal.samples:
+    episodes = [None] * len(seeds)
+    errors = []
+
+    def take_slot():
+        with lock:
+            if state["slot"] >= proposal.samples:
                 raise RuntimeError("sample bank full")
+            state["slot"] += 1
+            return state["slot"] - 1
+
+    local = threading.local()
+
+    def episode(e, seed):
+        t0 = time.perf_counter()
+        # one context per thread: a context serialises its own calls
+        if threads > 1 and not hasattr(local, "ctx"):
+            local.ctx = Context(ctx.device)
+        sc = Scene(local.ctx if threads > 1 else ctx, W.star_so3_box(int(seed)))
+        s = sc.session
+        s.set_autotune(False)  # every tile gives identical results
+        if incremental:
+            s.set_incremental(True)
+        steps = 0
+        rows = []
+        while steps < max_steps:
+            gap = min(every - 1, max_steps - steps - 1)
+            if gap > 0:
+                s.run(gap, stamp=True)
+                steps += gap
+            slot = take_slot()
             proposal.record(s, slot)
             r = s.step(stamp=True)
+            steps += 1
             if r["orientation"] < 0:
                 break
             minima = s.orientation_minima()
             u = tuple(r["key"][a] - sc.w.env_first[a] for a in range(3))
-            rec["slot"].append(slot)
-            rec["desc"].append(sc.desc)
-            rec["target"].append(orientation_target(minima, r["orientation"]))
-            rec["mask"].append(sc.fits)
-            rec["region"].append(region_index(u, sc.w.env_shape))
-            rec["orientation"].append(r["orientation"])
             zmin = min(m[0] for m in minima if m is not None)
-            rec["tied"].append(np.array([m is not None and m[0] == zmin for m in minima] +
-                                        [False] * (96 - len(minima)), bool))
-            rec["episode"].append(e)
-            slot += 1
-            steps += 1
+            rows.append((slot, orientation_target(minima, r["orientation"]), region_index(u, sc.w.env_shape),
+                         r["orientation"], np.array([m is not None and m[0] == zmin for m in minima] +
+                                                    [False] * (96 - len(minima)), bool)))
+        parts = sum(1 for c in s.copies())
         dt = time.perf_counter() - t0
-        episodes.append(dict(seed=int(seed), parts=steps, seconds=dt))
-        log(f"[collect] episode {e + 1}/{len(seeds)} seed={seed} parts={steps} {dt:.1f}s samples={slot - first_slot}")
+        with lock:
+            for slot, target, region, o, tied in rows:
+                rec["slot"].append(slot)
+                rec["desc"].append(sc.desc)
+                rec["target"].append(target)
+                rec["mask"].append(sc.fits)
+                rec["region"].append(region)
+                rec["orientation"].append(o)
+                rec["episode"].append(e)
+                rec["tied"].append(tied)
+            episodes[e] = dict(seed=int(seed), parts=parts, seconds=dt, samples=len(rows))
+            state["done"] += 1
+            if state["done"] % log_every == 0 or state["done"] == len(seeds):
+                log(f"[collect] {state['done']}/{len(seeds)} episodes, seed={seed} parts={parts} "
+                    f"samples={state['slot'] - first_slot} {time.perf_counter() - state['t0']:.1f}s")
         sc.close()
-    out = {k: np.asarray(v) for k, v in rec.items()}
+
+    def worker():
+        while True:
+            with lock:
+                e = state["next"]
+                state["next"] += 1
+            if e >= len(seeds) or errors:
+                return
+            try:
+                episode(e, seeds[e])
+            except Exception as ex:  # reported after the join
+                errors.append(ex)
+                return
+
+    pool = [threading.Thread(target=worker) for _ in range(max(1, threads))]
+    for t in pool:
+        t.start()
+    for t in pool:
+        t.join()
+    if errors:
+        raise errors[0]
+    order = np.lexsort((np.asarray(rec["slot"]), np.asarray(rec["episode"])))
+    out = {k: np.asarray(v)[order] for k, v in rec.items()}
     out["episodes"] = episodes
     return out
 

@@ -23,6 +23,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--episodes", type=int, default=200)
     ap.add_argument("--max-steps", type=int, default=100000)
+    ap.add_argument("--every", type=int, default=1, help="record every n-th step of an episode")
+    ap.add_argument("--threads", type=int, default=1, help="episodes collected concurrently")
+    ap.add_argument("--incremental", action="store_true", help="collect with incremental steps (same choices)")
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -33,6 +36,8 @@ def main(argv=None):
     ap.add_argument("--eval-seeds", type=int, default=3)
     ap.add_argument("--eval-fine", type=int, default=2, help="held-out seeds packed at 256^3")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--seed-base", type=int, default=1000, help="episode seeds are seed-base + 0 .. episodes - 1")
+    ap.add_argument("--skip-packing", action="store_true", help="train only (no packing comparison)")
     ap.add_argument("--out", default="gpurun_out/proposal")
     ap.add_argument("--init", default="", help="start from these weights (.npy)")
     args = ap.parse_args(argv)
@@ -47,7 +52,7 @@ def main(argv=None):
     os.makedirs(args.out, exist_ok=True)
     ctx = Context(0)
     prop = Proposal(ctx, max_batch=args.batch)
-    capacity = args.episodes * 400
+    capacity = args.episodes * (400 // args.every + 2)
     prop.reserve(capacity)
     if args.init:
         flat = np.load(args.init, allow_pickle=False)
@@ -56,9 +61,10 @@ def main(argv=None):
     prop.set_parameters(flat)
     init_flat = prop.parameters()
 
-    seeds = [1000 + i for i in range(args.episodes)]
+    seeds = [args.seed_base + i for i in range(args.episodes)]
     t0 = time.perf_counter()
-    data = L.collect(ctx, prop, seeds, max_steps=args.max_steps, log=log)
+    data = L.collect(ctx, prop, seeds, max_steps=args.max_steps, log=log, every=args.every,
+                     incremental=args.incremental, threads=args.threads, log_every=max(1, args.episodes // 40))
     t_collect = time.perf_counter() - t0
     n = len(data["slot"])
     held = max(1, args.episodes // 10)
@@ -80,11 +86,17 @@ def main(argv=None):
                 heat_weight=args.heat_weight, collect_seconds=t_collect, train_seconds=t_train,
                 history=hist, hits_untrained=before, hits_trained=after, hits_train=train_hits,
                 top_o=args.top_o, top_r=args.top_r, scene="star512-in-box128-h1-o96",
-                catalog_seed=W.PROPOSAL_CATALOG_SEED, episode_seeds=[seeds[0], seeds[-1]])
+                catalog_seed=W.PROPOSAL_CATALOG_SEED, episode_seeds=[seeds[0], seeds[-1]], every=args.every,
+                incremental=args.incremental, init=os.path.basename(args.init) if args.init else None)
     L.save_weights(os.path.join(args.out, "proposal_star96.npy"), flat, meta)
 
     # packing: plain vs learned (and the untrained network) on held-out seeds
     report = dict(meta=meta, packing=[])
+    if args.skip_packing:
+        with open(os.path.join(args.out, "report.json"), "w") as f:
+            json.dump(report, f, indent=1)
+        log("[done]")
+        return 0
     untrained = Proposal(ctx, max_batch=1)
     untrained.set_parameters(init_flat)
     for fine in (False, True):
