@@ -491,14 +491,15 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
     __syncthreads();
     T2* pen = lds + g * kYzPencil;
     static_assert(64 % P == 0, "a pencil's threads are lanes of one wave");
-    // phase A's exchanges touch only each group's own region
-    reg_fft_lean<T, N, R, false>(a, pen, tws, t, false);
-    // the second pencil is loaded only now: hoisting its loads above the first
-    // transform would hold both pencils and that transform's working set at once
-    __builtin_amdgcn_sched_barrier(0);
-    asm volatile("" ::: "memory");
+    // phase A's exchanges touch only each group's own region.  The second
+    // pencil's loads are issued inside the first transform, after its
+    // exchange writes (the first pencil's registers are free there): the
+    // loads then overlap the rest of that transform without holding both
+    // pencils and the transform's working set at once.
+    reg_fft_lean<T, N, R, false>(a, pen, tws, t, false, [&] {
 #pragma unroll
-    for (int q = 0; q < R; ++q) b[q] = nt_load(&p1[P * q]);
+        for (int q = 0; q < R; ++q) b[q] = nt_load(&p1[P * q]);
+    });
     reg_fft_lean<T, N, R, false>(b, pen, tws, t, false);
     const T max_out = T(d.mass < (long long)result->env_mass ? d.mass : (long long)result->env_mass);
     const bool any_excluded = excl[0] != 0;
@@ -541,18 +542,20 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
             __builtin_amdgcn_sched_barrier(0);
         }
         reg_fft_lean<T, N, R>(v, pen, tws, t, false);
-        // checks over every legal value, kept as running extremes (the
-        // residual NaN-propagating, so a NaN raises the residual flag as a
-        // per-value test would); zero counts are marked in bit 2 s + half,
-        // which orders them as their keys (z, then y)
-        // A value outside the legal range is replaced by 0 before rounding
-        // (residual 0, count 0: neutral for the extremes); its zero bit is
-        // cleared afterwards with the lane's legal-bit mask.
+        // checks over every value of the column's circular correlation, kept
+        // as running extremes (the residual NaN-propagating, so a NaN raises
+        // the residual flag as a per-value test would).  Every circular
+        // value, legal or not, is in exact arithmetic an integer in
+        // [0, min(masses)] and lies under the same a-priori bound, so the
+        // checks need no legality test per value (the legal values alone are
+        // the reference's; checking all of them is stricter).  Zero counts
+        // are marked in bit 2 s + half, which orders them as their keys (z,
+        // then y); the bits of values outside the legal range are cleared
+        // afterwards with the lane's legal-bit mask.
         unsigned int zeros = 0;
 #pragma unroll
         for (int s = 0; s < R; ++s) {
-            const bool z_in = t + P * s < d.lz;
-            const T va = z_in && y1 < d.ly ? v[s].x : T(0), vb = z_in && y1 + 1 < d.ly ? v[s].y : T(0);
+            const T va = v[s].x, vb = v[s].y;
             const T ra = rint(va), rb = rint(vb);
             r1 = __builtin_elementwise_maximum(r1, __builtin_elementwise_maximum(fabsf(va - ra), fabsf(vb - rb)));
             lo = fminf(lo, fminf(ra, rb));

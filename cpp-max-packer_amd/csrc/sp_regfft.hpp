@@ -175,8 +175,14 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
 // so the exchange itself needs no block barrier.  SYNC: the pencil's region
 // may still be read by other waves (it aliases a buffer they use), so the
 // block synchronises before the first write.
-template <typename T, int N, int R, bool SYNC = true>
-__device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd)
+struct NoWork {
+    __device__ void operator()() const {}
+};
+
+// `mid` runs between the exchange's writes and its reads, when the pencil's
+// registers hold nothing: a caller can issue loads for later work there.
+template <typename T, int N, int R, bool SYNC = true, typename Mid = NoWork>
+__device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd, Mid mid = {})
 {
     using S = RegShape<N, R>;
     static_assert(S::passes == 2 && R == 16, "one exchange of 16 x 16");
@@ -190,6 +196,7 @@ __device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* 
     // pad(16 t + s) = 17 t + s and pad(16 q + t) = 17 q + t for s, t < 16
 #pragma unroll
     for (int s = 0; s < R; ++s) lds[(R + 1) * t + s] = v[s];
+    mid();
     __builtin_amdgcn_wave_barrier();
     // second pass: l = R, m = 1, j = t, k = 0
 #pragma unroll

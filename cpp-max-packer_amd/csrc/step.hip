@@ -469,23 +469,28 @@ void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>
 
 // P1 with the column-major output of the fused inverse y and z pass (FP32,
 // N = 256, radix 16; the tile's width is kept when it is one of 8, 16, 32).
+// Returns the grid (blocks) and sets *tiles to the row tiles.
 template <typename T>
-void rf_mul_xt_launch(Tile tile, hipStream_t st, int cus, int slots, int ny, int nh, const cx<T>* H, long long h_o,
-                      int h_px, const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
-                      const cx<T>* tw, T sb_scale)
+int rf_mul_xt_launch(Tile tile, hipStream_t st, int cus, int slots, int ny, int nh, const cx<T>* H, long long h_o,
+                     int h_px, const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
+                     const cx<T>* tw, T sb_scale, long long* tiles)
 {
     constexpr int N = 256, R = 16;
+    int used = 0;
     auto go = [&](auto ww) {
         constexpr int W = decltype(ww)::value;
         const std::size_t lds = rf_lds<T, N, R>(W);
+        *tiles = (long long)((ny + W - 1) / W) * nh;
         const int grid = mul_grid(reinterpret_cast<const void*>(&rf_mul_x<T, N, R, W, true>), W * (N / R), lds, cus,
-                                  (long long)((ny + W - 1) / W) * nh, slots);
+                                  *tiles, slots);
+        used = grid;
         hipLaunchKernelGGL((rf_mul_x<T, N, R, W, true>), dim3(grid), dim3(W * (N / R)), lds, st, H, h_o, h_px, SB, ny,
                            nh, out, out_o, descs, act, tw, sb_scale);
     };
     if (tile.r == R && tile.w == 8) go(std::integral_constant<int, 8> {});
     else if (tile.r == R && tile.w == 32) go(std::integral_constant<int, 32> {});
     else go(std::integral_constant<int, 16> {});
+    return used;
 }
 
 template <typename T, typename In>
@@ -1303,8 +1308,8 @@ public:
             if constexpr (std::is_same_v<T, float>) {
                 if (fused_yz()) {
                     // the fused pass's 1/(nx ny nz) = 2^-24, applied exactly to the environment rows
-                    rf_mul_xt_launch<T>(tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act,
-                                        twiddles<T>(nx), T(1) / (T(nx) * T(ny) * T(nz)));
+                    mul_grid_ = rf_mul_xt_launch<T>(tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd,
+                                                    act, twiddles<T>(nx), T(1) / (T(nx) * T(ny) * T(nz)), &mul_tiles_);
                     return;
                 }
             }
@@ -1829,6 +1834,12 @@ public:
         v.descs = d_descs_.as<OrientDesc>();
         v.stream = stream_;
         return v;
+    }
+
+    void product_grid(std::uint32_t& blocks, std::uint64_t& tiles) const
+    {
+        blocks = std::uint32_t(mul_grid_);
+        tiles = std::uint64_t(mul_tiles_);
     }
 
     // Back-to-back launches of one pass of the step (as a rocprof kernel trace
@@ -2381,6 +2392,8 @@ private:
     std::size_t max_lds_ { 65536 };
     int cus_ { 1 };  // compute units of the session's device
     int h2p_ { 0 };  // H2 row pitch (values)
+    int mul_grid_ {};          // blocks of the last column-major product pass
+    long long mul_tiles_ {};   // its row tiles (grid < tiles: shares of tiles x orientations)
     bool tracks_copies_ {};
     DeviceBuffer mask_, counts_, copylog_, fault_, rm_key_;
     std::set<std::uint64_t> removed_;
@@ -2711,6 +2724,12 @@ int sp_hip_session_profile_step(sp_hip_session* s, int stamp, float* kernel_ms, 
             }
         }
     });
+}
+
+int sp_hip_session_product_grid(sp_hip_session* s, uint32_t* blocks, uint64_t* tiles)
+{
+    if (!s || !blocks || !tiles) return SP_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] { s->s->product_grid(*blocks, *tiles); });
 }
 
 int sp_hip_session_time_pass(sp_hip_session* s, const char* pass, uint32_t launches, float* ms)

@@ -575,6 +575,12 @@ class Session:
         raw = names.raw
         return [(raw[32 * i:32 * i + 32].split(b"\0")[0].decode(), float(ms[i])) for i in range(n.value)]
 
+    def product_grid(self):
+        """(blocks, row tiles) of the last column-major product pass."""
+        b, t = C.c_uint32(), C.c_uint64()
+        self._check(lib().sp_hip_session_product_grid(self._h, C.byref(b), C.byref(t)))
+        return int(b.value), int(t.value)
+
     def time_pass(self, name, launches=20):
         """ms per launch of one pass of the step, launched back to back in a graph."""
         ms = C.c_float()

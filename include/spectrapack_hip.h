@@ -331,6 +331,11 @@ int sp_hip_session_orientation_minima(sp_hip_session* s, uint64_t* keys, uint32_
  * output from the same inputs, so results and occupancy are unchanged.  No
  * counterpart in the reference (measurement of this build). */
 int sp_hip_session_time_pass(sp_hip_session* s, const char* pass, uint32_t launches, float* ms);
+/* Grid of the last column-major product pass (fused path): its blocks and its
+ * row tiles; fewer blocks than tiles means each block took an even share of
+ * row tiles x active orientations.  0, 0 before one ran.  No counterpart in
+ * the reference (a scheduling detail of this build, for tests). */
+int sp_hip_session_product_grid(sp_hip_session* s, uint32_t* blocks, uint64_t* tiles);
 /* Stream the session launches on (hipStream_t), for events around its work. */
 void* sp_hip_session_stream(sp_hip_session* s);
 /* Device bytes held by the session. */

@@ -194,19 +194,27 @@ def test_fused_product_pass_tiles(ctx):
 
 
 def test_product_pass_shares_break_inside_row_tiles(ctx):
-    """The product pass shares row tiles x active orientations out evenly over
-    a grid of resident blocks, so a block's share can begin and end inside a
-    row tile.  With 1, 2, 5 and 7 active orientations (shares breaking at
-    different orientations) every active orientation keeps the lowest feasible
-    corner it has when all orientations run, and the others report none, on
-    the fused and on the separate passes at 256^3."""
+    """The column-major product pass (fused path) shares row tiles x active
+    orientations out evenly over a grid of resident blocks, so a block's share
+    can begin and end inside a row tile; the test pins the tile width 16,
+    whose 2064 row tiles at 256^3 do not fill whole rounds of resident blocks,
+    and checks that the shared grid ran.  With 1, 2, 5 and 7 active
+    orientations (shares breaking at different orientations) every active
+    orientation keeps the lowest feasible corner it has when all orientations
+    run, and the others report none.  The separate passes (row-major product
+    pass, one block per row tile) are checked the same way."""
     rs = np.random.default_rng(41)
     kernels = kernels_with_long_axes(rs, small=6)
     env = (rs.uniform(0, 1, (256, 256, 256)) < 0.01).astype(np.uint8)
     for fused in (True, False):
         s = make_session(ctx, env, (0, 0, 0), kernels, fused)
         assert s.fused_passes() == fused
+        if fused:
+            s.set_tile(0, 16, 16)
         s.step(stamp=False)
+        if fused:
+            blocks, tiles = s.product_grid()
+            assert tiles == 2064 and 0 < blocks < tiles, (blocks, tiles)
         full = s.orientation_minima()
         assert sum(m is not None for m in full) >= 6
         for active in ([4], [0, 8], [1, 2, 3, 6, 7], [8, 7, 5, 4, 2, 1, 0]):
