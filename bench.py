@@ -42,10 +42,14 @@ def step_model_bytes(nx, ny, nz, orientations, precision):
     return 48 * nc + orientations * (120 * nc + 4 * N)
 
 
-def kernel_bytes(name, n, kernels, ext, elem):
-    """Algorithmic bytes one launch of each step kernel must move (all orientations)."""
+def kernel_bytes(name, n, kernels, ext, elem, fused=False):
+    """Algorithmic bytes one launch of each step kernel must move (all
+    orientations).  fused: the product pass writes the fused path's
+    column-major output, whose kz = 0 and kz = nz/2 rows share one plane
+    (nz/2 planes per column instead of nz/2 + 1)."""
     nx, ny, nz = n
     nh = nz // 2 + 1
+    planes = nz // 2 if fused else nh
     ex, ey, ez = ext
     fit = [k for k in kernels if all(ext[a] >= k["shape"][a] for a in range(3))]
     if name == "env_r2c_z":
@@ -59,14 +63,16 @@ def kernel_bytes(name, n, kernels, ext, elem):
     if name == "mul_x":
         # each orientation's kernel rows (Kx values) in and legal columns (Lx) out;
         # the environment spectrum once per launch (orientations are batched)
-        return nh * ny * nx * elem + sum(nh * ny * (k["shape"][0] + (ex - k["shape"][0] + 1)) * elem for k in fit)
+        return nh * ny * nx * elem + sum((nh * k["shape"][0] + planes * (ex - k["shape"][0] + 1)) * ny * elem
+                                         for k in fit)
     if name == "inv_c2c_y":
         return sum(nh * (ny + ey - k["shape"][1] + 1) * (ex - k["shape"][0] + 1) * elem for k in fit)
     if name == "c2r_z_argmin":
         return sum(nh * (ey - k["shape"][1] + 1) * (ex - k["shape"][0] + 1) * elem for k in fit)
     if name == "yz_argmin":
-        # fused inverse y + z: every kz line of every legal x column, read once
-        return sum(nh * ny * (ex - k["shape"][0] + 1) * elem for k in fit)
+        # fused inverse y + z: every plane (kz = 0 and nz/2 packed) of every
+        # legal x column, read once
+        return sum((nz // 2) * ny * (ex - k["shape"][0] + 1) * elem for k in fit)
     return 0
 
 
@@ -316,7 +322,7 @@ def main():
     n = tuple(int(sp_len) for sp_len in _fft_shape(w.env_shape))
     elem = 8 if prec == 32 else 16
     dominant = max(prof, key=prof.get)
-    dom_bytes = kernel_bytes(dominant, n, kernels, w.env_shape, elem)
+    dom_bytes = kernel_bytes(dominant, n, kernels, w.env_shape, elem, fused=s.fused_passes())
     # the dominant pass launched back to back (a graph of 20 launches between
     # two events on the session stream), as a rocprof kernel trace times it
     dom_ms = s.time_pass(dominant, 20)

@@ -107,8 +107,13 @@ __global__ void __launch_bounds__(W*(N / R)) rf_rows(cx<T>* data, int rows, long
 // fastest so that every store covers runs of W consecutive y.  The
 // environment rows are multiplied by sb_scale whenever they are loaded (the
 // fused pass takes its 1/(nx ny nz) here, an exact power of two on its grids).
+// XT also packs the two planes whose rows are Hermitian in y, kz = 0 and
+// kz = n/2 (both real in z for a real correlation): plane 0 of the output
+// holds row(kz 0) + i row(kz n/2) and is written by rf_mul_x_plane0, so the
+// output has nh - 1 = n/2 planes and every x column of it is n/2 complex
+// pencils; here XT covers the planes kz = 1 .. n/2 - 1.
 // The work is (row tile, orientation) units, tile-major: nh * ceil(ny / W)
-// tiles of W rows times the active orientations.  The grid is 1-D and block
+// tiles of W rows (nh - 2 with XT) times the active orientations.  The grid is 1-D and block
 // b takes units [b U / G, (b + 1) U / G), loading the environment rows again
 // whenever its range enters a new tile; a grid of one block per tile gives
 // each block one tile with all orientations.  A grid of as many blocks as the
@@ -131,7 +136,9 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
     const int w = XT ? int(threadIdx.x) % W : int(threadIdx.x) / P;
     const int count = act[0];
     const int ytiles = (ny + W - 1) / W;
-    const long long units = (long long)ytiles * nh * count;
+    const int planes = XT ? nh - 1 : nh;       // output planes
+    const int kz0 = XT ? 1 : 0;                // first row plane of the units
+    const long long units = (long long)ytiles * (nh - 2 * kz0) * count;
     const long long u0 = units * blockIdx.x / gridDim.x, u1 = units * (blockIdx.x + 1) / gridDim.x;
     if (u0 >= u1) return;  // uniform across the block
     load_tw_lds<T>(tws, tw, N);
@@ -143,7 +150,7 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
         const long long base = (long long)tile * count;
         const int i0 = int(u - base), i1 = int(min((long long)count, u1 - base));
         u = base + i1;
-        const int kz = tile / ytiles, y = (tile % ytiles) * W + w;
+        const int kz = kz0 + tile / ytiles, y = (tile % ytiles) * W + w;
         const bool live = y < ny;
         const long long row = (long long)kz * ny + y;
         T2 sb[R];
@@ -183,7 +190,7 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
                 // a fresh copy of the column stride per orientation: the store
                 // addresses are then formed here instead of being kept in
                 // registers across the loop
-                long long xs = (long long)nh * ny;
+                long long xs = (long long)planes * ny;
                 asm volatile("" : "+s"(xs));
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
@@ -199,6 +206,67 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
                     if (x < lx) nt_store(&orow[x], v[s]);
                 }
             }
+        }
+    }
+}
+
+
+// Plane 0 of the column-major product output (fused path, see rf_mul_x):
+// row(kz 0) + i row(kz n/2) per (row, orientation).  The inverse x transform
+// is linear, so the two rows' products are combined before it, P0 + i P_half,
+// and a packed row takes three transforms.  One block per (tile of W rows,
+// active orientation slot); lanes as rf_mul_x<..., true> (pencil fastest).
+template <typename T, int N, int R, int W>
+__global__ void __launch_bounds__(W*(N / R))
+rf_mul_x_plane0(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __restrict__ SB, int ny, int nh,
+                cx<T>* __restrict__ out, long long out_o, const OrientDesc* __restrict__ descs,
+                const int* __restrict__ act, const cx<T>* __restrict__ tw, T sb_scale)
+{
+    using T2 = cx<T>;
+    using S = RegShape<N, R>;
+    constexpr int P = S::P;
+    extern __shared__ unsigned char smem_raw[];
+    T2* tws = reinterpret_cast<T2*>(smem_raw);
+    T2* lds = tws + N;
+    if (int(blockIdx.y) >= act[0]) return;  // uniform
+    const int o = act[kActHeader + blockIdx.y];
+    const int kx = descs[o].kx, lx = descs[o].lx;
+    if (lx <= 0) return;  // uniform
+    const int t = int(threadIdx.x) / W, w = int(threadIdx.x) % W;
+    const int y = int(blockIdx.x) * W + w;
+    const bool live = y < ny;
+    load_tw_lds<T>(tws, tw, N);
+    __syncthreads();
+    T2* pl = lds + w * S::stride;
+    const long long rows[2] = { (long long)y, (long long)(nh - 1) * ny + y };
+    T2 p[R];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const T2* hrow = H + o * h_o + rows[h] * h_px;
+        T2 v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int x = t + P * q;
+            v[q] = (live && x < kx) ? nt_load(&hrow[x]) : T2 { 0, 0 };
+        }
+        reg_fft<T, N, R>(v, pl, tws, t, true);
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            const T2 e = live ? SB[rows[h] * N + t + P * s] : T2 { 0, 0 };
+            const T2 q = cmulc(T2 { e.x * sb_scale, e.y * sb_scale }, v[s]);
+            p[s] = h == 0 ? q : T2 { p[s].x - q.y, p[s].y + q.x };
+        }
+        // the second row's loads are not issued while the first is live
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    reg_fft<T, N, R>(p, pl, tws, t, false);
+    if (live) {
+        T2* ocol = out + o * out_o + y;
+        const long long xs = (long long)(nh - 1) * ny;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            const int x = t + P * s;
+            if (x < lx) nt_store(&ocol[x * xs], p[s]);
         }
     }
 }
@@ -404,16 +472,15 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
 
 
 // Fused inverse y + C2R along z + checks + argmin for N = ny = nz = 256 at
-// radix 16.  One block of 1024 threads owns one x column of one orientation:
-// the column's half spectrum in[o][x][kz][y] (129 x 256 values, written by
-// rf_mul_x<..., true> and already scaled by 1/(nx ny nz)) is read once,
-// transformed along y in registers, and transformed along z in two chunks of
-// up to 128 legal rows through LDS, so the inverse y output never goes back
-// to memory.
-//  * y pass: 64 groups of 16 threads, two pencils per group (kz = g and
-//    g + 64).  The kz = 0 and kz = 128 planes of a real correlation are
-//    Hermitian in y, so their columns transform together as one complex
-//    pencil (kz0 + i kz128): the real outputs come back as the real and
+// radix 16.  A block of 1024 threads owns one x column of one orientation at
+// a time: the column's half spectrum in[o][x][kz][y] (128 x 256 values,
+// written by rf_mul_x<..., true> with plane 0 holding kz 0 + i kz 128, and
+// already scaled by 1/(nx ny nz)) is read once, transformed along y in
+// registers, and transformed along z in two chunks of up to 128 legal rows
+// through LDS, so the inverse y output never goes back to memory.
+//  * y pass: 64 groups of 16 threads, two pencils per group (planes g and
+//    g + 64).  Plane 0 is Hermitian in y in both parts, so its transform
+//    gives the kz = 0 values as real parts and the kz = 128 values as
 //    imaginary parts.  Thread t keeps Y[kz][y = t + 16 s] for s < 16.
 //  * z pass: the rows of a chunk are staged as stage[kz][row] (line 0 keeps
 //    the packed kz0 + i kz128 values); group g takes rows (2g, 2g + 1) of the
@@ -422,12 +489,20 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
 //    outputs are the two rows' values at z = t + 16 s.
 // The whole column is in registers between the two passes (64 VGPRs of the
 // 128 a thread has at 1024 threads), so the transforms use reg_fft_lean
-// (exchange twiddles from a [q][t] table, reads four at a time).  The checks
-// are those of rf_c2r_z_argmin over the legal values, kept as running
-// extremes: the largest residual (NaN-propagating, so a NaN raises the
-// residual flag as a per-value test does), the smallest and largest rounded
-// count, and a bit per zero count; the lowest feasible key is taken from the
-// zero bits after the transforms.
+// (exchange twiddles from a [q][t] table, reads four at a time).
+// The grid is one resident block per CU; block b takes the columns
+// c = b, b + G, ... of the (x, active slot) list, c = x * count + slot, and
+// skips columns outside its orientation's legal x range.  The first pencil
+// of the block's next column is loaded while the last chunk of the current
+// one is transformed (its registers are free once that chunk is staged), so
+// a column's first loads overlap the previous column's work.
+// The checks are those of rf_c2r_z_argmin kept as running extremes: the
+// largest residual (NaN-propagating, so a NaN raises the residual flag as a
+// per-value test does), the smallest and largest rounded count, and a bit
+// per zero count; the lowest feasible key of a column is taken from the
+// zero bits after its transforms.  Keys, residuals and flags are reduced
+// once per block, the per-orientation minima (when asked for) once per
+// column.
 constexpr int kYzThreads = 1024;
 constexpr int kYzPencil = RegShape<256, 16>::stride;  // LDS values per exchange pencil
 constexpr int kYzStageRow = 132;                       // LDS values per staged kz line (128 rows + pad)
@@ -436,7 +511,23 @@ template <typename T> constexpr std::size_t yz_lds_bytes()
 {
     constexpr std::size_t fft = std::size_t(64) * kYzPencil;
     constexpr std::size_t stage = std::size_t(128) * kYzStageRow;
-    return sizeof(cx<T>) * (256 + (fft > stage ? fft : stage));
+    // twiddle table, exchange / staging region, one parked word per thread
+    return sizeof(cx<T>) * (256 + (fft > stage ? fft : stage)) + sizeof(unsigned int) * kYzThreads;
+}
+
+// Smallest / largest over a wave (every lane gets the result).
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ float wave_max_f32(float v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
 }
 
 template <typename T>
@@ -444,171 +535,249 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
                                                             const OrientDesc* __restrict__ descs,
                                                             StepDeviceResult* __restrict__ result,
                                                             const unsigned long long* __restrict__ excl, StepFilter filt,
-                                                            const cx<T>* __restrict__ tw)
+                                                            const cx<T>* __restrict__ tw, int slots, int lmax)
 {
     using T2 = cx<T>;
-    constexpr int N = 256, R = 16, P = 16, NH = N / 2 + 1, G = kYzThreads / P;
+    constexpr int N = 256, R = 16, P = 16, NP = N / 2, G = kYzThreads / P;
     using S = RegShape<N, R>;
-    static_assert(S::P == P && G == 64, "256-point pencils of 16 threads");
+    static_assert(S::P == P && G == 64 && 2 * G == NP, "256-point pencils of 16 threads, two planes per group");
     static_assert(std::is_same_v<T, float>, "the fused pass runs in FP32 (staged pairs are read as float4)");
+    static_assert(64 % P == 0, "a pencil's threads are lanes of one wave");
+    constexpr int kRegion = 64 * kYzPencil > 128 * kYzStageRow ? 64 * kYzPencil : 128 * kYzStageRow;
     extern __shared__ unsigned char smem_raw[];
     T2* tws = reinterpret_cast<T2*>(smem_raw);
     T2* lds = tws + N;
+    unsigned int* parked = reinterpret_cast<unsigned int*>(lds + kRegion);
     __shared__ unsigned long long best;
     __shared__ unsigned int res_bits, flags;
-    const int o = slot_orientation(filt.act);
-    if (o < 0) return;
-    const OrientDesc d = descs[o];
-    const int x = blockIdx.x;
-    if (d.lx <= 0 || d.ly <= 0 || d.lz <= 0 || x >= d.lx) return;
+    const int count = filt.act ? min(filt.act[0], slots) : slots;
+    if (count <= 0 || lmax <= 0) return;  // uniform
+    const int total = lmax * count;
+    // the first legal column at or after c in this block's sequence, or total
+    auto next_col = [&](int c, int& o, int& x) {
+        for (; c < total; c += int(gridDim.x)) {
+            const int sl = c % count, xx = c / count;
+            const int oo = filt.act ? filt.act[kActHeader + sl] : sl;
+            const OrientDesc& dd = descs[oo];
+            if (xx < dd.lx && dd.ly > 0 && dd.lz > 0) {
+                o = oo;
+                x = xx;
+                return c;
+            }
+        }
+        return total;
+    };
+    int o = 0, x = 0;
+    int c = next_col(int(blockIdx.x), o, x);
+    if (c >= total) return;  // uniform
     if (threadIdx.x == 0) {
         best = ~0ULL;
         res_bits = 0;
         flags = 0;
     }
-    const int t = threadIdx.x % P, g = threadIdx.x / P;
     if (threadIdx.x < N) {
         // inverse exchange twiddles as [q][t]: conj(w^(t q))
         const int q = threadIdx.x / R, tt = threadIdx.x % R;
         const T2 w = tw[(tt * q) & (N - 1)];
         tws[threadIdx.x] = T2 { w.x, -w.y };
     }
-    const T2* col = in + o * in_o + (long long)x * NH * N;
-    // y pass: both pencils' loads first
+    // lane indices, formed afresh where they are used (values derived from
+    // them, such as LDS and load addresses, are then not hoisted out of the
+    // column loop and held across it)
+    auto lane_t = [] {
+        int v = int(threadIdx.x) % P;
+        asm volatile("" : "+v"(v));
+        __builtin_assume(v >= 0 && v < P);
+        return v;
+    };
+    auto lane_g = [] {
+        int v = int(threadIdx.x) / P;
+        asm volatile("" : "+v"(v));
+        __builtin_assume(v >= 0 && v < G);
+        return v;
+    };
+    constexpr long long kColumn = (long long)NP * N;  // values per x column
     T2 a[R], b[R];
-    const T2* p0 = col + (long long)g * N + t;
-    const T2* p1 = col + (long long)(g + G) * N + t;
+    {
+        const T2* p0 = in + o * in_o + x * kColumn + (long long)lane_g() * N + lane_t();
 #pragma unroll
-    for (int q = 0; q < R; ++q) a[q] = nt_load(&p0[P * q]);
-    if (g == 0) {
-        const T2* pn = col + (long long)(N / 2) * N + t;
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const T2 c = nt_load(&pn[P * q]);
-            a[q] = T2 { a[q].x - c.y, a[q].y + c.x };
-        }
+        for (int q = 0; q < R; ++q) a[q] = nt_load(&p0[P * q]);
     }
     __syncthreads();
-    T2* pen = lds + g * kYzPencil;
-    static_assert(64 % P == 0, "a pencil's threads are lanes of one wave");
-    // phase A's exchanges touch only each group's own region.  The second
-    // pencil's loads are issued inside the first transform, after its
-    // exchange writes (the first pencil's registers are free there): the
-    // loads then overlap the rest of that transform without holding both
-    // pencils and the transform's working set at once.
-    reg_fft_lean<T, N, R, false>(a, pen, tws, t, false, [&] {
-#pragma unroll
-        for (int q = 0; q < R; ++q) b[q] = nt_load(&p1[P * q]);
-    });
-    reg_fft_lean<T, N, R, false>(b, pen, tws, t, false);
-    const T max_out = T(d.mass < (long long)result->env_mass ? d.mass : (long long)result->env_mass);
     const bool any_excluded = excl[0] != 0;
     const bool regions_on = filt.act && filt.act[1];
-    T r1 = T(0), lo = T(0), hi = T(0);
+    // per lane: largest residual and smallest rounded count over every
+    // column; per wave (uniform): the lowest feasible key and the range flag
+    T r1 = T(0), lo = T(0);
+    unsigned long long wave_key = ~0ULL;
+    unsigned int wave_range = 0;
     T2* stage = lds;
-    unsigned int zero_bits[2] = { 0u, 0u };
-    // chunks of rows [128 c, 128 c + 128); the loop is unrolled so that the
-    // register arrays stay statically indexed
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const int row0 = 128 * c;
-        if (row0 >= d.ly) break;  // uniform over the block
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int r = t + P * s;  // row within the chunk: y = row0 + r
-            stage[g * kYzStageRow + r] = a[8 * c + s];
-            stage[(g + G) * kYzStageRow + r] = b[8 * c + s];
-        }
-        __syncthreads();
-        const int y1 = row0 + 2 * g;
-        T2 v[R];
-        // line 0 holds (kz0 + i kz128) of both rows: k = 0 takes the real
-        // parts and k = 128 the imaginary parts (thread 0, q = 0 and 8)
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-#pragma unroll
-            for (int q = 4 * h; q < 4 * h + 4; ++q) {
-                const int k = t + P * q;
-                const bool mirror = k > N / 2;
-                const int kk = (k == N / 2) ? 0 : mirror ? N - k : k;
-                const float4 ab = *reinterpret_cast<const float4*>(&stage[kk * kYzStageRow + 2 * g]);
-                const T2 u { ab.x, ab.y }, w { ab.z, ab.w };
-                T2 e = mirror ? T2 { u.x + w.y, w.x - u.y } : T2 { u.x - w.y, u.y + w.x };
-                if (q == 0 && t == 0) e = T2 { u.x, w.x };
-                if (q == 8 && t == 0) e = T2 { u.y, w.y };
-                v[q] = e;
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        reg_fft_lean<T, N, R>(v, pen, tws, t, false);
-        // checks over every value of the column's circular correlation, kept
-        // as running extremes (the residual NaN-propagating, so a NaN raises
-        // the residual flag as a per-value test would).  Every circular
-        // value, legal or not, is in exact arithmetic an integer in
-        // [0, min(masses)] and lies under the same a-priori bound, so the
-        // checks need no legality test per value (the legal values alone are
-        // the reference's; checking all of them is stricter).  Zero counts
-        // are marked in bit 2 s + half, which orders them as their keys (z,
-        // then y); the bits of values outside the legal range are cleared
-        // afterwards with the lane's legal-bit mask.
-        unsigned int zeros = 0;
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-            const T va = v[s].x, vb = v[s].y;
-            const T ra = rint(va), rb = rint(vb);
-            r1 = __builtin_elementwise_maximum(r1, __builtin_elementwise_maximum(fabsf(va - ra), fabsf(vb - rb)));
-            lo = fminf(lo, fminf(ra, rb));
-            hi = fmaxf(hi, fmaxf(ra, rb));
-            zeros |= (ra == T(0) ? 1u << (2 * s) : 0u) | (rb == T(0) ? 2u << (2 * s) : 0u);
-        }
+    while (c < total) {
+        const OrientDesc d = descs[o];
         {
+            const int t = lane_t(), g = lane_g();
+            T2* pen = lds + g * kYzPencil;
+            const T2* p1 = in + o * in_o + x * kColumn + (long long)(g + G) * N + t;
+            // y pass.  Its exchanges touch only each group's own region.  The
+            // second pencil's loads are issued inside the first transform,
+            // after its exchange writes, when the first pencil's registers are
+            // free.
+            reg_fft_lean<T, N, R, false>(a, pen, tws, t, false, [&] {
+#pragma unroll
+                for (int q = 0; q < R; ++q) b[q] = nt_load(&p1[P * q]);
+            });
+            // (the two transforms are not interleaved by the scheduler: their
+            // working sets together do not fit the registers)
+            __builtin_amdgcn_sched_barrier(0);
+            reg_fft_lean<T, N, R, false>(b, pen, tws, t, false);
+        }
+        int o2 = o, x2 = x;
+        const int c2 = next_col(c + int(gridDim.x), o2, x2);
+        const T max_out = T(d.mass < (long long)result->env_mass ? d.mass : (long long)result->env_mass);
+        T col_hi = T(0);
+        unsigned int zero_last = 0;
+        // one chunk of rows [128 C, 128 C + 128); PF: the next column's first
+        // pencil is loaded once this chunk is staged (the last chunk); PARK:
+        // this chunk's zero bits are parked in LDS until the column ends
+        auto chunk = [&](auto cc, auto pf, auto park) {
+            constexpr int C = decltype(cc)::value;
+            constexpr bool PF = decltype(pf)::value, PARK = decltype(park)::value;
+            const int row0 = 128 * C;
+            const int t = lane_t(), g = lane_g();
+            T2* pen = lds + g * kYzPencil;
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int r = t + P * s;  // row within the chunk: y = row0 + r
+                stage[g * kYzStageRow + r] = a[8 * C + s];
+                stage[(g + G) * kYzStageRow + r] = b[8 * C + s];
+            }
+            if (PF) {
+                // (zeros after the block's last column, so that the values
+                // just staged are dead on every path)
+                const bool more = c2 < total;
+                const T2* pn = in + o2 * in_o + x2 * kColumn + (long long)g * N + t;
+#pragma unroll
+                for (int q = 0; q < R; ++q) a[q] = more ? nt_load(&pn[P * q]) : T2 { 0, 0 };
+            }
+            __syncthreads();
+            if (row0 >= d.ly) {  // uniform: a column of at most 128 legal rows
+                zero_last = 0;
+                return;
+            }
+            const int y1 = row0 + 2 * g;
+            T2 v[R];
+            // line 0 holds (kz0 + i kz128) of both rows: k = 0 takes the real
+            // parts and k = 128 the imaginary parts (thread 0, q = 0 and 8)
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+#pragma unroll
+                for (int q = 4 * h; q < 4 * h + 4; ++q) {
+                    const int k = t + P * q;
+                    const bool mirror = k > N / 2;
+                    const int kk = (k == N / 2) ? 0 : mirror ? N - k : k;
+                    const float4 ab = *reinterpret_cast<const float4*>(&stage[kk * kYzStageRow + 2 * g]);
+                    const T2 u { ab.x, ab.y }, w { ab.z, ab.w };
+                    T2 e = mirror ? T2 { u.x + w.y, w.x - u.y } : T2 { u.x - w.y, u.y + w.x };
+                    if (q == 0 && t == 0) e = T2 { u.x, w.x };
+                    if (q == 8 && t == 0) e = T2 { u.y, w.y };
+                    v[q] = e;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            reg_fft_lean<T, N, R>(v, pen, tws, t, false);
+            // checks over every value of the column's circular correlation,
+            // kept as running extremes.  Every circular value, legal or not,
+            // is in exact arithmetic an integer in [0, min(masses)] and lies
+            // under the same a-priori bound, so the checks need no legality
+            // test per value (the legal values alone are the reference's;
+            // checking all of them is stricter).  Zero counts are marked in
+            // bit 2 s + half, which orders them as their keys (z, then y);
+            // the bits of values outside the legal range are cleared with the
+            // lane's legal-bit mask.
+            unsigned int zeros = 0;
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                const T va = v[s].x, vb = v[s].y;
+                const T ra = rint(va), rb = rint(vb);
+                r1 = __builtin_elementwise_maximum(r1, __builtin_elementwise_maximum(fabsf(va - ra), fabsf(vb - rb)));
+                lo = fminf(lo, fminf(ra, rb));
+                col_hi = fmaxf(col_hi, fmaxf(ra, rb));
+                zeros |= (ra == T(0) ? 1u << (2 * s) : 0u) | (rb == T(0) ? 2u << (2 * s) : 0u);
+            }
             // legal bits: 2 s + half with t + 16 s < lz and y1 + half < ly
             const int ns = min(max((d.lz - t + P - 1) / P, 0), R);
             const unsigned int zmask = ns >= R ? ~0u : (1u << (2 * ns)) - 1u;
             const unsigned int ymask = (y1 < d.ly ? 0x55555555u : 0u) | (y1 + 1 < d.ly ? 0xAAAAAAAAu : 0u);
-            zero_bits[c] = zeros & zmask & ymask;
-        }
-    }
-    unsigned int my_flags = (!(r1 < T(0.25)) ? 1u : 0u) | (lo < T(0) || hi > max_out ? 2u : 0u);
-    float my_res = float(r1);
-    unsigned long long mine = ~0ULL;
-    // this thread's lowest feasible key per chunk: the first marked value
-    // that is neither rejected nor outside the allowed regions (scanned after
-    // the transforms, when no spectrum is held any more)
+            zeros &= zmask & ymask;
+            if (PARK) parked[threadIdx.x] = zeros;
+            else zero_last = zeros;
+        };
+        using C0 = std::integral_constant<int, 0>;
+        using C1 = std::integral_constant<int, 1>;
+        chunk(C0 {}, std::false_type {}, std::true_type {});
+        chunk(C1 {}, std::true_type {}, std::false_type {});
+        // the range check's upper end is per orientation
+        if (wave_max_f32(col_hi) > max_out) wave_range = 2u;
+        // this thread's lowest feasible key in the column: per chunk the
+        // first marked value (bit order is key order within a chunk) that is
+        // neither rejected nor outside the allowed regions, and the lower of
+        // the two chunks' keys (z comes first in a key, so either chunk can
+        // hold the lower one)
+        unsigned long long col_key = ~0ULL;
+        {
+            const int t = lane_t(), g = lane_g();
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        unsigned int zeros = zero_bits[c];
-        const int y1 = 128 * c + 2 * g;
-        while (zeros) {
-            const int i = __builtin_ctz(zeros);
-            const int z = t + P * (i >> 1), y = y1 + (i & 1);
-            const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
-                                           ((unsigned long long)x << 16) | (unsigned long long)o;
-            if ((!any_excluded || !excluded(excl, key)) && (!regions_on || region_allowed(filt, x, y, z))) {
-                mine = key < mine ? key : mine;
-                break;
+            for (int C = 0; C < 2; ++C) {
+                unsigned int zeros = C == 0 ? parked[threadIdx.x] : zero_last;
+                const int y1 = 128 * C + 2 * g;
+                while (zeros) {
+                    const int i = __builtin_ctz(zeros);
+                    const int z = t + P * (i >> 1), y = y1 + (i & 1);
+                    const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
+                                                   ((unsigned long long)x << 16) | (unsigned long long)o;
+                    bool ok = !any_excluded || !excluded(excl, key);
+                    if (ok && regions_on) {
+                        // (a copy of the filter per use: the region divisions
+                        // are then not prepared outside the column loop)
+                        StepFilter f = filt;
+                        asm volatile("" : "+s"(f.ex), "+s"(f.ey), "+s"(f.ez));
+                        ok = region_allowed(f, x, y, z);
+                    }
+                    if (ok) {
+                        col_key = key < col_key ? key : col_key;
+                        break;
+                    }
+                    zeros &= zeros - 1;
+                }
             }
-            zeros &= zeros - 1;
         }
+        if (__builtin_amdgcn_ballot_w64(col_key != ~0ULL)) {
+            // the wave's minimum, held as a scalar
+            col_key = wave_min_u64(col_key);
+            col_key = ((unsigned long long)__builtin_amdgcn_readfirstlane(unsigned(col_key >> 32)) << 32) |
+                      (unsigned long long)__builtin_amdgcn_readfirstlane(unsigned(col_key));
+            wave_key = col_key < wave_key ? col_key : wave_key;
+            if (filt.orient_best && (threadIdx.x & 63) == 0) atomicMin(&filt.orient_best[o], col_key);
+        }
+        c = c2;
+        o = o2;
+        x = x2;
     }
+    unsigned int my_flags = (!(r1 < T(0.25)) ? 1u : 0u) | (lo < T(0) ? 2u : 0u) | wave_range;
+    float my_res = float(r1);
     for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long om = __shfl_down(mine, off, 64);
-        mine = om < mine ? om : mine;
         my_res = __builtin_elementwise_maximum(my_res, __shfl_down(my_res, off, 64));
         my_flags |= __shfl_down(my_flags, off, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-        if (mine != ~0ULL) atomicMin(&best, mine);
+        if (wave_key != ~0ULL) atomicMin(&best, wave_key);
         if (!(my_res <= 0.f)) atomicMax(&res_bits, __float_as_uint(my_res));
         if (my_flags) atomicOr(&flags, my_flags);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (best != ~0ULL) {
-            atomicMin(&result->key, best);
-            if (filt.orient_best) atomicMin(&filt.orient_best[o], best);
-        }
+        if (best != ~0ULL) atomicMin(&result->key, best);
         if (res_bits) atomicMax(&result->max_residual_bits, res_bits);
         if (flags) atomicOr(&result->flags, flags);
     }

@@ -480,7 +480,13 @@ int rf_mul_xt_launch(Tile tile, hipStream_t st, int cus, int slots, int ny, int 
     auto go = [&](auto ww) {
         constexpr int W = decltype(ww)::value;
         const std::size_t lds = rf_lds<T, N, R>(W);
-        *tiles = (long long)((ny + W - 1) / W) * nh;
+        // the output packs kz = 0 and kz = n/2 into plane 0 (its own launch, one
+        // block per row tile and orientation); the main pass has the planes
+        // kz = 1 .. n/2 - 1
+        const int ytiles = (ny + W - 1) / W;
+        hipLaunchKernelGGL((rf_mul_x_plane0<T, N, R, W>), dim3(ytiles, slots), dim3(W * (N / R)), lds, st, H, h_o, h_px,
+                           SB, ny, nh, out, out_o, descs, act, tw, sb_scale);
+        *tiles = (long long)ytiles * (nh - 2);
         const int grid = mul_grid(reinterpret_cast<const void*>(&rf_mul_x<T, N, R, W, true>), W * (N / R), lds, cus,
                                   *tiles, slots);
         used = grid;
@@ -1362,9 +1368,11 @@ public:
             const long long to = (long long)(nz / 2 + 1) * ny * nx;
             const StepFilter filt { act, region_.as<unsigned char>(), ext_[0], ext_[1], ext_[2],
                                     orient_best_.as<unsigned long long>() };
-            hipLaunchKernelGGL((rf_yz_argmin<T>), dim3(lmax_[0], 1, slots), dim3(kYzThreads), yz_lds_bytes<T>(),
-                               stream_, t_.as<cx<T>>(), to, d_descs_.as<OrientDesc>(), result_.as<StepDeviceResult>(),
-                               excl_.as<unsigned long long>(), filt, twiddles<T>(nz));
+            // one resident block per CU walks its share of the legal columns
+            const int grid = std::max(1, std::min(cus_, lmax_[0] * slots));
+            hipLaunchKernelGGL((rf_yz_argmin<T>), dim3(grid), dim3(kYzThreads), yz_lds_bytes<T>(), stream_,
+                               t_.as<cx<T>>(), to, d_descs_.as<OrientDesc>(), result_.as<StepDeviceResult>(),
+                               excl_.as<unsigned long long>(), filt, twiddles<T>(nz), slots, lmax_[0]);
         }
         else
             throw std::logic_error("the fused inverse y and z pass runs in FP32");

@@ -32,8 +32,11 @@ def test_kernel_bytes_small_window():
     # product pass: each fitting orientation's kernel rows in and legal columns out, the environment spectrum once
     want_mul = nh * 12 * 16 * e + nh * 12 * (3 + 12) * e + nh * 12 * (5 + 10) * e
     assert bench.kernel_bytes("mul_x", n, ks, ext, e) == want_mul
-    # fused inverse y + z: every kz line of every legal column
-    assert bench.kernel_bytes("yz_argmin", n, ks, ext, e) == nh * 12 * 12 * e + nh * 12 * 10 * e
+    # the fused path's product pass: kz = 0 and kz = 5 share one of 5 output planes
+    want_mul_fused = nh * 12 * 16 * e + (nh * 3 + 5 * 12) * 12 * e + (nh * 5 + 5 * 10) * 12 * e
+    assert bench.kernel_bytes("mul_x", n, ks, ext, e, fused=True) == want_mul_fused
+    # fused inverse y + z: every plane of every legal column (5 planes)
+    assert bench.kernel_bytes("yz_argmin", n, ks, ext, e) == 5 * 12 * 12 * e + 5 * 12 * 10 * e
     # separate inverse y: ny rows in and Ly rows out per legal column; inverse z: the Ly rows once
     assert bench.kernel_bytes("inv_c2c_y", n, ks, ext, e) == nh * (12 + 11) * 12 * e + nh * (12 + 1) * 10 * e
     assert bench.kernel_bytes("c2r_z_argmin", n, ks, ext, e) == nh * 11 * 12 * e + nh * 1 * 10 * e

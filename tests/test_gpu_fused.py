@@ -197,8 +197,9 @@ def test_product_pass_shares_break_inside_row_tiles(ctx):
     """The column-major product pass (fused path) shares row tiles x active
     orientations out evenly over a grid of resident blocks, so a block's share
     can begin and end inside a row tile; the test pins the tile width 16,
-    whose 2064 row tiles at 256^3 do not fill whole rounds of resident blocks,
-    and checks that the shared grid ran.  With 1, 2, 5 and 7 active
+    whose 2032 row tiles at 256^3 (16 per plane, planes kz = 1 .. 127; the
+    packed plane of kz = 0 and kz = 128 has its own launch) do not fill whole
+    rounds of resident blocks, and checks that the shared grid ran.  With 1, 2, 5 and 7 active
     orientations (shares breaking at different orientations) every active
     orientation keeps the lowest feasible corner it has when all orientations
     run, and the others report none.  The separate passes (row-major product
@@ -214,7 +215,7 @@ def test_product_pass_shares_break_inside_row_tiles(ctx):
         s.step(stamp=False)
         if fused:
             blocks, tiles = s.product_grid()
-            assert tiles == 2064 and 0 < blocks < tiles, (blocks, tiles)
+            assert tiles == 2032 and 0 < blocks < tiles, (blocks, tiles)
         full = s.orientation_minima()
         assert sum(m is not None for m in full) >= 6
         for active in ([4], [0, 8], [1, 2, 3, 6, 7], [8, 7, 5, 4, 2, 1, 0]):
