@@ -101,30 +101,17 @@ __global__ void __launch_bounds__(W*(N / R)) rf_rows(cx<T>* data, int rows, long
 // registers, and for each of its orientations finishes the kernel's x
 // transform from its partially transformed row H[kz][y][x < kx], multiplies
 // by the conjugate, runs the inverse x transform and stores x < lx into
-// out[o][kz][y][x].
-// XT: the output is stored column-major instead, out[o][x][kz][y] (x stride
-// nh * ny), for the fused inverse y and z pass; lanes then take the pencil
-// fastest so that every store covers runs of W consecutive y.  The
-// environment rows are multiplied by sb_scale whenever they are loaded (the
-// fused pass takes its 1/(nx ny nz) here, an exact power of two on its grids).
-// XT also packs the two planes whose rows are Hermitian in y, kz = 0 and
-// kz = n/2 (both real in z for a real correlation): plane 0 of the output
-// holds row(kz 0) + i row(kz n/2) and is written by rf_mul_x_plane0, so the
-// output has nh - 1 = n/2 planes and every x column of it is n/2 complex
-// pencils; here XT covers the planes kz = 1 .. n/2 - 1.
+// out[o][kz][y][x].  (The fused path's column-major form is rf_mul_xt.)
 // The work is (row tile, orientation) units, tile-major: nh * ceil(ny / W)
-// tiles of W rows (nh - 2 with XT) times the active orientations.  The grid is 1-D and block
+// tiles of W rows times the active orientations.  The grid is 1-D and block
 // b takes units [b U / G, (b + 1) U / G), loading the environment rows again
 // whenever its range enters a new tile; a grid of one block per tile gives
-// each block one tile with all orientations.  A grid of as many blocks as the
-// device holds at once ends together, where one block per tile can leave a
-// last partial round (nh = n/2 + 1 kz rows do not divide into whole rounds of
-// resident blocks).
-template <typename T, int N, int R, int W, bool XT = false>
+// each block one tile with all orientations.
+template <typename T, int N, int R, int W>
 __global__ void __launch_bounds__(W*(N / R))
 rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __restrict__ SB, int ny, int nh,
          cx<T>* __restrict__ out, long long out_o, const OrientDesc* __restrict__ descs, const int* __restrict__ act,
-         const cx<T>* __restrict__ tw, T sb_scale)
+         const cx<T>* __restrict__ tw)
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
@@ -132,13 +119,11 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
     extern __shared__ unsigned char smem_raw[];
     T2* tws = reinterpret_cast<T2*>(smem_raw);
     T2* lds = tws + N;
-    const int t = XT ? int(threadIdx.x) / W : int(threadIdx.x) % P;
-    const int w = XT ? int(threadIdx.x) % W : int(threadIdx.x) / P;
+    const int t = int(threadIdx.x) % P;
+    const int w = int(threadIdx.x) / P;
     const int count = act[0];
     const int ytiles = (ny + W - 1) / W;
-    const int planes = XT ? nh - 1 : nh;       // output planes
-    const int kz0 = XT ? 1 : 0;                // first row plane of the units
-    const long long units = (long long)ytiles * (nh - 2 * kz0) * count;
+    const long long units = (long long)ytiles * nh * count;
     const long long u0 = units * blockIdx.x / gridDim.x, u1 = units * (blockIdx.x + 1) / gridDim.x;
     if (u0 >= u1) return;  // uniform across the block
     load_tw_lds<T>(tws, tw, N);
@@ -150,16 +135,12 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
         const long long base = (long long)tile * count;
         const int i0 = int(u - base), i1 = int(min((long long)count, u1 - base));
         u = base + i1;
-        const int kz = kz0 + tile / ytiles, y = (tile % ytiles) * W + w;
+        const int kz = tile / ytiles, y = (tile % ytiles) * W + w;
         const bool live = y < ny;
         const long long row = (long long)kz * ny + y;
         T2 sb[R];
 #pragma unroll
         for (int s = 0; s < R; ++s) sb[s] = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
-        if (XT) {
-#pragma unroll
-            for (int s = 0; s < R; ++s) sb[s] = T2 { sb[s].x * sb_scale, sb[s].y * sb_scale };
-        }
         for (int i = i0; i < i1; ++i) {
             const int o = act[kActHeader + i];
             const int kx = descs[o].kx, lx = descs[o].lx;
@@ -173,10 +154,10 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
             }
             // the kernel row has kx <= h_px nonzero values: at radix 16 with up to
             // three leading groups of 16, the first pass leaves the zeros out
-            // (uniform over the block)
-            // row-major form: a pencil is P consecutive lanes of one wave, so its
-            // exchanges are wave-local (the table was synchronised before the loop)
-            constexpr bool WV = !XT && 64 % P == 0;
+            // (uniform over the block).  A pencil is P consecutive lanes of one
+            // wave, so its exchanges are wave-local (the table was synchronised
+            // before the loop).
+            constexpr bool WV = 64 % P == 0;
             const int lead = (kx + P - 1) / P;
             if (R == 16 && lead == 1) reg_fft<T, N, R, 1, WV>(v, pl, tws, t, true);
             else if (R == 16 && lead == 2) reg_fft<T, N, R, 2, WV>(v, pl, tws, t, true);
@@ -185,7 +166,143 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
 #pragma unroll
             for (int s = 0; s < R; ++s) v[s] = cmulc(sb[s], v[s]);
             reg_fft<T, N, R, 0, WV>(v, pl, tws, t, false);
-            if (live && XT) {
+            if (live) {
+                T2* orow = out + o * out_o + row * N;
+#pragma unroll
+                for (int s = 0; s < R; ++s) {
+                    const int x = t + P * s;
+                    if (x < lx) nt_store(&orow[x], v[s]);
+                }
+            }
+        }
+    }
+}
+
+// The column-major product pass of the fused path (FP32, N = 256, radix 16).
+// Same work as rf_mul_x (kernel x transform, product with conj, inverse x)
+// with the output out[o][x][plane][y] (x stride (nh - 1) ny) that
+// rf_yz_argmin reads:
+//  * lanes take the pencil fastest (w = lane % W rows of a tile, t = lane / W
+//    the 16 threads of a pencil), so that a store covers W consecutive y;
+//  * the kernel spectra are kept y-fastest, H[o][kz][x < h_lines][y], so a
+//    tile's kernel rows are read as whole 128-byte lines of W consecutive y
+//    per x (no line is fetched for the part of a row past its kx values);
+//  * the FFT exchange is element-major, lds[W pad(i) + w]: the W pencils of
+//    a tile sit side by side, so the lanes of an exchange write or read touch
+//    consecutive LDS addresses;
+//  * the exchange twiddles are [q][t] tables, one per direction.
+// The planes whose rows are Hermitian in y, kz = 0 and kz = n/2, are packed
+// into plane 0 as row(0) + i row(n/2) by rf_mul_xt_plane0; this kernel covers
+// the planes kz = 1 .. n/2 - 1, as (row tile, orientation) units shared out
+// over the grid as rf_mul_x does.
+constexpr int kXtPad = 272;  // LDS values per pencil of the element-major exchange (pad(255) + 1 rounded up)
+
+template <typename T> constexpr std::size_t xt_lds_bytes(int w)
+{
+    return sizeof(cx<T>) * (std::size_t(2 * 256) + std::size_t(w) * kXtPad);
+}
+
+// One 256-point transform (radix 16 x 16) of W pencils in the element-major
+// exchange; v in = x[t + 16 q], out = X[t + 16 s].  LEAD > 0: x[t + 16 q] is
+// zero for q >= LEAD.  xl = the exchange region + this lane's pencil w.
+template <typename T, int W, int LEAD>
+__device__ __forceinline__ void xfft256(cx<T>* v, cx<T>* xl, const cx<T>* tq, int t, bool fwd)
+{
+    constexpr int R = 16;
+    __builtin_assume(t >= 0 && t < R);
+    if constexpr (LEAD > 0) dft16_lead<T, LEAD>(v, fwd);
+    else dft16<T>(v, fwd);
+    __syncthreads();  // the previous exchange's reads are done
+    // pad(16 t + s) = 17 t + s and pad(16 q + t) = 17 q + t for s, t < 16
+#pragma unroll
+    for (int s = 0; s < R; ++s) xl[W * ((R + 1) * t + s)] = v[s];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        v[q] = xl[W * ((R + 1) * q + t)];
+        if (q > 0) v[q] = cmul(v[q], tq[q * R + t]);
+    }
+    dft16<T>(v, fwd);
+}
+
+// The kernel's x transform of one tile row (kx <= 48: the first pass leaves
+// the zero groups out).  hcol = H[o][kz][0][y] (x stride ny).
+template <typename T, int W>
+__device__ __forceinline__ void xt_kernel_row(cx<T>* v, const cx<T>* __restrict__ hcol, int ny, int kx, bool live,
+                                              cx<T>* xl, const cx<T>* tqf, int t)
+{
+    using T2 = cx<T>;
+    constexpr int R = 16, P = 16;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int x = t + P * q;
+        v[q] = (live && x < kx) ? nt_load(&hcol[(long long)x * ny]) : T2 { 0, 0 };
+    }
+    const int lead = (kx + P - 1) / P;  // uniform over the block
+    if (lead == 1) xfft256<T, W, 1>(v, xl, tqf, t, true);
+    else if (lead == 2) xfft256<T, W, 2>(v, xl, tqf, t, true);
+    else if (lead == 3) xfft256<T, W, 3>(v, xl, tqf, t, true);
+    else xfft256<T, W, 0>(v, xl, tqf, t, true);
+}
+
+template <typename T>
+__device__ __forceinline__ void xt_tables(cx<T>* tqf, cx<T>* tqi, const cx<T>* __restrict__ tw)
+{
+    constexpr int N = 256, R = 16;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const int q = i / R, tt = i % R;
+        const cx<T> w = tw[(tt * q) & (N - 1)];
+        tqf[i] = w;
+        tqi[i] = cx<T> { w.x, -w.y };
+    }
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(W * 16)
+rf_mul_xt(const cx<T>* __restrict__ H, long long h_o, int h_lines, const cx<T>* __restrict__ SB, int ny, int nh,
+          cx<T>* __restrict__ out, long long out_o, const OrientDesc* __restrict__ descs, const int* __restrict__ act,
+          const cx<T>* __restrict__ tw, T sb_scale)
+{
+    using T2 = cx<T>;
+    constexpr int N = 256, R = 16, P = 16;
+    extern __shared__ unsigned char smem_raw[];
+    T2* tqf = reinterpret_cast<T2*>(smem_raw);
+    T2* tqi = tqf + N;
+    const int t = int(threadIdx.x) / W, w = int(threadIdx.x) % W;
+    T2* xl = tqi + N + w;
+    const int count = act[0];
+    const int ytiles = (ny + W - 1) / W;
+    const int planes = nh - 1;  // output planes (plane 0 packed)
+    const long long units = (long long)ytiles * (nh - 2) * count;
+    const long long u0 = units * blockIdx.x / gridDim.x, u1 = units * (blockIdx.x + 1) / gridDim.x;
+    if (u0 >= u1) return;  // uniform across the block
+    xt_tables<T>(tqf, tqi, tw);
+    __syncthreads();
+    for (long long u = u0; u < u1;) {
+        // the part of this block's range in row tile `tile`: orientations [i0, i1)
+        const int tile = int(u / count);
+        const long long base = (long long)tile * count;
+        const int i0 = int(u - base), i1 = int(min((long long)count, u1 - base));
+        u = base + i1;
+        const int kz = 1 + tile / ytiles, y = (tile % ytiles) * W + w;
+        const bool live = y < ny;
+        const long long row = (long long)kz * ny + y;
+        T2 sb[R];
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            const T2 e = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
+            sb[s] = T2 { e.x * sb_scale, e.y * sb_scale };
+        }
+        for (int i = i0; i < i1; ++i) {
+            const int o = act[kActHeader + i];
+            const int kx = descs[o].kx, lx = descs[o].lx;
+            if (lx <= 0) continue;  // uniform across the block
+            T2 v[R];
+            xt_kernel_row<T, W>(v, H + o * h_o + (long long)kz * h_lines * ny + y, ny, kx, live, xl, tqf, t);
+#pragma unroll
+            for (int s = 0; s < R; ++s) v[s] = cmulc(sb[s], v[s]);
+            xfft256<T, W, 0>(v, xl, tqi, t, false);
+            if (live) {
                 T2* ocol = out + o * out_o + row;
                 // a fresh copy of the column stride per orientation: the store
                 // addresses are then formed here instead of being kept in
@@ -198,68 +315,52 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
                     if (x < lx) nt_store(&ocol[x * xs], v[s]);
                 }
             }
-            else if (live) {
-                T2* orow = out + o * out_o + row * N;
-#pragma unroll
-                for (int s = 0; s < R; ++s) {
-                    const int x = t + P * s;
-                    if (x < lx) nt_store(&orow[x], v[s]);
-                }
-            }
         }
     }
 }
 
-
-// Plane 0 of the column-major product output (fused path, see rf_mul_x):
-// row(kz 0) + i row(kz n/2) per (row, orientation).  The inverse x transform
-// is linear, so the two rows' products are combined before it, P0 + i P_half,
-// and a packed row takes three transforms.  One block per (tile of W rows,
-// active orientation slot); lanes as rf_mul_x<..., true> (pencil fastest).
-template <typename T, int N, int R, int W>
-__global__ void __launch_bounds__(W*(N / R))
-rf_mul_x_plane0(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __restrict__ SB, int ny, int nh,
-                cx<T>* __restrict__ out, long long out_o, const OrientDesc* __restrict__ descs,
-                const int* __restrict__ act, const cx<T>* __restrict__ tw, T sb_scale)
+// Plane 0 of rf_mul_xt's output: row(kz 0) + i row(kz n/2) per (row,
+// orientation).  The inverse x transform is linear, so the two rows'
+// products are combined before it, P0 + i P_half, and a packed row takes
+// three transforms.  One block per (tile of W rows, active orientation slot).
+template <typename T, int W>
+__global__ void __launch_bounds__(W * 16)
+rf_mul_xt_plane0(const cx<T>* __restrict__ H, long long h_o, int h_lines, const cx<T>* __restrict__ SB, int ny, int nh,
+                 cx<T>* __restrict__ out, long long out_o, const OrientDesc* __restrict__ descs,
+                 const int* __restrict__ act, const cx<T>* __restrict__ tw, T sb_scale)
 {
     using T2 = cx<T>;
-    using S = RegShape<N, R>;
-    constexpr int P = S::P;
+    constexpr int N = 256, R = 16, P = 16;
     extern __shared__ unsigned char smem_raw[];
-    T2* tws = reinterpret_cast<T2*>(smem_raw);
-    T2* lds = tws + N;
+    T2* tqf = reinterpret_cast<T2*>(smem_raw);
+    T2* tqi = tqf + N;
     if (int(blockIdx.y) >= act[0]) return;  // uniform
     const int o = act[kActHeader + blockIdx.y];
     const int kx = descs[o].kx, lx = descs[o].lx;
     if (lx <= 0) return;  // uniform
     const int t = int(threadIdx.x) / W, w = int(threadIdx.x) % W;
+    T2* xl = tqi + N + w;
     const int y = int(blockIdx.x) * W + w;
     const bool live = y < ny;
-    load_tw_lds<T>(tws, tw, N);
+    xt_tables<T>(tqf, tqi, tw);
     __syncthreads();
-    T2* pl = lds + w * S::stride;
-    const long long rows[2] = { (long long)y, (long long)(nh - 1) * ny + y };
     T2 p[R];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const T2* hrow = H + o * h_o + rows[h] * h_px;
+        const int kz = h == 0 ? 0 : nh - 1;
+        const long long row = (long long)kz * ny + y;
         T2 v[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int x = t + P * q;
-            v[q] = (live && x < kx) ? nt_load(&hrow[x]) : T2 { 0, 0 };
-        }
-        reg_fft<T, N, R>(v, pl, tws, t, true);
+        xt_kernel_row<T, W>(v, H + o * h_o + (long long)kz * h_lines * ny + y, ny, kx, live, xl, tqf, t);
 #pragma unroll
         for (int s = 0; s < R; ++s) {
-            const T2 e = live ? SB[rows[h] * N + t + P * s] : T2 { 0, 0 };
+            const T2 e = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
             const T2 q = cmulc(T2 { e.x * sb_scale, e.y * sb_scale }, v[s]);
             p[s] = h == 0 ? q : T2 { p[s].x - q.y, p[s].y + q.x };
         }
         // the second row's loads are not issued while the first is live
         __builtin_amdgcn_sched_barrier(0);
     }
-    reg_fft<T, N, R>(p, pl, tws, t, false);
+    xfft256<T, W, 0>(p, xl, tqi, t, false);
     if (live) {
         T2* ocol = out + o * out_o + y;
         const long long xs = (long long)(nh - 1) * ny;

@@ -462,36 +462,37 @@ void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>
         constexpr int N = TT::n, R = TT::r, W = TT::w;
         const int tiles = (ny + W - 1) / W * nh;
         hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3(tiles), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st, H, h_o, h_px,
-                           SB, ny, nh, out, out_o, descs, act, tw, T(1));
+                           SB, ny, nh, out, out_o, descs, act, tw);
     });
     if (!ok) throw DeviceError { "no such FFT tile" };
 }
 
 // P1 with the column-major output of the fused inverse y and z pass (FP32,
 // N = 256, radix 16; the tile's width is kept when it is one of 8, 16, 32).
-// Returns the grid (blocks) and sets *tiles to the row tiles.
+// H: the kernel spectra y-fastest, H[o][kz][x < h_lines][y] (h_o values per
+// orientation).  Returns the grid (blocks) of the main launch and sets *tiles
+// to its row tiles.
 template <typename T>
 int rf_mul_xt_launch(Tile tile, hipStream_t st, int cus, int slots, int ny, int nh, const cx<T>* H, long long h_o,
-                     int h_px, const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
+                     int h_lines, const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
                      const cx<T>* tw, T sb_scale, long long* tiles)
 {
-    constexpr int N = 256, R = 16;
+    constexpr int R = 16;
     int used = 0;
     auto go = [&](auto ww) {
         constexpr int W = decltype(ww)::value;
-        const std::size_t lds = rf_lds<T, N, R>(W);
+        const std::size_t lds = xt_lds_bytes<T>(W);
         // the output packs kz = 0 and kz = n/2 into plane 0 (its own launch, one
         // block per row tile and orientation); the main pass has the planes
         // kz = 1 .. n/2 - 1
         const int ytiles = (ny + W - 1) / W;
-        hipLaunchKernelGGL((rf_mul_x_plane0<T, N, R, W>), dim3(ytiles, slots), dim3(W * (N / R)), lds, st, H, h_o, h_px,
-                           SB, ny, nh, out, out_o, descs, act, tw, sb_scale);
-        *tiles = (long long)ytiles * (nh - 2);
-        const int grid = mul_grid(reinterpret_cast<const void*>(&rf_mul_x<T, N, R, W, true>), W * (N / R), lds, cus,
-                                  *tiles, slots);
-        used = grid;
-        hipLaunchKernelGGL((rf_mul_x<T, N, R, W, true>), dim3(grid), dim3(W * (N / R)), lds, st, H, h_o, h_px, SB, ny,
+        hipLaunchKernelGGL((rf_mul_xt_plane0<T, W>), dim3(ytiles, slots), dim3(W * R), lds, st, H, h_o, h_lines, SB, ny,
                            nh, out, out_o, descs, act, tw, sb_scale);
+        *tiles = (long long)ytiles * (nh - 2);
+        const int grid = mul_grid(reinterpret_cast<const void*>(&rf_mul_xt<T, W>), W * R, lds, cus, *tiles, slots);
+        used = grid;
+        hipLaunchKernelGGL((rf_mul_xt<T, W>), dim3(grid), dim3(W * R), lds, st, H, h_o, h_lines, SB, ny, nh, out, out_o,
+                           descs, act, tw, sb_scale);
     };
     if (tile.r == R && tile.w == 8) go(std::integral_constant<int, 8> {});
     else if (tile.r == R && tile.w == 32) go(std::integral_constant<int, 32> {});
@@ -1282,11 +1283,14 @@ public:
                                0, 0, 0, 0, H1, h1o, kmax_[0], kmax_[1], 0, 0, dd, act, twz, plans_[2], W);
         }
         mark("kernel_r2c_z");
-        // G2: kernel C2C along y for x < kx, reading its ky nonzero rows
+        // G2: kernel C2C along y for x < kx, reading its ky nonzero rows.  The
+        // fused path's product pass reads the result y-fastest,
+        // H2[o][kz][x < h2p][y] (the same size per orientation).
+        const bool yfast = fused_yz();
         if (rf_supported(ny) && use_rf_) {
             rf_cols_launch<T>(ny, default_tile(kPassInvY, ny), stream_, kmax_[0], nh, slots, H1, h1o, (long long)kmax_[0], 1LL,
-                              (long long)kmax_[1] * kmax_[0], H2, h2o, (long long)h2p_, 1LL,
-                              (long long)ny * h2p_, 0, 0, ny, dd, 1, act, twy, 1);
+                              (long long)kmax_[1] * kmax_[0], H2, h2o, yfast ? 1LL : (long long)h2p_,
+                              yfast ? (long long)ny : 1LL, (long long)ny * h2p_, 0, 0, ny, dd, 1, act, twy, 1);
         }
         else {
             const int W = choose_w(ny, e, 2, false, 16, 1);
@@ -1296,6 +1300,7 @@ public:
                                (long long)ny * h2p_, 0, 0, ny, dd, 1, act, twy, plans_[1], W, 1);
         }
         mark("kernel_c2c_y");
+        spectra_yfast_ = yfast;
     }
 
     // One register-FFT pass of the step (P1, P2 or P3) at a given tile.
@@ -1718,7 +1723,9 @@ public:
     // after preparation (outside any graph) and reused by every step.
     void ensure_kernel_spectra()
     {
-        if (!resident_ || spectra_ready_ || nfit_ == 0) return;
+        // (computed again when the fused pass is switched on or off: its
+        // product pass reads them in the other layout)
+        if (!resident_ || (spectra_ready_ && spectra_yfast_ == fused_yz()) || nfit_ == 0) return;
         const bool prof = prof_;
         prof_ = false;
         if (precision_ == 64) enqueue_kernel_spectra<double>(act_full_.as<int>(), nfit_);
@@ -2384,6 +2391,7 @@ private:
     bool use_rf_ { true };
     bool resident_ { true };
     bool spectra_ready_ {};
+    bool spectra_yfast_ {};    // H2 is y-fastest (the fused path's layout)
     bool owns_stream_ {};
     bool incremental_ {}, inc_setup_done_ {}, inc_effective_ {}, inc_ready_ {}, pending_ {};
     bool circular_ {};
