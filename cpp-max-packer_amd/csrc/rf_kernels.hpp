@@ -605,7 +605,8 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
 // once per block, the per-orientation minima (when asked for) once per
 // column.
 constexpr int kYzThreads = 1024;
-constexpr int kYzPencil = RegShape<256, 16>::stride;  // LDS values per exchange pencil
+constexpr int kYzRow = 18;                             // LDS values per exchange row (16-byte stores)
+constexpr int kYzPencil = 16 * kYzRow;                 // LDS values per exchange pencil
 constexpr int kYzStageRow = 132;                       // LDS values per staged kz line (128 rows + pad)
 
 template <typename T> constexpr std::size_t yz_lds_bytes()
@@ -645,8 +646,8 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
     static_assert(std::is_same_v<T, float>, "the fused pass runs in FP32 (staged pairs are read as float4)");
     static_assert(64 % P == 0, "a pencil's threads are lanes of one wave");
     constexpr int kRegion = 64 * kYzPencil > 128 * kYzStageRow ? 64 * kYzPencil : 128 * kYzStageRow;
-    extern __shared__ unsigned char smem_raw[];
-    T2* tws = reinterpret_cast<T2*>(smem_raw);
+    extern __shared__ __attribute__((aligned(16))) unsigned char yz_smem[];  // 16-byte stores and loads
+    T2* tws = reinterpret_cast<T2*>(yz_smem);
     T2* lds = tws + N;
     unsigned int* parked = reinterpret_cast<unsigned int*>(lds + kRegion);
     __shared__ unsigned long long best;
@@ -723,14 +724,15 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
             // second pencil's loads are issued inside the first transform,
             // after its exchange writes, when the first pencil's registers are
             // free.
-            reg_fft_lean<T, N, R, false>(a, pen, tws, t, false, [&] {
+            auto load_b = [&] {
 #pragma unroll
                 for (int q = 0; q < R; ++q) b[q] = nt_load(&p1[P * q]);
-            });
+            };
+            reg_fft_lean<T, N, R, false, decltype(load_b), kYzRow>(a, pen, tws, t, false, load_b);
             // (the two transforms are not interleaved by the scheduler: their
             // working sets together do not fit the registers)
             __builtin_amdgcn_sched_barrier(0);
-            reg_fft_lean<T, N, R, false>(b, pen, tws, t, false);
+            reg_fft_lean<T, N, R, false, NoWork, kYzRow>(b, pen, tws, t, false);
         }
         int o2 = o, x2 = x;
         const int c2 = next_col(c + int(gridDim.x), o2, x2);
@@ -754,12 +756,12 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
                 stage[(g + G) * kYzStageRow + r] = b[8 * C + s];
             }
             if (PF) {
-                // (zeros after the block's last column, so that the values
-                // just staged are dead on every path)
+                // (after the block's last column the current one is read
+                // again, unused: the loads stay unconditional)
                 const bool more = c2 < total;
-                const T2* pn = in + o2 * in_o + x2 * kColumn + (long long)g * N + t;
+                const T2* pn = in + (more ? o2 : o) * in_o + (more ? x2 : x) * kColumn + (long long)g * N + t;
 #pragma unroll
-                for (int q = 0; q < R; ++q) a[q] = more ? nt_load(&pn[P * q]) : T2 { 0, 0 };
+                for (int q = 0; q < R; ++q) a[q] = nt_load(&pn[P * q]);
             }
             __syncthreads();
             if (row0 >= d.ly) {  // uniform: a column of at most 128 legal rows
@@ -786,24 +788,29 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
-            reg_fft_lean<T, N, R>(v, pen, tws, t, false);
+            reg_fft_lean<T, N, R, true, NoWork, kYzRow>(v, pen, tws, t, false);
             // checks over every value of the column's circular correlation,
             // kept as running extremes.  Every circular value, legal or not,
             // is in exact arithmetic an integer in [0, min(masses)] and lies
             // under the same a-priori bound, so the checks need no legality
             // test per value (the legal values alone are the reference's;
-            // checking all of them is stricter).  Zero counts are marked in
-            // bit 2 s + half, which orders them as their keys (z, then y);
-            // the bits of values outside the legal range are cleared with the
-            // lane's legal-bit mask.
-            unsigned int zeros = 0;
-#pragma unroll
-            for (int s = 0; s < R; ++s) {
+            // checking all of them is stricter).
+            auto check = [&](int s, T& ra, T& rb) {
                 const T va = v[s].x, vb = v[s].y;
-                const T ra = rint(va), rb = rint(vb);
+                ra = rint(va);
+                rb = rint(vb);
                 r1 = __builtin_elementwise_maximum(r1, __builtin_elementwise_maximum(fabsf(va - ra), fabsf(vb - rb)));
                 lo = fminf(lo, fminf(ra, rb));
                 col_hi = fmaxf(col_hi, fmaxf(ra, rb));
+            };
+            // Zero counts are marked in bit 2 s + half, which orders them as
+            // their keys (z, then y); the bits of values outside the legal
+            // range are cleared with the lane's legal-bit mask.
+            unsigned int zeros = 0;
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                T ra, rb;
+                check(s, ra, rb);
                 zeros |= (ra == T(0) ? 1u << (2 * s) : 0u) | (rb == T(0) ? 2u << (2 * s) : 0u);
             }
             // legal bits: 2 s + half with t + 16 s < lz and y1 + half < ly

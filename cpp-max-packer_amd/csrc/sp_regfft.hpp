@@ -181,11 +181,15 @@ struct NoWork {
 
 // `mid` runs between the exchange's writes and its reads, when the pencil's
 // registers hold nothing: a caller can issue loads for later work there.
-template <typename T, int N, int R, bool SYNC = true, typename Mid = NoWork>
+// ROW: LDS values per exchange row (17: one pad value; 18 in FP32: rows
+// start on 16-byte boundaries and each thread writes its row as eight
+// 16-byte stores, the region itself being 16-byte aligned).
+template <typename T, int N, int R, bool SYNC = true, typename Mid = NoWork, int ROW = R + 1>
 __device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd, Mid mid = {})
 {
     using S = RegShape<N, R>;
     static_assert(S::passes == 2 && R == 16, "one exchange of 16 x 16");
+    static_assert(ROW == R + 1 || (ROW == R + 2 && sizeof(T) == 4), "row of 17 values, or of 18 FP32 values");
     // a fresh copy of t per call: the twiddles are then read again here
     // rather than kept in registers from an earlier call
     asm volatile("" : "+v"(t));
@@ -193,17 +197,24 @@ __device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* 
     dft_reg<T, R>(v, fwd);
     if (SYNC) __syncthreads();
     else __builtin_amdgcn_wave_barrier();
-    // pad(16 t + s) = 17 t + s and pad(16 q + t) = 17 q + t for s, t < 16
+    // row t holds X1[16 t + s]; the second pass reads column t (l = R, m = 1,
+    // j = t, k = 0)
+    if constexpr (ROW == R + 2) {
 #pragma unroll
-    for (int s = 0; s < R; ++s) lds[(R + 1) * t + s] = v[s];
+        for (int s = 0; s < R; s += 2)
+            *reinterpret_cast<float4*>(&lds[ROW * t + s]) = float4 { v[s].x, v[s].y, v[s + 1].x, v[s + 1].y };
+    }
+    else {
+#pragma unroll
+        for (int s = 0; s < R; ++s) lds[ROW * t + s] = v[s];
+    }
     mid();
     __builtin_amdgcn_wave_barrier();
-    // second pass: l = R, m = 1, j = t, k = 0
 #pragma unroll
     for (int q0 = 0; q0 < R; q0 += 4) {
 #pragma unroll
         for (int q = q0; q < q0 + 4; ++q) {
-            v[q] = lds[(R + 1) * q + t];
+            v[q] = lds[ROW * q + t];
             if (q > 0) v[q] = cmul(v[q], tq[q * R + t]);
         }
         __builtin_amdgcn_sched_barrier(0);
