@@ -168,8 +168,13 @@ def main():
     ap.add_argument("--learned-seeds", default="9000,9001,9002",
                     help="held-out part seeds packed with and without proposals")
     ap.add_argument("--weights", default="", help="proposal weights (.npy) for the learned leg (default: the package's)")
-    ap.add_argument("--top-o", type=int, default=8)
-    ap.add_argument("--top-r", type=int, default=512)
+    ap.add_argument("--top-o", type=int, default=12,
+                    help="learned leg: orientations searched per lazy step (and per restricted step)")
+    ap.add_argument("--top-r", type=int, default=512, help="learned leg: regions of the restricted step")
+    ap.add_argument("--prior-weight", type=float, default=2.0,
+                    help="learned leg: weight of the orientation prior added to the network's logits")
+    ap.add_argument("--restricted-top-o", type=int, default=8,
+                    help="learned leg: orientations of the restricted (non-lazy) proposal shown beside it")
     ap.add_argument("--only-learned", action="store_true",
                     help="run only learned-proposal episodes (no plain step timing, no plain episodes); for profiling")
     ap.add_argument("--no-ranked", action="store_true", help="skip the ranked-page (proximity) timing")
@@ -475,9 +480,13 @@ def ranked_timing(sp, ctx, w, env, kernels, prec, args):
 
 def learned_comparison(ctx, args, plain=True):
     """Packs held-out star parts at 256^3 with 96 orientations to exhaustion,
-    once with the plain search and once with the learned proposal (top
-    orientations in the top regions, full search as fallback); wall-clock
-    steps/s of both loops (one step and its result read-back per iteration)."""
+    with the plain search and with the learned proposal in its lazy form
+    (each orientation's last exact minimum kept while its corner stays free;
+    of the orientations the new occupancy blocks, the top_o by network logit
+    plus the orientation prior are searched), and for reference with the
+    restricted proposal (the network's top orientations in its top regions,
+    full search as fallback); wall-clock steps/s of each loop (one step and
+    its result read-back per iteration)."""
     import spectrapack_amd as sp
     from spectrapack_amd import learned as L
     from spectrapack_amd import workloads as W
@@ -491,27 +500,40 @@ def learned_comparison(ctx, args, plain=True):
     rows = []
     for seed in [int(v) for v in args.learned_seeds.split(",")]:
         sc = L.Scene(ctx, W.star_so3_fine_box(seed))
-        learned = L.run_episode(sc, prop, args.top_o, args.top_r)
+        learned = L.run_episode_lazy(sc, prop, args.top_o, prior_weight=args.prior_weight)
         if not plain:
             log(f"[bench] learned proposal seed {seed}: {learned['parts']} parts {learned['steps_per_s']:.1f} steps/s")
             rows.append(dict(seed=seed, learned=learned))
             sc.close()
             continue
+        restricted = L.run_episode(sc, prop, args.restricted_top_o, args.top_r)
         pl = L.run_episode(sc)
         log(f"[bench] learned proposal seed {seed}: plain {pl['parts']} parts {pl['steps_per_s']:.1f} steps/s, "
-            f"learned {learned['parts']} parts {learned['steps_per_s']:.1f} steps/s ({learned['fallbacks']} fallbacks)")
-        rows.append(dict(seed=seed, plain=pl, learned=learned))
+            f"lazy {learned['parts']} parts {learned['steps_per_s']:.1f} steps/s "
+            f"({learned['searched_per_step']:.1f} orientations searched per step, {learned['full_searches']} full), "
+            f"restricted {restricted['parts']} parts {restricted['steps_per_s']:.1f} steps/s")
+        rows.append(dict(seed=seed, plain=pl, learned=learned, restricted=restricted))
         sc.close()
     if not plain:
-        return {"episodes": rows, "top_orientations": args.top_o, "top_regions": args.top_r}
-    pp = sum(r["plain"]["parts"] for r in rows)
-    lp = sum(r["learned"]["parts"] for r in rows)
-    ps = sum(r["plain"]["steps"] for r in rows) / sum(r["plain"]["seconds"] for r in rows)
-    ls = sum(r["learned"]["steps"] for r in rows) / sum(r["learned"]["seconds"] for r in rows)
-    return {"workload": W.star_so3_fine_box(rows[0]["seed"]).name, "top_orientations": args.top_o,
-            "top_regions": args.top_r, "plain_parts": pp, "learned_parts": lp, "parts_ratio": lp / max(pp, 1),
+        return {"episodes": rows, "top_orientations": args.top_o, "prior_weight": args.prior_weight}
+
+    def total(key):
+        return (sum(r[key]["parts"] for r in rows),
+                sum(r[key]["steps"] for r in rows) / sum(r[key]["seconds"] for r in rows))
+    pp, ps = total("plain")
+    lp, ls = total("learned")
+    rp, rsp = total("restricted")
+    return {"workload": W.star_so3_fine_box(rows[0]["seed"]).name, "method": "lazy",
+            "top_orientations": args.top_o, "prior_weight": args.prior_weight,
+            "plain_parts": pp, "learned_parts": lp, "parts_ratio": lp / max(pp, 1),
             "plain_steps_per_s": ps, "learned_steps_per_s": ls, "speedup": ls / ps,
-            "fallbacks": sum(r["learned"]["fallbacks"] for r in rows), "episodes": rows,
+            "searched_per_step": sum(r["learned"]["searched_per_step"] * r["learned"]["steps"] for r in rows) /
+            max(sum(r["learned"]["steps"] for r in rows), 1),
+            "full_searches": sum(r["learned"]["full_searches"] for r in rows),
+            "restricted": {"top_orientations": args.restricted_top_o, "top_regions": args.top_r, "parts": rp,
+                           "parts_ratio": rp / max(pp, 1), "steps_per_s": rsp, "speedup": rsp / ps,
+                           "fallbacks": sum(r["restricted"]["fallbacks"] for r in rows)},
+            "episodes": rows,
             "training": {k: meta.get(k) for k in ("episodes", "samples", "epochs", "scene", "hits_trained",
                                                   "hits_untrained")}}
 

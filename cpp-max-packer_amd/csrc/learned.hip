@@ -662,7 +662,8 @@ __device__ __forceinline__ unsigned descending_float_key(float f)
     return ~ascending;
 }
 
-__global__ __launch_bounds__(1024) void k_select(const float* __restrict__ logits, const float* __restrict__ heat,
+__global__ __launch_bounds__(1024) void k_select(const float* __restrict__ logits, const float* __restrict__ prior,
+                                                 const float* __restrict__ heat,
                                                  const OrientDesc* __restrict__ descs, int orientations,
                                                  const int* __restrict__ cfg, int* __restrict__ act,
                                                  u8* __restrict__ region)
@@ -670,22 +671,34 @@ __global__ __launch_bounds__(1024) void k_select(const float* __restrict__ logit
     __shared__ unsigned long long keys[kV2];
     __shared__ int nvalid;
     const int top_o = cfg[0], top_r = cfg[1];
-    for (int v = threadIdx.x; v < kV2; v += blockDim.x)
-        keys[v] = ((unsigned long long)descending_float_key(heat[v]) << 32) | unsigned(v);
+    const bool all_regions = top_r >= kV2;  // uniform: no region mask, no sort
+    if (!all_regions) {
+        for (int v = threadIdx.x; v < kV2; v += blockDim.x)
+            keys[v] = ((unsigned long long)descending_float_key(heat[v]) << 32) | unsigned(v);
+    }
     if (threadIdx.x == 0) nvalid = 0;
     __syncthreads();
     const int o = threadIdx.x;
     const bool valid = o < orientations && o < kOrient && descs[o].lx > 0;
     if (valid) atomicAdd(&nvalid, 1);
     if (valid) {
-        const float lo = logits[o];
+        // orientation score: the logit plus the scene's prior term
+        const float lo = logits[o] + prior[o];
         int rank = 0;
         for (int q = 0; q < orientations && q < kOrient; ++q) {
             if (q == o || descs[q].lx <= 0) continue;
-            const float lq = logits[q];
+            const float lq = logits[q] + prior[q];
             rank += (lq > lo || (lq == lo && q < o)) ? 1 : 0;
         }
         if (rank < top_o) act[kActHeader + rank] = o;
+    }
+    if (all_regions) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            act[0] = nvalid < top_o ? nvalid : top_o;
+            act[1] = 0;
+        }
+        return;
     }
     // bitonic sort, ascending
     for (int size = 2; size <= kV2; size <<= 1) {
@@ -712,6 +725,116 @@ __global__ __launch_bounds__(1024) void k_select(const float* __restrict__ logit
         act[3] = 16;
         act[4] = 16;
     }
+}
+
+// Lazy exact minima, the exact part of a lazy learned step.  For every
+// orientation the proposal keeps the last lowest feasible key found for it
+// (key) and whether that key is still its current minimum (valid).
+// Occupancy only grows, so an orientation's feasible set only shrinks: a
+// minimum that is still free is still the minimum, and one that the new
+// occupancy blocks is a lower bound of the new minimum (keys order as the
+// plain rule, (z, y, x, orientation)).
+
+// After a full search every orientation's minimum is current.
+__global__ void k_lazy_init(const unsigned long long* __restrict__ orient_best, int orientations,
+                            unsigned long long* __restrict__ key, unsigned* __restrict__ valid)
+{
+    const int o = threadIdx.x;
+    if (o >= kOrient) return;
+    key[o] = o < orientations ? orient_best[o] : ~0ULL;
+    valid[o] = 1;
+}
+
+// One block per orientation: a current key whose kernel now overlaps the
+// occupancy (or leaves the window) is no longer the minimum.
+__global__ void k_lazy_check(const u8* __restrict__ env, int ex, int ey, int ez, const u8* __restrict__ kcells,
+                             const OrientDesc* __restrict__ descs, const unsigned long long* __restrict__ key,
+                             unsigned* __restrict__ valid)
+{
+    const int o = blockIdx.x;
+    const unsigned long long k = key[o];
+    if (!valid[o] || k == ~0ULL) return;  // uniform
+    __shared__ unsigned hit;
+    if (threadIdx.x == 0) hit = 0;
+    __syncthreads();
+    const int ux = int((k >> 16) & 0xffffULL), uy = int((k >> 32) & 0xffffULL), uz = int(k >> 48);
+    const OrientDesc d = descs[o];
+    const long long n = (long long)d.kx * d.ky * d.kz;
+    unsigned mine = 0;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+        if (!kcells[d.cells_offset + i]) continue;
+        const int gx = ux + int(i % d.kx), gy = uy + int((i / d.kx) % d.ky), gz = uz + int(i / ((long long)d.kx * d.ky));
+        if (gx >= ex || gy >= ey || gz >= ez || env[gx + (long long)ex * (gy + (long long)ey * gz)]) mine = 1;
+    }
+    if (mine) atomicOr(&hit, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0 && hit) valid[o] = 0;
+}
+
+// The search set of a lazy learned step: the orientations whose key is not
+// current and whose lower bound lies below the lowest current key (only
+// they can beat it); the top_o of them by score (logit + prior; ties: lower
+// index) go into the active list.  No region mask.
+__global__ __launch_bounds__(128) void k_lazy_select(const float* __restrict__ logits, const float* __restrict__ prior,
+                                                     const OrientDesc* __restrict__ descs, int orientations,
+                                                     const int* __restrict__ cfg,
+                                                     const unsigned long long* __restrict__ key,
+                                                     const unsigned* __restrict__ valid, int* __restrict__ act)
+{
+    __shared__ unsigned long long best;
+    __shared__ int ncand;
+    const int top_o = cfg[0];
+    const int o = threadIdx.x;
+    const int no = orientations < kOrient ? orientations : kOrient;
+    if (o == 0) {
+        best = ~0ULL;
+        ncand = 0;
+    }
+    __syncthreads();
+    const bool fits = o < no && descs[o].lx > 0;
+    if (fits && valid[o]) atomicMin(&best, key[o]);
+    __syncthreads();
+    const unsigned long long b = best;
+    auto candidate = [&](int q) { return q < no && descs[q].lx > 0 && !valid[q] && key[q] < b; };
+    if (candidate(o)) {
+        atomicAdd(&ncand, 1);
+        const float so = logits[o] + prior[o];
+        int rank = 0;
+        for (int q = 0; q < no; ++q) {
+            if (q == o || !candidate(q)) continue;
+            const float sq = logits[q] + prior[q];
+            rank += (sq > so || (sq == so && q < o)) ? 1 : 0;
+        }
+        if (rank < top_o) act[kActHeader + rank] = o;
+    }
+    __syncthreads();
+    if (o == 0) {
+        act[0] = ncand < top_o ? ncand : top_o;
+        act[1] = 0;
+    }
+}
+
+// After the search: the searched orientations' minima become current, and
+// the step's choice is the lowest current key (the recheck and the stamp
+// read it from the result).
+__global__ __launch_bounds__(128) void k_lazy_merge(const unsigned long long* __restrict__ orient_best,
+                                                    const int* __restrict__ act, int orientations,
+                                                    unsigned long long* __restrict__ key,
+                                                    unsigned* __restrict__ valid, StepDeviceResult* __restrict__ r)
+{
+    __shared__ unsigned long long best;
+    const int i = threadIdx.x;
+    const int no = orientations < kOrient ? orientations : kOrient;
+    if (i == 0) best = ~0ULL;
+    if (i < act[0]) {
+        const int o = act[kActHeader + i];
+        key[o] = orient_best[o];
+        valid[o] = 1;
+    }
+    __syncthreads();
+    if (i < no && valid[i]) atomicMin(&best, key[i]);
+    __syncthreads();
+    if (i == 0) r->key = best;
 }
 
 // Lane-layout self-test of the MFMA tile: C = A (32x16) * B (16x32).
@@ -764,6 +887,10 @@ public:
         infer_slot_.allocate(sizeof(unsigned));
         infer_desc_.allocate(sizeof(float) * kDesc);
         cfg_.allocate(sizeof(int) * 2);
+        prior_.allocate(sizeof(float) * kOrient);
+        lazy_key_.allocate(sizeof(unsigned long long) * kOrient);
+        lazy_valid_.allocate(sizeof(unsigned) * kOrient);
+        SP_HIP_CHECK(hipMemsetAsync(prior_.get(), 0, prior_.bytes(), stream_));
         SP_HIP_CHECK(hipMemsetAsync(infer_slot_.get(), 0, sizeof(unsigned), stream_));
         SP_HIP_CHECK(hipMemsetAsync(m_.get(), 0, m_.bytes(), stream_));
         SP_HIP_CHECK(hipMemsetAsync(v_.get(), 0, v_.bytes(), stream_));
@@ -784,6 +911,22 @@ public:
         SP_HIP_CHECK(hipMemsetAsync(v_.get(), 0, v_.bytes(), stream_));
         adam_t_ = 0;
         cast(stream_);
+        SP_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    // The scene's orientation prior: selection ranks logit[o] + weight * prior[o]
+    // (zero beyond count, and everywhere until set).
+    void set_prior(const float* prior, std::uint32_t count, float weight)
+    {
+        if (count > std::uint32_t(kOrient)) throw std::invalid_argument("the prior has at most 96 entries");
+        if (!std::isfinite(weight)) throw std::invalid_argument("prior weight");
+        float h[kOrient] {};
+        for (std::uint32_t o = 0; o != count; ++o) {
+            if (!std::isfinite(prior[o])) throw std::invalid_argument("prior values must be finite");
+            h[o] = weight * prior[o];
+        }
+        use_device(ctx_);
+        SP_HIP_CHECK(hipMemcpyAsync(prior_.get(), h, sizeof h, hipMemcpyHostToDevice, stream_));
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
@@ -914,8 +1057,8 @@ public:
             hipLaunchKernelGGL(k_downsample, dim3(kV0 / 256), dim3(256), 0, st, v.env, v.ext[0], v.ext[1], v.ext[2],
                                infer_x_.as<u8>());
             forward(st, infer_x_.as<u8>(), infer_slot_.as<unsigned>(), 1, infer_desc_.as<float>());
-            hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, logits_.as<float>(), heat_.as<float>(), v.descs,
-                               v.orientations, cfg_.as<int>(), v.act, v.region);
+            hipLaunchKernelGGL(k_select, dim3(1), dim3(1024), 0, st, logits_.as<float>(), prior_.as<float>(),
+                               heat_.as<float>(), v.descs, v.orientations, cfg_.as<int>(), v.act, v.region);
         };
         session_launch_proposed(s, this, propose, int(top_o), stamp);
         session_read_result(s, out);
@@ -926,6 +1069,74 @@ public:
             used_fallback = 1;
         }
     }
+
+    // One lazy learned step (see k_lazy_*): the orientations whose last
+    // minimum the new occupancy blocked and that could beat the lowest
+    // current minimum are ranked by the network (plus the prior), the top_o
+    // of them are searched exactly, and the choice is the lowest current
+    // minimum.  Exact (the plain rule's choice) whenever no more than top_o
+    // orientations need a search.  The first step on a session after a
+    // reset, and a step that finds nothing, run the full search, which makes
+    // every minimum current.
+    void lazy_step(sp_hip_session* s, const float* desc, std::uint32_t top_o, bool stamp, sp_step_result& out,
+                   int& used_full, std::uint32_t& searched)
+    {
+        const SessionView v = session_view(s);
+        check_extent(v);
+        if (v.orientations > kOrient) throw std::invalid_argument("the proposal ranks at most 96 orientations");
+        if (top_o == 0 || top_o > std::uint32_t(v.orientations)) throw std::invalid_argument("top orientations");
+        use_device(ctx_);
+        used_full = 0;
+        searched = 0;
+        auto full = [&] {
+            session_launch_full(s, stamp);
+            hipLaunchKernelGGL(k_lazy_init, dim3(1), dim3(128), 0, v.stream, v.orient_best, v.orientations,
+                               lazy_key_.as<unsigned long long>(), lazy_valid_.as<unsigned>());
+            session_read_result(s, out);
+            lazy_session_ = s;
+            used_full = 1;
+            searched = std::uint32_t(v.fitting);
+        };
+        if (lazy_session_ != s) {
+            full();
+            return;
+        }
+        if (std::memcmp(desc, last_desc_, sizeof last_desc_) != 0 || !desc_set_) {
+            std::memcpy(last_desc_, desc, sizeof last_desc_);
+            SP_HIP_CHECK(hipMemcpyAsync(infer_desc_.get(), last_desc_, sizeof last_desc_, hipMemcpyHostToDevice,
+                                        v.stream));
+            desc_set_ = true;
+        }
+        if (int(top_o) != cfg_host_[0]) {
+            cfg_host_[0] = int(top_o);
+            SP_HIP_CHECK(hipMemcpyAsync(cfg_.get(), cfg_host_, sizeof cfg_host_, hipMemcpyHostToDevice, v.stream));
+        }
+        auto propose = [&, v](hipStream_t st) {
+            hipLaunchKernelGGL(k_lazy_check, dim3(v.orientations), dim3(256), 0, st, v.env, v.ext[0], v.ext[1],
+                               v.ext[2], v.kcells, v.descs, lazy_key_.as<unsigned long long>(),
+                               lazy_valid_.as<unsigned>());
+            hipLaunchKernelGGL(k_downsample, dim3(kV0 / 256), dim3(256), 0, st, v.env, v.ext[0], v.ext[1], v.ext[2],
+                               infer_x_.as<u8>());
+            forward(st, infer_x_.as<u8>(), infer_slot_.as<unsigned>(), 1, infer_desc_.as<float>());
+            hipLaunchKernelGGL(k_lazy_select, dim3(1), dim3(128), 0, st, logits_.as<float>(), prior_.as<float>(),
+                               v.descs, v.orientations, cfg_.as<int>(), lazy_key_.as<unsigned long long>(),
+                               lazy_valid_.as<unsigned>(), v.act);
+        };
+        const std::function<void(hipStream_t)> merge = [&, v](hipStream_t st) {
+            hipLaunchKernelGGL(k_lazy_merge, dim3(1), dim3(128), 0, st, v.orient_best, v.act, v.orientations,
+                               lazy_key_.as<unsigned long long>(), lazy_valid_.as<unsigned>(), v.result);
+        };
+        session_launch_proposed(s, this, propose, int(top_o), stamp, &merge);
+        int n = 0;
+        SP_HIP_CHECK(hipMemcpyAsync(&n, v.act, sizeof n, hipMemcpyDeviceToHost, v.stream));
+        session_read_result(s, out);
+        searched = std::uint32_t(n);
+        // nothing current and feasible: orientations left unsearched may
+        // still hold a placement
+        if (out.orientation < 0) full();
+    }
+
+    void lazy_reset() { lazy_session_ = nullptr; }
 
     void mfma_probe(const std::uint16_t* a, const std::uint16_t* b, float* c)
     {
@@ -1045,7 +1256,9 @@ private:
     hipStream_t stream_ {};
     DeviceBuffer params_, grads_, m_, v_, w1_, w_[3], wt_[3];
     DeviceBuffer slots_, desc_, film_, a1_, a2_, a3_, a4_, heat_, logits_, usave_, hsave_, msave_, pool_;
-    DeviceBuffer infer_x_, infer_slot_, infer_desc_, cfg_;
+    DeviceBuffer infer_x_, infer_slot_, infer_desc_, cfg_, prior_;
+    DeviceBuffer lazy_key_, lazy_valid_;   // per orientation: last minimum, and whether it is current
+    const sp_hip_session* lazy_session_ {}; // the session lazy_key_ belongs to (null: none)
     DeviceBuffer dz4_, dz3_, dz2_, dz1_, partial_, sums_[4], head_partial_, target_, omask_, region_, losses_;
     DeviceBuffer bank_;
     std::uint32_t bank_count_ {};
@@ -1123,6 +1336,12 @@ int sp_hip_proposal_set_parameters(sp_hip_proposal* p, const float* params, uint
     return proposal_guarded(p->ctx, [&] { p->p->set_parameters(params, count); });
 }
 
+int sp_hip_proposal_set_prior(sp_hip_proposal* p, const float* prior, uint32_t count, float weight)
+{
+    if (!p || (count && !prior)) return SP_INVALID_ARGUMENT;
+    return proposal_guarded(p->ctx, [&] { p->p->set_prior(prior, count, weight); });
+}
+
 int sp_hip_proposal_get_parameters(sp_hip_proposal* p, float* params, uint64_t count)
 {
     if (!p || !params) return SP_INVALID_ARGUMENT;
@@ -1187,6 +1406,31 @@ int sp_hip_proposal_step(sp_hip_proposal* p, sp_hip_session* s, const float desc
         return SP_CORRELATION_NUMERIC;
     }
     return rc;
+}
+
+int sp_hip_proposal_lazy_step(sp_hip_proposal* p, sp_hip_session* s, const float descriptor[32],
+                              uint32_t top_orientations, int stamp, sp_step_result* out, int* used_full,
+                              uint32_t* searched)
+{
+    if (!p || !s || !descriptor || !out) return SP_INVALID_ARGUMENT;
+    int full = 0;
+    std::uint32_t n = 0;
+    const int rc = proposal_guarded(p->ctx, [&] {
+        p->p->lazy_step(s, descriptor, top_orientations, stamp != 0, *out, full, n);
+    });
+    if (used_full) *used_full = full;
+    if (searched) *searched = n;
+    if (rc == SP_OK && ((out->numeric_flags | out->run_flags) & 7u)) {
+        p->ctx->c.last_error = "step numeric check failed";
+        return SP_CORRELATION_NUMERIC;
+    }
+    return rc;
+}
+
+int sp_hip_proposal_lazy_reset(sp_hip_proposal* p)
+{
+    if (!p) return SP_INVALID_ARGUMENT;
+    return proposal_guarded(p->ctx, [&] { p->p->lazy_reset(); });
 }
 
 int sp_hip_proposal_mfma_probe(sp_hip_proposal* p, const uint16_t* a, const uint16_t* b, float* c)

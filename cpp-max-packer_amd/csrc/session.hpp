@@ -24,6 +24,9 @@ struct SessionView {
     unsigned char* region;     // region mask, up to kMaxRegionsPerAxis^3 bytes
     int max_regions_per_axis;
     const OrientDesc* descs;   // one per orientation (lx <= 0: does not fit)
+    const unsigned char* kcells;      // kernel cells, X-fastest, at descs[o].cells_offset
+    unsigned long long* orient_best;  // per orientation, the last step's lowest feasible key (~0: none)
+    StepDeviceResult* result;         // the step's result (key read by the recheck and the stamp)
     hipStream_t stream;
 };
 
@@ -32,9 +35,11 @@ SessionView session_view(sp_hip_session* s);
 
 // Runs `propose` (which fills act / region on the session stream) followed by
 // the subset search over `slots` grid slots; recorded once per owner as a
-// hipGraph when the session captures graphs.
+// hipGraph when the session captures graphs.  merge (nullable) runs after the
+// search and before the recheck of the chosen corner and the stamp: it may
+// replace the result's key.
 void session_launch_proposed(sp_hip_session* s, const void* owner, const std::function<void(hipStream_t)>& propose,
-                             int slots, bool stamp);
+                             int slots, bool stamp, const std::function<void(hipStream_t)>* merge = nullptr);
 
 // The plain search over every orientation, whatever restriction is set.
 void session_launch_full(sp_hip_session* s, bool stamp);

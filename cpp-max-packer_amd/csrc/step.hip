@@ -1640,7 +1640,8 @@ public:
         }
     }
 
-    template <typename T> void enqueue_step(bool stamp, const int* act, int slots)
+    template <typename T>
+    void enqueue_step(bool stamp, const int* act, int slots, const std::function<void(hipStream_t)>* merge = nullptr)
     {
         using T2 = cx<T>;
         const int nx = n_[0], ny = n_[1], nz = n_[2];
@@ -1712,6 +1713,7 @@ public:
             run_pass<T>(kPassC2r, tile_[kPassC2r], act, slots);
             mark("c2r_z_argmin");
         }
+        if (merge) (*merge)(stream_);
         enqueue_verify();
         if (stamp) {
             enqueue_stamp(nullptr, 0, 0, nullptr);
@@ -1813,7 +1815,8 @@ public:
     // A proposal step: `propose` enqueues work on the session stream that
     // writes the subset list (at most `slots` orientations) and the region
     // mask on the device, then the subset search runs; both in one graph.
-    void launch_proposed(const void* owner, const std::function<void(hipStream_t)>& propose, int slots, bool stamp)
+    void launch_proposed(const void* owner, const std::function<void(hipStream_t)>& propose, int slots, bool stamp,
+                         const std::function<void(hipStream_t)>* merge = nullptr)
     {
         if (incremental_) throw std::invalid_argument("proposal steps need a session without incremental steps");
         prepare();
@@ -1824,15 +1827,15 @@ public:
         if (stamp) ++env_rev_;
         auto record = [&] {
             propose(stream_);
-            if (precision_ == 64) enqueue_step<double>(stamp, act_sub_.as<int>(), slots);
-            else enqueue_step<float>(stamp, act_sub_.as<int>(), slots);
+            if (precision_ == 64) enqueue_step<double>(stamp, act_sub_.as<int>(), slots, merge);
+            else enqueue_step<float>(stamp, act_sub_.as<int>(), slots, merge);
         };
         if (!desc_.capture_graph) {
             record();
             SP_HIP_CHECK(hipGetLastError());
             return;
         }
-        run_graph(2, stamp, owner, slots, record);
+        run_graph(merge ? 3 : 2, stamp, owner, slots, record);
     }
 
     SessionView view()
@@ -1847,6 +1850,9 @@ public:
         v.region = region_.as<unsigned char>();
         v.max_regions_per_axis = kMaxRegionsPerAxis;
         v.descs = d_descs_.as<OrientDesc>();
+        v.kcells = kcells_.as<unsigned char>();
+        v.orient_best = orient_best_.as<unsigned long long>();
+        v.result = result_.as<StepDeviceResult>();
         v.stream = stream_;
         return v;
     }
@@ -2440,9 +2446,9 @@ namespace sphip {
 SessionView session_view(sp_hip_session* s) { return s->s->view(); }
 
 void session_launch_proposed(sp_hip_session* s, const void* owner, const std::function<void(hipStream_t)>& propose,
-                             int slots, bool stamp)
+                             int slots, bool stamp, const std::function<void(hipStream_t)>* merge)
 {
-    s->s->launch_proposed(owner, propose, slots, stamp);
+    s->s->launch_proposed(owner, propose, slots, stamp, merge);
 }
 
 void session_launch_full(sp_hip_session* s, bool stamp) { s->s->launch_mode(stamp, false); }

@@ -626,6 +626,13 @@ class Proposal:
         a = np.ascontiguousarray(np.asarray(flat, dtype=np.float32).ravel())
         self._check(lib().sp_hip_proposal_set_parameters(self._h, _p(a, C.c_float), C.c_uint64(a.size)))
 
+    def set_prior(self, prior, weight=1.0):
+        """Per-orientation prior added to the logits (times weight) when a
+        learned step ranks orientations; an empty prior clears it."""
+        a = np.ascontiguousarray(np.asarray(prior, dtype=np.float32).ravel())
+        self._check(lib().sp_hip_proposal_set_prior(self._h, _p(a, C.c_float) if a.size else None,
+                                                   C.c_uint32(a.size), C.c_float(weight)))
+
     def parameters(self):
         a = np.zeros(self.parameter_count, np.float32)
         self._check(lib().sp_hip_proposal_get_parameters(self._h, _p(a, C.c_float), C.c_uint64(a.size)))
@@ -672,6 +679,27 @@ class Proposal:
                                                 C.c_float(heat_weight), C.c_float(lr), C.c_int(1 if update else 0),
                                                 _p(losses, C.c_float), _p(g, C.c_float) if gradients else None))
         return (float(losses[0]), float(losses[1])), g
+
+    def lazy_step(self, session: "Session", descriptor, top_orientations=16, stamp=True):
+        """One lazy learned step (sp_hip_proposal_lazy_step): exact minima kept
+        per orientation, the top-ranked blocked orientations searched."""
+        d = np.ascontiguousarray(np.asarray(descriptor, dtype=np.float32).ravel())
+        if d.size != 32:
+            raise ValueError("descriptor must hold 32 floats")
+        r = StepResult()
+        full = C.c_int(0)
+        searched = C.c_uint32(0)
+        self._check(lib().sp_hip_proposal_lazy_step(self._h, session._h, _p(d, C.c_float),
+                                                    C.c_uint32(top_orientations), C.c_int(1 if stamp else 0),
+                                                    C.byref(r), C.byref(full), C.byref(searched)))
+        out = Session._result(r)
+        out["used_full"] = bool(full.value)
+        out["searched"] = int(searched.value)
+        return out
+
+    def lazy_reset(self):
+        """Forget the kept minima (the next lazy step runs the full search)."""
+        self._check(lib().sp_hip_proposal_lazy_reset(self._h))
 
     def step(self, session: "Session", descriptor, top_orientations=8, top_regions=512, stamp=True, fallback=True):
         d = np.ascontiguousarray(np.asarray(descriptor, dtype=np.float32).ravel())

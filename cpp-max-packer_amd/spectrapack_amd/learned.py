@@ -57,6 +57,22 @@ def orientation_heights(kernels):
     return np.array(out)
 
 
+def orientation_prior(kernels, valid):
+    """The orientation prior of a learned step (96 entries): per fitting
+    orientation the negated standardised mean resting height of its kernel
+    (orientation_heights), so that orientations resting low rank high; 0 for
+    the others."""
+    h = orientation_heights(kernels)
+    v = np.asarray(valid[:len(kernels)], bool)
+    out = np.zeros(96, np.float32)
+    if v.any():
+        sd = float(h[v].std())
+        vals = np.zeros(len(kernels), np.float32)
+        vals[v] = -(h[v] - h[v].mean()) / (sd if sd > 0 else 1.0)
+        out[:len(kernels)] = vals
+    return out
+
+
 def descriptor(kernels, catalog, env_shape):
     """32 floats describing a part through its oriented kernels (cells
     [z, y, x], shape (x, y, z)) and the catalog that produced them:
@@ -157,6 +173,7 @@ class Scene:
         es = w.env_shape
         self.fits = np.array([all(k["shape"][a] <= es[a] for a in range(3)) for k in self.kernels] +
                              [False] * (96 - len(self.kernels)), np.uint8)
+        self.prior = orientation_prior(self.kernels, self.fits)
         self.reset()
 
     def reset(self):
@@ -266,6 +283,160 @@ def collect(ctx, proposal: Proposal, seeds, max_steps=100000, log=print, first_s
     return out
 
 
+def lazy_labels(prev, cur):
+    """Labels of one step for the lazy learned step, from the plain search's
+    per-orientation minima before (prev, the last step) and after (cur) the
+    last placement (None: no feasible corner).  An orientation's minimum
+    changes only when the placement blocked it, and then the old minimum
+    bounds the new one from below; the orientations whose minimum is
+    unchanged are current.  Candidates: blocked orientations whose bound
+    lies below the lowest current minimum (a lazy step ranks and searches
+    them).  Returns (candidate mask, target), the target uniform over the
+    candidates whose new minimum reaches the step's lowest z, or None when
+    the ranking cannot change the step's height (a current minimum lies at
+    the lowest z, or no candidate reaches it)."""
+    def key(m, o):
+        return (m[0], m[1], m[2], o)
+    n = len(cur)
+    best = None
+    for o in range(n):
+        if prev[o] is not None and cur[o] == prev[o]:
+            k = key(cur[o], o)
+            best = k if best is None or k < best else best
+    cand = np.zeros(96, bool)
+    for o in range(n):
+        if prev[o] is None or cur[o] == prev[o]:
+            continue  # infeasible stays infeasible; unchanged minima are current
+        if best is None or key(prev[o], o) < best:
+            cand[o] = True
+    zs = [m[0] for m in cur if m is not None]
+    if not zs or not cand.any():
+        return cand, None
+    zmin = min(zs)
+    if best is not None and best[0] <= zmin:
+        return cand, None  # a current minimum already lies at the lowest z
+    reach = np.array([bool(cand[o]) and cur[o] is not None and cur[o][0] == zmin for o in range(n)] +
+                     [False] * (96 - n))
+    if not reach.any():
+        return cand, None
+    t = np.zeros(96, np.float32)
+    t[reach] = 1.0 / reach.sum()
+    return cand, t
+
+
+def collect_lazy(ctx, proposal: Proposal, seeds, max_steps=100000, log=print, first_slot=0, threads=1, log_every=1,
+                 min_candidates=2):
+    """Plain-search episodes (incremental steps, identical choices) labelled
+    for the lazy learned step (lazy_labels): the occupancy before each step
+    is recorded, and kept as a sample when at least min_candidates
+    orientations are candidates and one of them reaches the step's lowest z;
+    the sample's orientation mask is the candidate set.  Returns the labels
+    and per-episode statistics, as collect does."""
+    import threading
+    lock = threading.Lock()
+    state = dict(slot=first_slot, next=0, done=0, t0=time.perf_counter())
+    rec = dict(slot=[], desc=[], target=[], mask=[], region=[], orientation=[], episode=[], tied=[])
+    episodes = [None] * len(seeds)
+    errors = []
+    local = threading.local()
+
+    def take_slot():
+        with lock:
+            if state["slot"] >= proposal.samples:
+                raise RuntimeError("sample bank full")
+            state["slot"] += 1
+            return state["slot"] - 1
+
+    def episode(e, seed):
+        t0 = time.perf_counter()
+        if threads > 1 and not hasattr(local, "ctx"):
+            local.ctx = Context(ctx.device)
+        sc = Scene(local.ctx if threads > 1 else ctx, W.star_so3_box(int(seed)))
+        s = sc.session
+        s.set_autotune(False)
+        s.set_incremental(True)
+        rows = []
+        prev = None
+        slot = None
+        steps = 0
+        while steps < max_steps:
+            if slot is None:
+                slot = take_slot()
+            proposal.record(s, slot)
+            r = s.step(stamp=True)
+            steps += 1
+            if r["orientation"] < 0:
+                break
+            cur = s.orientation_minima()
+            if prev is not None:
+                cand, target = lazy_labels(prev, cur)
+                if target is not None and cand.sum() >= min_candidates:
+                    u = tuple(r["key"][a] - sc.w.env_first[a] for a in range(3))
+                    rows.append((slot, target, cand.astype(np.uint8), region_index(u, sc.w.env_shape),
+                                 r["orientation"], target > 0))
+                    slot = None  # kept; the next step takes a new slot
+            prev = cur
+        parts = sum(1 for c in s.copies())
+        dt = time.perf_counter() - t0
+        with lock:
+            for slot_, target, mask, region, o, tied in rows:
+                rec["slot"].append(slot_)
+                rec["desc"].append(sc.desc)
+                rec["target"].append(target)
+                rec["mask"].append(mask)
+                rec["region"].append(region)
+                rec["orientation"].append(o)
+                rec["episode"].append(e)
+                rec["tied"].append(tied)
+            episodes[e] = dict(seed=int(seed), parts=parts, seconds=dt, samples=len(rows))
+            state["done"] += 1
+            if state["done"] % log_every == 0 or state["done"] == len(seeds):
+                log(f"[collect-lazy] {state['done']}/{len(seeds)} episodes, seed={seed} parts={parts} "
+                    f"samples={len(rec['slot'])} {time.perf_counter() - state['t0']:.1f}s")
+        sc.close()
+
+    def worker():
+        while True:
+            with lock:
+                e = state["next"]
+                state["next"] += 1
+            if e >= len(seeds) or errors:
+                return
+            try:
+                episode(e, seeds[e])
+            except Exception as ex:  # reported after the join
+                errors.append(ex)
+                return
+
+    pool = [threading.Thread(target=worker) for _ in range(max(1, threads))]
+    for t in pool:
+        t.start()
+    for t in pool:
+        t.join()
+    if errors:
+        raise errors[0]
+    order = np.lexsort((np.asarray(rec["slot"]), np.asarray(rec["episode"])))
+    out = {k: np.asarray(v)[order] for k, v in rec.items()}
+    out["episodes"] = episodes
+    return out
+
+
+def lazy_hits(proposal: Proposal, data, idx, top_o=12, prior=None, prior_weight=0.0, batch=32):
+    """Fraction of lazy-labelled samples whose top_o candidates by score
+    (logit + prior_weight x prior; prior: [n][96] or None) contain a
+    candidate that reaches the step's lowest z."""
+    hits = 0
+    for i in range(0, len(idx), batch):
+        j = idx[i:i + batch]
+        logits, _ = proposal.forward(data["slot"][j], data["desc"][j])
+        for k, s in enumerate(j):
+            sc = logits[k] + (prior_weight * prior[s] if prior is not None else 0.0)
+            sc = np.where(data["mask"][s] != 0, sc, -np.inf)
+            top = np.argsort(-sc, kind="stable")[:top_o]
+            hits += bool(data["tied"][s][top].any())
+    return hits / max(len(idx), 1)
+
+
 def ranking_hits(proposal: Proposal, data, idx, top_o=8, top_r=512, batch=32):
     """Fractions of samples whose plain choice lies in the proposal: chosen
     orientation among the top_o valid logits, chosen region among the top_r
@@ -317,10 +488,13 @@ def train(proposal: Proposal, data, train_idx, epochs=6, batch=32, lr=1e-3, heat
 
 
 def run_episode(scene: Scene, proposal: Proposal | None = None, top_o=8, top_r=512, max_steps=100000,
-                verify_z=False):
+                verify_z=False, prior_weight=0.0):
     """Packs until nothing fits: the plain search, or proposals with the full
     search as fallback.  Every choice is stamped.  Returns parts, steps,
     seconds and fallbacks; steps include the final one that finds nothing.
+
+    prior_weight: proposals rank orientations by logit + prior_weight x the
+    scene's orientation prior (orientation_prior).
 
     verify_z: a proposal's choice is kept only when it lies at the lowest z
     any step can still reach, else the step is the full search.  The lowest
@@ -330,6 +504,8 @@ def run_episode(scene: Scene, proposal: Proposal | None = None, top_o=8, top_r=5
     raises the bound."""
     s = scene.session
     ez = scene.w.env_first[2]
+    if proposal is not None:
+        proposal.set_prior(scene.prior, prior_weight)
     # warm-up outside the timed loop: tuning and graph capture of both modes
     scene.reset()
     if proposal is not None:
@@ -361,6 +537,37 @@ def run_episode(scene: Scene, proposal: Proposal | None = None, top_o=8, top_r=5
         parts += 1
     dt = time.perf_counter() - t0
     return dict(parts=parts, steps=steps, seconds=dt, steps_per_s=steps / dt, fallbacks=fallbacks)
+
+
+def run_episode_lazy(scene: Scene, proposal: Proposal, top_o=16, prior_weight=0.0, max_steps=100000):
+    """Packs until nothing fits with lazy learned steps (Proposal.lazy_step):
+    every orientation's last exact minimum is kept while its corner stays
+    free, and of the orientations the new occupancy blocked, the top_o by
+    score are searched.  Returns parts, steps, seconds, the full searches and
+    the mean number of orientations searched per step."""
+    s = scene.session
+    proposal.set_prior(scene.prior, prior_weight)
+    # warm-up outside the timed loop: tuning and graph capture of both modes
+    scene.reset()
+    proposal.lazy_reset()
+    for _ in range(2):
+        proposal.lazy_step(s, scene.desc, top_o, stamp=True)
+    scene.reset()
+    proposal.lazy_reset()
+    parts = steps = full = searched = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        r = proposal.lazy_step(s, scene.desc, top_o, stamp=True)
+        full += r["used_full"]
+        searched += r["searched"]
+        steps += 1
+        if r["orientation"] < 0:
+            break
+        parts += 1
+    dt = time.perf_counter() - t0
+    proposal.lazy_reset()
+    return dict(parts=parts, steps=steps, seconds=dt, steps_per_s=steps / dt, full_searches=full,
+                searched_per_step=searched / max(steps, 1))
 
 
 def run_episode_static(scene: Scene, orientations, max_steps=100000):

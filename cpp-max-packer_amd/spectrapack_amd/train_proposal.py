@@ -2,11 +2,15 @@
 with and without it.
 
     python -m spectrapack_amd.train_proposal --episodes 200 --out DIR
+    python -m spectrapack_amd.train_proposal --labels lazy --init W.npy --out DIR
 
 Collects plain-search episodes of the 130^3 star scene with 96 orientations,
 trains on all but the last tenth of them, reports ranking hit rates on the
 held-out tenth, then packs held-out seeds with the plain search and with
 proposals (at 130^3 and at 256^3), writing weights and a JSON report to DIR.
+--labels lazy labels the steps for the lazy learned step instead
+(learned.lazy_labels: rank the blocked candidates, target the ones reaching
+the lowest z) and packs with lazy steps.
 """
 from __future__ import annotations
 
@@ -40,7 +44,12 @@ def main(argv=None):
     ap.add_argument("--skip-packing", action="store_true", help="train only (no packing comparison)")
     ap.add_argument("--out", default="gpurun_out/proposal")
     ap.add_argument("--init", default="", help="start from these weights (.npy)")
+    ap.add_argument("--labels", choices=("plain", "lazy"), default="plain")
+    ap.add_argument("--prior-weight", type=float, default=2.0, help="lazy packing: weight of the orientation prior")
+    ap.add_argument("--eval-seed-base", type=int, default=9000, help="first held-out seed packed")
     args = ap.parse_args(argv)
+    if args.labels == "lazy":
+        return main_lazy(args)
 
     from spectrapack_amd import Context, Proposal
     from spectrapack_amd import learned as L
@@ -123,6 +132,80 @@ def main(argv=None):
             sc.close()
             with open(os.path.join(args.out, "report.json"), "w") as f:
                 json.dump(report, f, indent=1)
+    log("[done]")
+    return 0
+
+
+def main_lazy(args):
+    """Training on lazy-step labels (see the module docstring)."""
+    from spectrapack_amd import Context, Proposal
+    from spectrapack_amd import learned as L
+    from spectrapack_amd import workloads as W
+
+    def log(msg):
+        print(msg, flush=True)
+
+    os.makedirs(args.out, exist_ok=True)
+    ctx = Context(0)
+    prop = Proposal(ctx, max_batch=args.batch)
+    prop.reserve(args.episodes * 150)
+    flat = np.load(args.init, allow_pickle=False) if args.init else L.initial_parameters(args.seed)
+    prop.set_parameters(flat)
+    init_flat = prop.parameters()
+    seeds = [args.seed_base + i for i in range(args.episodes)]
+    t0 = time.perf_counter()
+    data = L.collect_lazy(ctx, prop, seeds, max_steps=args.max_steps, log=log, threads=args.threads,
+                          log_every=max(1, args.episodes // 40))
+    t_collect = time.perf_counter() - t0
+    n = len(data["slot"])
+    held = max(1, args.episodes // 10)
+    test_idx = np.nonzero(data["episode"] >= args.episodes - held)[0]
+    train_idx = np.nonzero(data["episode"] < args.episodes - held)[0]
+    # the prior of each sample's part (its scene's orientation prior)
+    priors = {}
+    prior = np.zeros((n, 96), np.float32)
+    for i in range(n):
+        e = int(data["episode"][i])
+        if e not in priors:
+            sc = L.Scene(ctx, W.star_so3_box(seeds[e]))
+            priors[e] = sc.prior
+            sc.close()
+        prior[i] = priors[e]
+    log(f"[data] samples={n} train={len(train_idx)} held-out={len(test_idx)} collect={t_collect:.1f}s")
+    hits = {}
+    for name, w in (("before", args.prior_weight), ("before_no_prior", 0.0)):
+        hits[name] = L.lazy_hits(prop, data, test_idx, args.top_o, prior, w)
+    t0 = time.perf_counter()
+    hist = L.train(prop, data, train_idx, epochs=args.epochs, batch=args.batch, lr=args.lr,
+                   heat_weight=args.heat_weight, seed=args.seed, log=log)
+    t_train = time.perf_counter() - t0
+    hits["after"] = L.lazy_hits(prop, data, test_idx, args.top_o, prior, args.prior_weight)
+    hits["after_no_prior"] = L.lazy_hits(prop, data, test_idx, args.top_o, prior, 0.0)
+    hits["prior_only"] = L.lazy_hits(prop, data, test_idx, args.top_o, prior * 1e6, 1.0)
+    log(f"[rank] top-{args.top_o} held-out hits {hits}")
+    meta = dict(labels="lazy", episodes=args.episodes, samples=int(n), train_samples=int(len(train_idx)),
+                heldout_samples=int(len(test_idx)), epochs=args.epochs, batch=args.batch, lr=args.lr,
+                heat_weight=args.heat_weight, collect_seconds=t_collect, train_seconds=t_train, history=hist,
+                lazy_hits=hits, top_o=args.top_o, prior_weight=args.prior_weight,
+                scene="star512-in-box128-h1-o96", catalog_seed=W.PROPOSAL_CATALOG_SEED,
+                episode_seeds=[seeds[0], seeds[-1]], init=os.path.basename(args.init) if args.init else None)
+    L.save_weights(os.path.join(args.out, "proposal_star96.npy"), prop.parameters(), meta)
+    report = dict(meta=meta, packing=[])
+    if not args.skip_packing:
+        before = Proposal(ctx, max_batch=1)
+        before.set_parameters(init_flat)
+        for k in range(args.eval_fine):
+            seed = args.eval_seed_base + k
+            sc = L.Scene(ctx, W.star_so3_fine_box(seed))
+            plain = L.run_episode(sc)
+            lazy = L.run_episode_lazy(sc, prop, args.top_o, args.prior_weight)
+            lazy0 = L.run_episode_lazy(sc, before, args.top_o, args.prior_weight)
+            log(f"[pack] seed={seed} plain {plain['parts']} | lazy trained {lazy['parts']} "
+                f"{lazy['steps_per_s'] / plain['steps_per_s']:.2f}x | lazy before {lazy0['parts']}")
+            report["packing"].append(dict(seed=seed, plain=plain, lazy=lazy, lazy_before=lazy0))
+            sc.close()
+    with open(os.path.join(args.out, "report.json"), "w") as f:
+        json.dump(report, f, indent=1)
     log("[done]")
     return 0
 

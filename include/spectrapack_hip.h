@@ -486,14 +486,38 @@ int sp_hip_proposal_forward(sp_hip_proposal* p, const uint32_t* slots, uint32_t 
 int sp_hip_proposal_train(sp_hip_proposal* p, const uint32_t* slots, uint32_t batch, const float* descriptors,
                           const float* orientation_target, const uint8_t* orientation_mask, const int32_t* region,
                           float heat_weight, float learning_rate, int update, float losses[2], float* gradients);
+/* The scene's orientation prior for sp_hip_proposal_step: orientations are
+ * ranked by logit[o] + weight * prior[o] (prior[o] = 0 for o >= count, and
+ * for every o until this is called).  count <= 96; values finite. */
+int sp_hip_proposal_set_prior(sp_hip_proposal* p, const float* prior, uint32_t count, float weight);
 /* One learned placement step on a session with at most 96 orientations and
  * at least 64 cells per axis: the top_orientations fitting orientations by
- * logit (ties: lower index) searched exactly in the top_regions regions by
- * heat logit, all in one graph.  When nothing there is feasible and fallback
+ * score (logit plus the prior term; ties: lower index) searched exactly in
+ * the top_regions regions by heat logit (4096: every region, no region
+ * mask), all in one graph.  When nothing there is feasible and fallback
  * != 0, the full search runs (*used_fallback = 1).  out->orientation is -1
  * when no step was found. */
 int sp_hip_proposal_step(sp_hip_proposal* p, sp_hip_session* s, const float descriptor[32], uint32_t top_orientations,
                          uint32_t top_regions, int stamp, int fallback, sp_step_result* out, int* used_fallback);
+/* One lazy learned step.  The proposal keeps every orientation's last
+ * lowest feasible corner; occupancy only grows, so a corner still free is
+ * still that orientation's minimum, and a blocked one bounds the new minimum
+ * from below.  The orientations whose corner the occupancy blocked and whose
+ * bound lies below the lowest current minimum are ranked by score (logit
+ * plus the prior term; ties: lower index), the top_orientations of them are
+ * searched exactly (every region), and the step places the lowest current
+ * minimum: the plain search's choice whenever no more than
+ * top_orientations orientations needed a search.  The first lazy step on a
+ * session (after sp_hip_proposal_lazy_reset or on another session) and a
+ * step whose known minima hold nothing feasible run the full search
+ * (*used_full = 1), which makes every minimum current.  *searched: the
+ * orientations searched (nullable outputs). */
+int sp_hip_proposal_lazy_step(sp_hip_proposal* p, sp_hip_session* s, const float descriptor[32],
+                              uint32_t top_orientations, int stamp, sp_step_result* out, int* used_full,
+                              uint32_t* searched);
+/* Forgets the kept minima: the next lazy step runs the full search.  Call it
+ * whenever the session's occupancy is set or cells are freed. */
+int sp_hip_proposal_lazy_reset(sp_hip_proposal* p);
 /* Self-test of the MFMA tile layout: c[32][32] = a[32][16] * b[16][32], BF16
  * bit patterns in, FP32 out. */
 int sp_hip_proposal_mfma_probe(sp_hip_proposal* p, const uint16_t* a, const uint16_t* b, float* c);

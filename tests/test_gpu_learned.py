@@ -246,3 +246,105 @@ def test_learned_step_rejects_small_or_wide_sessions(ctx, oracle, proposal):
         s.set_kernel(o, np.ones((2, 2, 2), np.uint8), (2, 2, 2), (0, 0, 0))
     with pytest.raises(sp.Failure):
         proposal.step(s, np.zeros(32, np.float32))
+
+
+def test_prior_enters_the_ranking(ctx, oracle, proposal):
+    """With a prior set, the learned step ranks orientations by logit +
+    weight x prior; with every region allowed (4096) no region mask applies:
+    the step is the restricted search over those orientations."""
+    first, env, kernels = _box_scene(oracle)
+    s = _session(ctx, env, first, kernels)
+    for _ in range(4):
+        s.step(stamp=True)
+    env = s.environment()
+    rs = np.random.default_rng(9)
+    desc = rs.normal(size=32).astype(np.float32)
+    proposal.set_parameters(params(9))
+    proposal.reserve(1)
+    proposal.record(s, 0)
+    logits, _ = proposal.forward([0], desc[None])
+    es = env.shape[::-1]
+    fits = np.array([all(k["shape"][a] <= es[a] for a in range(3)) for k in kernels])
+    prior = rs.normal(size=len(kernels)).astype(np.float32)
+    _, _, fields = oracle.place_step(env, first, kernels, direct=True, fields=True)
+    for weight in (0.0, 0.5, 50.0):
+        proposal.set_prior(prior, weight)
+        score = logits[0, :len(kernels)] + np.float32(weight) * prior
+        orients = _top(score, 3, fits)
+        o_exp, u_exp, _ = restricted_choice(fields, es, orients)
+        r = proposal.step(s, desc, top_orientations=3, top_regions=4096, stamp=False, fallback=False)
+        print(f"[prior] weight={weight} orientations={orients} -> {r['orientation']}", flush=True)
+        assert r["orientation"] == o_exp
+        if o_exp >= 0:
+            assert tuple(r["key"][a] - first[a] for a in range(3))[::-1] == u_exp
+    proposal.set_prior([], 0.0)
+
+
+def test_lazy_step_against_its_restatement(ctx, oracle, proposal):
+    """Lazy learned steps (3 orientations searched per step) on the box
+    scene, against a host restatement on the oracle's overlap fields: keep
+    each orientation's minimum, drop it when the occupancy covers its corner,
+    search the 3 best-scored orientations whose dropped minimum lies below the
+    lowest kept one, place the lowest kept minimum; the first step, and a
+    step with nothing kept, is the full search.  Every step's choice and the
+    final occupancy match bit for bit."""
+    first, env, kernels = _box_scene(oracle)
+    s = _session(ctx, env, first, kernels)
+    rs = np.random.default_rng(10)
+    desc = rs.normal(size=32).astype(np.float32)
+    proposal.set_parameters(params(10))
+    proposal.reserve(1)
+    prior = rs.normal(size=len(kernels)).astype(np.float32)
+    weight = 0.7
+    proposal.set_prior(prior, weight)
+    proposal.lazy_reset()
+    es = env.shape[::-1]
+    n = len(kernels)
+    fits = [all(k["shape"][a] <= es[a] for a in range(3)) for k in kernels]
+    top = 3
+    cur = env.copy()
+    keys, valid = None, None
+
+    def full(minima):
+        return list(minima), [True] * n
+
+    def lowest(keys, valid):
+        c = [(keys[o], o) for o in range(n) if valid[o] and keys[o] is not None]
+        return min(c) if c else None
+
+    partial = 0
+    for step in range(16):
+        _, _, fields = oracle.place_step(cur, first, kernels, direct=True, fields=True)
+        minima = restricted_choice(fields, es)[2]
+        proposal.record(s, 0)
+        logits, _ = proposal.forward([0], desc[None])
+        if keys is None:
+            keys, valid = full(minima)
+        else:
+            for o in range(n):
+                if valid[o] and keys[o] is not None and fields[o][keys[o]] != 0:
+                    valid[o] = False
+            best = lowest(keys, valid)
+            cand = [o for o in range(n) if fits[o] and not valid[o] and (best is None or (keys[o], o) < best)]
+            score = logits[0, :n] + np.float32(weight) * prior
+            pick = sorted(cand, key=lambda o: (-float(score[o]), o))[:top]
+            partial += len(cand) > top
+            for o in pick:
+                keys[o], valid[o] = minima[o], True
+            if lowest(keys, valid) is None:
+                keys, valid = full(minima)
+        want = lowest(keys, valid)
+        r = proposal.lazy_step(s, desc, top, stamp=True)
+        print(f"[lazy] step {step}: searched {r['searched']} full {r['used_full']} -> {r['orientation']}", flush=True)
+        if want is None:
+            assert r["orientation"] == -1
+            break
+        (uz, uy, ux), o = want
+        assert r["orientation"] == o, step
+        assert tuple(r["key"][a] - first[a] for a in range(3)) == (ux, uy, uz), step
+        cur = oracle.stamp(cur, first, kernels[o], tuple(r["translation"]))
+        # a placed minimum is no longer free: the next step drops it
+    assert np.array_equal(s.environment(), cur)
+    assert partial > 0  # some steps searched only part of the candidates
+    proposal.set_prior([], 0.0)
+    proposal.lazy_reset()
