@@ -343,8 +343,8 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             tj = json.load(f)
-        if tj.get("precision") == prec and tj.get("kernel") == dominant:
-            traffic = tj.get("hbm_bytes_per_launch")
+        if tj.get("precision") == prec:
+            traffic = tj.get("per_kernel_hbm_bytes", {}).get(dominant)
     # issue side of the same kernel: VALU wave-instructions per launch measured
     # with rocprofv3 (SQ_INSTS_VALU) on this workload, against the chip's issue
     # rate of one VALU wave-instruction per CU per clock
