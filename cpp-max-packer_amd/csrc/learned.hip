@@ -745,30 +745,35 @@ __global__ void k_lazy_init(const unsigned long long* __restrict__ orient_best, 
     valid[o] = 1;
 }
 
-// One block per orientation: a current key whose kernel now overlaps the
-// occupancy (or leaves the window) is no longer the minimum.
-__global__ void k_lazy_check(const u8* __restrict__ env, int ex, int ey, int ez, const u8* __restrict__ kcells,
-                             const OrientDesc* __restrict__ descs, const unsigned long long* __restrict__ key,
-                             unsigned* __restrict__ valid)
+// A current key whose kernel now overlaps the occupancy (or leaves the
+// window) is no longer the minimum.  Blocks (orientation, slice): each wave
+// takes kernel rows (z, y) in turn, its lanes the row's x cells, so no cell
+// index is divided; every block that finds an overlap clears the flag.
+constexpr int kLazySlices = 8;
+
+__global__ __launch_bounds__(256) void k_lazy_check(const u8* __restrict__ env, int ex, int ey, int ez,
+                                                    const u8* __restrict__ kcells,
+                                                    const OrientDesc* __restrict__ descs,
+                                                    const unsigned long long* __restrict__ key,
+                                                    unsigned* __restrict__ valid)
 {
     const int o = blockIdx.x;
     const unsigned long long k = key[o];
     if (!valid[o] || k == ~0ULL) return;  // uniform
-    __shared__ unsigned hit;
-    if (threadIdx.x == 0) hit = 0;
-    __syncthreads();
     const int ux = int((k >> 16) & 0xffffULL), uy = int((k >> 32) & 0xffffULL), uz = int(k >> 48);
     const OrientDesc d = descs[o];
-    const long long n = (long long)d.kx * d.ky * d.kz;
-    unsigned mine = 0;
-    for (long long i = threadIdx.x; i < n; i += blockDim.x) {
-        if (!kcells[d.cells_offset + i]) continue;
-        const int gx = ux + int(i % d.kx), gy = uy + int((i / d.kx) % d.ky), gz = uz + int(i / ((long long)d.kx * d.ky));
-        if (gx >= ex || gy >= ey || gz >= ez || env[gx + (long long)ex * (gy + (long long)ey * gz)]) mine = 1;
+    const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    const int rows = d.ky * d.kz;
+    unsigned hit = 0;
+    for (int r = blockIdx.y * waves + (threadIdx.x >> 6); r < rows; r += waves * gridDim.y) {
+        const int z = r / d.ky, y = r - z * d.ky;
+        const u8* krow = kcells + d.cells_offset + (long long)r * d.kx;
+        const bool inside = uz + z < ez && uy + y < ey;
+        const long long erow = (long long)ex * ((uy + y) + (long long)ey * (uz + z)) + ux;
+        for (int x = lane; x < d.kx; x += 64)
+            if (krow[x] && (!inside || ux + x >= ex || env[erow + x])) hit = 1;
     }
-    if (mine) atomicOr(&hit, 1u);
-    __syncthreads();
-    if (threadIdx.x == 0 && hit) valid[o] = 0;
+    if (__any(hit) && lane == 0) valid[o] = 0;
 }
 
 // The search set of a lazy learned step: the orientations whose key is not
@@ -783,6 +788,8 @@ __global__ __launch_bounds__(128) void k_lazy_select(const float* __restrict__ l
 {
     __shared__ unsigned long long best;
     __shared__ int ncand;
+    __shared__ float score[kOrient];
+    __shared__ unsigned char cand[kOrient];
     const int top_o = cfg[0];
     const int o = threadIdx.x;
     const int no = orientations < kOrient ? orientations : kOrient;
@@ -792,22 +799,27 @@ __global__ __launch_bounds__(128) void k_lazy_select(const float* __restrict__ l
     }
     __syncthreads();
     const bool fits = o < no && descs[o].lx > 0;
-    if (fits && valid[o]) atomicMin(&best, key[o]);
+    const bool cur = fits && valid[o] != 0;
+    const unsigned long long ko = o < no ? key[o] : ~0ULL;
+    if (cur) atomicMin(&best, ko);
     __syncthreads();
-    const unsigned long long b = best;
-    auto candidate = [&](int q) { return q < no && descs[q].lx > 0 && !valid[q] && key[q] < b; };
-    if (candidate(o)) {
-        atomicAdd(&ncand, 1);
-        const float so = logits[o] + prior[o];
+    const bool is_cand = fits && !cur && ko < best;
+    if (o < kOrient) {
+        cand[o] = is_cand ? 1 : 0;
+        score[o] = o < no ? logits[o] + prior[o] : 0.f;
+    }
+    if (is_cand) atomicAdd(&ncand, 1);
+    __syncthreads();
+    if (is_cand) {
+        const float so = score[o];
         int rank = 0;
         for (int q = 0; q < no; ++q) {
-            if (q == o || !candidate(q)) continue;
-            const float sq = logits[q] + prior[q];
+            if (q == o || !cand[q]) continue;
+            const float sq = score[q];
             rank += (sq > so || (sq == so && q < o)) ? 1 : 0;
         }
         if (rank < top_o) act[kActHeader + rank] = o;
     }
-    __syncthreads();
     if (o == 0) {
         act[0] = ncand < top_o ? ncand : top_o;
         act[1] = 0;
@@ -1112,7 +1124,7 @@ public:
             SP_HIP_CHECK(hipMemcpyAsync(cfg_.get(), cfg_host_, sizeof cfg_host_, hipMemcpyHostToDevice, v.stream));
         }
         auto propose = [&, v](hipStream_t st) {
-            hipLaunchKernelGGL(k_lazy_check, dim3(v.orientations), dim3(256), 0, st, v.env, v.ext[0], v.ext[1],
+            hipLaunchKernelGGL(k_lazy_check, dim3(v.orientations, kLazySlices), dim3(256), 0, st, v.env, v.ext[0], v.ext[1],
                                v.ext[2], v.kcells, v.descs, lazy_key_.as<unsigned long long>(),
                                lazy_valid_.as<unsigned>());
             hipLaunchKernelGGL(k_downsample, dim3(kV0 / 256), dim3(256), 0, st, v.env, v.ext[0], v.ext[1], v.ext[2],

@@ -77,15 +77,21 @@ __global__ void k_verify_choice(const unsigned char* __restrict__ env, int ex, i
     const int o = int(key & 0xffffULL);
     const int ux = int((key >> 16) & 0xffffULL), uy = int((key >> 32) & 0xffffULL), uz = int(key >> 48);
     const OrientDesc d = descs[o];
-    const long long n = (long long)d.kx * d.ky * d.kz;
+    // each wave takes kernel rows (z, y) in turn and its lanes the row's x
+    // cells (no per-cell index division)
+    const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    const int rows = d.ky * d.kz;
     unsigned int hit = 0;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        if (!kcells[d.cells_offset + i]) continue;
-        const int gx = ux + int(i % d.kx), gy = uy + int((i / d.kx) % d.ky), gz = uz + int(i / ((long long)d.kx * d.ky));
+    for (int rw = blockIdx.x * waves + (threadIdx.x >> 6); rw < rows; rw += waves * gridDim.x) {
+        const int z = rw / d.ky, y = rw - z * d.ky;
+        const unsigned char* krow = kcells + d.cells_offset + (long long)rw * d.kx;
         // a corner outside the window, or a kernel cell beyond it, is an overlap
-        if (gx >= ex || gy >= ey || gz >= ez || env[gx + (long long)ex * (gy + (long long)ey * gz)]) hit = 1;
+        const bool inside = uz + z < ez && uy + y < ey;
+        const long long erow = (long long)ex * ((uy + y) + (long long)ey * (uz + z)) + ux;
+        for (int x = lane; x < d.kx; x += 64)
+            if (krow[x] && (!inside || ux + x >= ex || env[erow + x])) hit = 1;
     }
-    if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(&r->flags, 4u);
+    if (__any(hit) && lane == 0) atomicOr(&r->flags, 4u);
 }
 
 // Placed copies (conservative_fields.cpp:1666-1887, BlockedField): every
