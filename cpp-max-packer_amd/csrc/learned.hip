@@ -1105,11 +1105,12 @@ public:
             hipLaunchKernelGGL(k_lazy_init, dim3(1), dim3(128), 0, v.stream, v.orient_best, v.orientations,
                                lazy_key_.as<unsigned long long>(), lazy_valid_.as<unsigned>());
             session_read_result(s, out);
-            lazy_session_ = s;
+            lazy_serial_ = v.serial;
+            lazy_rev_ = v.reset_rev;
             used_full = 1;
             searched = std::uint32_t(v.fitting);
         };
-        if (lazy_session_ != s) {
+        if (lazy_serial_ != v.serial || lazy_rev_ != v.reset_rev) {
             full();
             return;
         }
@@ -1148,7 +1149,7 @@ public:
         if (out.orientation < 0) full();
     }
 
-    void lazy_reset() { lazy_session_ = nullptr; }
+    void lazy_reset() { lazy_serial_ = 0; }
 
     void mfma_probe(const std::uint16_t* a, const std::uint16_t* b, float* c)
     {
@@ -1270,7 +1271,10 @@ private:
     DeviceBuffer slots_, desc_, film_, a1_, a2_, a3_, a4_, heat_, logits_, usave_, hsave_, msave_, pool_;
     DeviceBuffer infer_x_, infer_slot_, infer_desc_, cfg_, prior_;
     DeviceBuffer lazy_key_, lazy_valid_;   // per orientation: last minimum, and whether it is current
-    const sp_hip_session* lazy_session_ {}; // the session lazy_key_ belongs to (null: none)
+    // the session lazy_key_ belongs to (serial 0: none) and its feasibility
+    // revision then: a step on another session, or after any change other
+    // than stamping, searches every orientation again
+    std::uint64_t lazy_serial_ {}, lazy_rev_ {};
     DeviceBuffer dz4_, dz3_, dz2_, dz1_, partial_, sums_[4], head_partial_, target_, omask_, region_, losses_;
     DeviceBuffer bank_;
     std::uint32_t bank_count_ {};

@@ -769,55 +769,68 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
                 return;
             }
             const int y1 = row0 + 2 * g;
+            // wave w holds rows row0 + 8 w ... row0 + 8 w + 7; a wave none of
+            // whose rows is legal (y >= ly) transforms and checks nothing and
+            // takes only the transform's block barrier (uniform per wave)
+            const bool live = row0 + 8 * __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6) < d.ly;
             T2 v[R];
-            // line 0 holds (kz0 + i kz128) of both rows: k = 0 takes the real
-            // parts and k = 128 the imaginary parts (thread 0, q = 0 and 8)
+            if (live) {
+                // line 0 holds (kz0 + i kz128) of both rows: k = 0 takes the
+                // real parts and k = 128 the imaginary parts (thread 0, q = 0
+                // and 8)
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
+                for (int h = 0; h < 4; ++h) {
 #pragma unroll
-                for (int q = 4 * h; q < 4 * h + 4; ++q) {
-                    const int k = t + P * q;
-                    const bool mirror = k > N / 2;
-                    const int kk = (k == N / 2) ? 0 : mirror ? N - k : k;
-                    const float4 ab = *reinterpret_cast<const float4*>(&stage[kk * kYzStageRow + 2 * g]);
-                    const T2 u { ab.x, ab.y }, w { ab.z, ab.w };
-                    T2 e = mirror ? T2 { u.x + w.y, w.x - u.y } : T2 { u.x - w.y, u.y + w.x };
-                    if (q == 0 && t == 0) e = T2 { u.x, w.x };
-                    if (q == 8 && t == 0) e = T2 { u.y, w.y };
-                    v[q] = e;
+                    for (int q = 4 * h; q < 4 * h + 4; ++q) {
+                        const int k = t + P * q;
+                        const bool mirror = k > N / 2;
+                        const int kk = (k == N / 2) ? 0 : mirror ? N - k : k;
+                        const float4 ab = *reinterpret_cast<const float4*>(&stage[kk * kYzStageRow + 2 * g]);
+                        const T2 u { ab.x, ab.y }, w { ab.z, ab.w };
+                        T2 e = mirror ? T2 { u.x + w.y, w.x - u.y } : T2 { u.x - w.y, u.y + w.x };
+                        if (q == 0 && t == 0) e = T2 { u.x, w.x };
+                        if (q == 8 && t == 0) e = T2 { u.y, w.y };
+                        v[q] = e;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                __builtin_amdgcn_sched_barrier(0);
+                dft_reg<T, R>(v, false);
             }
-            reg_fft_lean<T, N, R, true, NoWork, kYzRow>(v, pen, tws, t, false);
-            // checks over every value of the column's circular correlation,
-            // kept as running extremes.  Every circular value, legal or not,
-            // is in exact arithmetic an integer in [0, min(masses)] and lies
-            // under the same a-priori bound, so the checks need no legality
-            // test per value (the legal values alone are the reference's;
-            // checking all of them is stricter).
-            auto check = [&](int s, T& ra, T& rb) {
-                const T va = v[s].x, vb = v[s].y;
-                ra = rint(va);
-                rb = rint(vb);
-                r1 = __builtin_elementwise_maximum(r1, __builtin_elementwise_maximum(fabsf(va - ra), fabsf(vb - rb)));
-                lo = fminf(lo, fminf(ra, rb));
-                col_hi = fmaxf(col_hi, fmaxf(ra, rb));
-            };
-            // Zero counts are marked in bit 2 s + half, which orders them as
-            // their keys (z, then y); the bits of values outside the legal
-            // range are cleared with the lane's legal-bit mask.
+            // the exchange regions alias the staged lines other waves read
+            __syncthreads();
             unsigned int zeros = 0;
+            if (live) {
+                reg_fft_lean_second<T, N, R, NoWork, kYzRow>(v, pen, tws, t, false);
+                // checks over every value of the wave's rows, kept as running
+                // extremes.  Every circular value, legal or not, is in exact
+                // arithmetic an integer in [0, min(masses)] and lies under the
+                // same a-priori bound, so the checks need no legality test per
+                // value (the legal values alone are the reference's; checking
+                // more of them is stricter).
+                auto check = [&](int s, T& ra, T& rb) {
+                    const T va = v[s].x, vb = v[s].y;
+                    ra = rint(va);
+                    rb = rint(vb);
+                    r1 = __builtin_elementwise_maximum(r1,
+                                                       __builtin_elementwise_maximum(fabsf(va - ra), fabsf(vb - rb)));
+                    lo = fminf(lo, fminf(ra, rb));
+                    col_hi = fmaxf(col_hi, fmaxf(ra, rb));
+                };
+                // Zero counts are marked in bit 2 s + half, which orders them
+                // as their keys (z, then y); the bits of values outside the
+                // legal range are cleared with the lane's legal-bit mask.
 #pragma unroll
-            for (int s = 0; s < R; ++s) {
-                T ra, rb;
-                check(s, ra, rb);
-                zeros |= (ra == T(0) ? 1u << (2 * s) : 0u) | (rb == T(0) ? 2u << (2 * s) : 0u);
+                for (int s = 0; s < R; ++s) {
+                    T ra, rb;
+                    check(s, ra, rb);
+                    zeros |= (ra == T(0) ? 1u << (2 * s) : 0u) | (rb == T(0) ? 2u << (2 * s) : 0u);
+                }
+                // legal bits: 2 s + half with t + 16 s < lz and y1 + half < ly
+                const int ns = min(max((d.lz - t + P - 1) / P, 0), R);
+                const unsigned int zmask = ns >= R ? ~0u : (1u << (2 * ns)) - 1u;
+                const unsigned int ymask = (y1 < d.ly ? 0x55555555u : 0u) | (y1 + 1 < d.ly ? 0xAAAAAAAAu : 0u);
+                zeros &= zmask & ymask;
             }
-            // legal bits: 2 s + half with t + 16 s < lz and y1 + half < ly
-            const int ns = min(max((d.lz - t + P - 1) / P, 0), R);
-            const unsigned int zmask = ns >= R ? ~0u : (1u << (2 * ns)) - 1u;
-            const unsigned int ymask = (y1 < d.ly ? 0x55555555u : 0u) | (y1 + 1 < d.ly ? 0xAAAAAAAAu : 0u);
-            zeros &= zmask & ymask;
             if (PARK) parked[threadIdx.x] = zeros;
             else zero_last = zeros;
         };

@@ -28,6 +28,8 @@ struct SessionView {
     unsigned long long* orient_best;  // per orientation, the last step's lowest feasible key (~0: none)
     StepDeviceResult* result;         // the step's result (key read by the recheck and the stamp)
     hipStream_t stream;
+    std::uint64_t serial;     // unique per session in the process
+    std::uint64_t reset_rev;  // changes when feasibility changes other than by stamping
 };
 
 // Prepares the session if needed and returns its device view.

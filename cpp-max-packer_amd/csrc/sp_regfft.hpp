@@ -164,39 +164,23 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
     }
 }
 
-// reg_fft for N = R * R (one exchange) with the exchange twiddles taken from
-// a [q][t] table, tq[q * R + t] = w^(t q) in the transform's direction, so
-// that every twiddle read is one base address plus a constant, and with the
-// exchange reads and their twiddles issued four at a time (a kernel holding
-// other data in registers keeps the transform's working set small).  Same
-// arithmetic as reg_fft.
-// The pencil's 16 threads are lanes of one wave (t = lane % 16 within a group
-// of 16 consecutive lanes), and LDS instructions of a wave complete in order,
-// so the exchange itself needs no block barrier.  SYNC: the pencil's region
-// may still be read by other waves (it aliases a buffer they use), so the
-// block synchronises before the first write.
 struct NoWork {
     __device__ void operator()() const {}
 };
 
-// `mid` runs between the exchange's writes and its reads, when the pencil's
-// registers hold nothing: a caller can issue loads for later work there.
-// ROW: LDS values per exchange row (17: one pad value; 18 in FP32: rows
-// start on 16-byte boundaries and each thread writes its row as eight
-// 16-byte stores, the region itself being 16-byte aligned).
-template <typename T, int N, int R, bool SYNC = true, typename Mid = NoWork, int ROW = R + 1>
-__device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd, Mid mid = {})
+// reg_fft_lean (below) after its first radix-R pass and its barrier: the
+// exchange and the second pass.  A caller that splits the transform around a
+// block barrier, some waves taking only the barrier, runs dft_reg, the
+// barrier, then this.
+template <typename T, int N, int R, typename Mid = NoWork, int ROW = R + 1>
+__device__ __forceinline__ void reg_fft_lean_second(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd,
+                                                    Mid mid = {})
 {
     using S = RegShape<N, R>;
     static_assert(S::passes == 2 && R == 16, "one exchange of 16 x 16");
     static_assert(ROW == R + 1 || (ROW == R + 2 && sizeof(T) == 4), "row of 17 values, or of 18 FP32 values");
-    // a fresh copy of t per call: the twiddles are then read again here
-    // rather than kept in registers from an earlier call
     asm volatile("" : "+v"(t));
     __builtin_assume(t >= 0 && t < R);
-    dft_reg<T, R>(v, fwd);
-    if (SYNC) __syncthreads();
-    else __builtin_amdgcn_wave_barrier();
     // row t holds X1[16 t + s]; the second pass reads column t (l = R, m = 1,
     // j = t, k = 0)
     if constexpr (ROW == R + 2) {
@@ -220,6 +204,35 @@ __device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* 
         __builtin_amdgcn_sched_barrier(0);
     }
     dft_reg<T, R>(v, fwd);
+}
+
+// reg_fft for N = R * R (one exchange) with the exchange twiddles taken from
+// a [q][t] table, tq[q * R + t] = w^(t q) in the transform's direction, so
+// that every twiddle read is one base address plus a constant, and with the
+// exchange reads and their twiddles issued four at a time (a kernel holding
+// other data in registers keeps the transform's working set small).  Same
+// arithmetic as reg_fft.
+// The pencil's 16 threads are lanes of one wave (t = lane % 16 within a group
+// of 16 consecutive lanes), and LDS instructions of a wave complete in order,
+// so the exchange itself needs no block barrier.  SYNC: the pencil's region
+// may still be read by other waves (it aliases a buffer they use), so the
+// block synchronises before the first write.
+// `mid` runs between the exchange's writes and its reads, when the pencil's
+// registers hold nothing: a caller can issue loads for later work there.
+// ROW: LDS values per exchange row (17: one pad value; 18 in FP32: rows
+// start on 16-byte boundaries and each thread writes its row as eight
+// 16-byte stores, the region itself being 16-byte aligned).
+template <typename T, int N, int R, bool SYNC = true, typename Mid = NoWork, int ROW = R + 1>
+__device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd, Mid mid = {})
+{
+    // a fresh copy of t per call: the twiddles are then read again here
+    // rather than kept in registers from an earlier call
+    asm volatile("" : "+v"(t));
+    __builtin_assume(t >= 0 && t < R);
+    dft_reg<T, R>(v, fwd);
+    if (SYNC) __syncthreads();
+    else __builtin_amdgcn_wave_barrier();
+    reg_fft_lean_second<T, N, R, Mid, ROW>(v, lds, tq, t, fwd, mid);
 }
 
 }  // namespace sphip

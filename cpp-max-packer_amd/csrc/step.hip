@@ -6,6 +6,7 @@
 // step runs on one stream; with capture_graph the sequence is recorded once as
 // a hipGraph and replayed per step.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -635,6 +636,7 @@ public:
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
         inc_ready_ = pending_ = false;
         ++env_rev_;
+        ++reset_rev_;
     }
 
     void get_environment(std::uint8_t* cells, std::uint64_t count)
@@ -757,6 +759,7 @@ public:
         if (it != excluded_.end() && *it == key) return;
         if (excluded_.size() >= kMaxExcluded) throw std::length_error("too many rejected translations");
         excluded_.insert(it, key);
+        ++reset_rev_;
         std::vector<unsigned long long> buf(1 + excluded_.size());
         buf[0] = excluded_.size();
         std::copy(excluded_.begin(), excluded_.end(), buf.begin() + 1);
@@ -910,6 +913,7 @@ public:
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
         removed_.insert(id);
         ++env_rev_;
+        ++reset_rev_;
         if (fault) throw std::domain_error("FIELD_COUNT_UNDERFLOW");
         if (ur.flags) throw std::domain_error("CORRELATION_NUMERIC");
     }
@@ -1247,6 +1251,7 @@ public:
                 tile_ms_[p] = 0.f;
             }
         prepared_ = true;
+        ++reset_rev_;
     }
 
     // Optional per-kernel timing: when prof_ is set, an event is recorded after
@@ -1860,6 +1865,8 @@ public:
         v.orient_best = orient_best_.as<unsigned long long>();
         v.result = result_.as<StepDeviceResult>();
         v.stream = stream_;
+        v.serial = serial_;
+        v.reset_rev = reset_rev_;
         return v;
     }
 
@@ -2430,6 +2437,17 @@ private:
     std::vector<const char*> prof_names_;
     // occupancy revision (set_environment and every stamp) and ranking state
     std::uint64_t env_rev_ {}, rank_rev_ { ~0ull };
+    // the session's process-wide serial, and the revision of every change
+    // other than a stamp that alters which positions are feasible (a new
+    // environment or kernel, an excluded translation, a removed copy, a new
+    // clearance): between two such changes occupancy only grows
+    const std::uint64_t serial_ { next_serial() };
+    std::uint64_t reset_rev_ {};
+    static std::uint64_t next_serial()
+    {
+        static std::atomic<std::uint64_t> n { 0 };
+        return ++n;
+    }
     bool real_env_ {}, rank_on_ {}, fixed_radix_ {};
     DeviceBuffer envd_;
     sp_ranking_desc rank_desc_ {};

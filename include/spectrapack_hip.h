@@ -507,16 +507,17 @@ int sp_hip_proposal_step(sp_hip_proposal* p, sp_hip_session* s, const float desc
  * plus the prior term; ties: lower index), the top_orientations of them are
  * searched exactly (every region), and the step places the lowest current
  * minimum: the plain search's choice whenever no more than
- * top_orientations orientations needed a search.  The first lazy step on a
- * session (after sp_hip_proposal_lazy_reset or on another session) and a
- * step whose known minima hold nothing feasible run the full search
- * (*used_full = 1), which makes every minimum current.  *searched: the
- * orientations searched (nullable outputs). */
+ * top_orientations orientations needed a search.  The full search runs
+ * instead (*used_full = 1), making every minimum current, on the first lazy
+ * step on a session, after sp_hip_proposal_lazy_reset, after any change to
+ * the session other than stamping that alters which positions are feasible
+ * (sp_hip_session_set_environment, a kernel, an excluded translation, a
+ * removed copy, the pair clearance), and when the known minima hold nothing
+ * feasible.  *searched: the orientations searched (nullable outputs). */
 int sp_hip_proposal_lazy_step(sp_hip_proposal* p, sp_hip_session* s, const float descriptor[32],
                               uint32_t top_orientations, int stamp, sp_step_result* out, int* used_full,
                               uint32_t* searched);
-/* Forgets the kept minima: the next lazy step runs the full search.  Call it
- * whenever the session's occupancy is set or cells are freed. */
+/* Forgets the kept minima: the next lazy step runs the full search. */
 int sp_hip_proposal_lazy_reset(sp_hip_proposal* p);
 /* Self-test of the MFMA tile layout: c[32][32] = a[32][16] * b[16][32], BF16
  * bit patterns in, FP32 out. */

@@ -348,3 +348,51 @@ def test_lazy_step_against_its_restatement(ctx, oracle, proposal):
     assert partial > 0  # some steps searched only part of the candidates
     proposal.set_prior([], 0.0)
     proposal.lazy_reset()
+
+
+@pytest.mark.gpu
+def test_lazy_step_follows_session_changes(ctx, oracle, proposal):
+    """The kept minima hold only while occupancy grows by stamping.  An
+    excluded translation, a new environment and another session each make
+    the next lazy step run the full search, whose choice is the plain step's
+    (an excluded corner is never placed again; a cleared environment gives
+    back the first step's corner)."""
+    first, env, kernels = _box_scene(oracle)
+    s = _session(ctx, env, first, kernels)
+    desc = np.random.default_rng(11).normal(size=32).astype(np.float32)
+    proposal.set_parameters(params(11))
+    proposal.reserve(1)
+    proposal.lazy_reset()
+    pose = lambda r: (r["orientation"], tuple(r["translation"]))
+
+    r0 = proposal.lazy_step(s, desc, 3, stamp=True)
+    assert r0["used_full"] and r0["orientation"] >= 0
+    for _ in range(3):
+        r = proposal.lazy_step(s, desc, 3, stamp=True)
+        assert r["orientation"] >= 0
+    r = proposal.lazy_step(s, desc, 3, stamp=False)
+    assert not r["used_full"]
+    plain = s.step(stamp=False)
+    assert pose(r) == pose(plain)
+
+    s.exclude(plain["orientation"], plain["translation"])
+    r = proposal.lazy_step(s, desc, 3, stamp=False)
+    assert r["used_full"]
+    assert pose(r) != pose(plain)
+    assert pose(r) == pose(s.step(stamp=False))
+
+    s.set_environment(env)
+    r = proposal.lazy_step(s, desc, 3, stamp=False)
+    assert r["used_full"]
+    assert pose(r) == pose(r0)
+
+    s2 = _session(ctx, env, first, kernels)
+    s2.stamp(r0["orientation"], r0["translation"])
+    r = proposal.lazy_step(s2, desc, 3, stamp=False)
+    assert r["used_full"]
+    assert pose(r) == pose(s2.step(stamp=False))
+    r = proposal.lazy_step(s, desc, 3, stamp=False)
+    assert r["used_full"] and pose(r) == pose(r0)
+    s2.close()
+    s.close()
+    proposal.lazy_reset()
