@@ -225,19 +225,37 @@ __device__ __forceinline__ void xfft256(cx<T>* v, cx<T>* xl, const cx<T>* tq, in
     dft16<T>(v, fwd);
 }
 
-// The kernel's x transform of one tile row (kx <= 48: the first pass leaves
-// the zero groups out).  hcol = H[o][kz][0][y] (x stride ny).
-template <typename T, int W>
-__device__ __forceinline__ void xt_kernel_row(cx<T>* v, const cx<T>* __restrict__ hcol, int ny, int kx, bool live,
-                                              cx<T>* xl, const cx<T>* tqf, int t)
+// Values per lane of a kernel row that rf_mul_xt loads one orientation
+// ahead: x = t + 16 q for q < kXtAhead, every value of a row with kx <= 48.
+constexpr int kXtAhead = 3;
+
+// The values x = t + 16 q, q >= kXtAhead, of a kernel row (zero past kx;
+// none are loaded for kx <= 48).  hcol = H[o][kz][0][y] (x stride ny).
+template <typename T>
+__device__ __forceinline__ void xt_kernel_rest(cx<T>* v, const cx<T>* __restrict__ hcol, int ny, int kx, bool live,
+                                               int t)
 {
     using T2 = cx<T>;
     constexpr int R = 16, P = 16;
+    if (kx > kXtAhead * P) {  // uniform over the block
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int x = t + P * q;
-        v[q] = (live && x < kx) ? nt_load(&hcol[(long long)x * ny]) : T2 { 0, 0 };
+        for (int q = kXtAhead; q < R; ++q) {
+            const int x = t + P * q;
+            v[q] = (live && x < kx) ? nt_load(&hcol[(long long)x * ny]) : T2 { 0, 0 };
+        }
     }
+    else {
+#pragma unroll
+        for (int q = kXtAhead; q < R; ++q) v[q] = T2 { 0, 0 };
+    }
+}
+
+// The kernel's x transform of one tile row (kx <= 48: the first pass leaves
+// the zero groups out); v holds the row, zero past kx.
+template <typename T, int W>
+__device__ __forceinline__ void xt_kernel_fft(cx<T>* v, int kx, cx<T>* xl, const cx<T>* tqf, int t)
+{
+    constexpr int P = 16;
     const int lead = (kx + P - 1) / P;  // uniform over the block
     if (lead == 1) xfft256<T, W, 1>(v, xl, tqf, t, true);
     else if (lead == 2) xfft256<T, W, 2>(v, xl, tqf, t, true);
@@ -287,32 +305,67 @@ rf_mul_xt(const cx<T>* __restrict__ H, long long h_o, int h_lines, const cx<T>* 
         const int kz = 1 + tile / ytiles, y = (tile % ytiles) * W + w;
         const bool live = y < ny;
         const long long row = (long long)kz * ny + y;
+        // byte offset of (t, y) within an x plane group of the output (below
+        // 2^31: at most 15 (nh - 1) ny + ny values)
+        const unsigned int lane_off = unsigned(y + t * planes * ny) * unsigned(sizeof(T2));
         T2 sb[R];
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             const T2 e = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
             sb[s] = T2 { e.x * sb_scale, e.y * sb_scale };
         }
+        // The next orientation's descriptor and the first values of its
+        // kernel row are loaded while the current one is transformed.  The
+        // loads are issued every iteration from clamped addresses (values
+        // past kx are replaced by zero when taken), and taken after the
+        // inverse transform, before the stores: the wait for them then falls
+        // where they are the only memory operations in flight.
+        int on = 0, kxn = 0, lxn = 0;
+        T2 ahead[kXtAhead], first[kXtAhead];
+        const int yc = live ? y : ny - 1;
+        auto fetch = [&](int i) {
+            on = act[kActHeader + i];
+            kxn = descs[on].kx;
+            lxn = descs[on].lx;
+            const int last = min(max(kxn, 1), h_lines) - 1;
+            const T2* hcol = H + on * h_o + (long long)kz * h_lines * ny + yc;
+#pragma unroll
+            for (int q = 0; q < kXtAhead; ++q) ahead[q] = nt_load(&hcol[(long long)min(t + P * q, last) * ny]);
+        };
+        auto take = [&] {
+#pragma unroll
+            for (int q = 0; q < kXtAhead; ++q) first[q] = (live && t + P * q < kxn) ? ahead[q] : T2 { 0, 0 };
+        };
+        fetch(i0);
+        take();
         for (int i = i0; i < i1; ++i) {
-            const int o = act[kActHeader + i];
-            const int kx = descs[o].kx, lx = descs[o].lx;
-            if (lx <= 0) continue;  // uniform across the block
+            const int o = on, kx = kxn, lx = lxn;
             T2 v[R];
-            xt_kernel_row<T, W>(v, H + o * h_o + (long long)kz * h_lines * ny + y, ny, kx, live, xl, tqf, t);
+#pragma unroll
+            for (int q = 0; q < kXtAhead; ++q) v[q] = first[q];
+            xt_kernel_rest<T>(v, H + o * h_o + (long long)kz * h_lines * ny + y, ny, kx, live, t);
+            fetch(min(i + 1, i1 - 1));
+            if (lx <= 0) {  // uniform across the block
+                take();
+                continue;
+            }
+            xt_kernel_fft<T, W>(v, kx, xl, tqf, t);
 #pragma unroll
             for (int s = 0; s < R; ++s) v[s] = cmulc(sb[s], v[s]);
             xfft256<T, W, 0>(v, xl, tqi, t, false);
+            take();
             if (live) {
-                T2* ocol = out + o * out_o + row;
-                // a fresh copy of the column stride per orientation: the store
-                // addresses are then formed here instead of being kept in
-                // registers across the loop
-                long long xs = (long long)planes * ny;
-                asm volatile("" : "+s"(xs));
+                // value x = t + 16 s at out[o][x][kz][y]: a base uniform over
+                // the block for each s (x = 16 s) plus this lane's byte offset
+                // of (t, y), so that no store address is formed in vector
+                // registers
+                const long long xs = (long long)planes * ny;
+                T2* const base = out + o * out_o + (long long)kz * ny;
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const int x = t + P * s;
-                    if (x < lx) nt_store(&ocol[x * xs], v[s]);
+                    char* const bs = reinterpret_cast<char*>(base + (long long)(P * s) * xs);
+                    if (x < lx) nt_store(reinterpret_cast<T2*>(bs + lane_off), v[s]);
                 }
             }
         }
@@ -350,7 +403,14 @@ rf_mul_xt_plane0(const cx<T>* __restrict__ H, long long h_o, int h_lines, const 
         const int kz = h == 0 ? 0 : nh - 1;
         const long long row = (long long)kz * ny + y;
         T2 v[R];
-        xt_kernel_row<T, W>(v, H + o * h_o + (long long)kz * h_lines * ny + y, ny, kx, live, xl, tqf, t);
+        const T2* hcol = H + o * h_o + (long long)kz * h_lines * ny + y;
+#pragma unroll
+        for (int q = 0; q < kXtAhead; ++q) {
+            const int x = t + P * q;
+            v[q] = (live && x < kx) ? nt_load(&hcol[(long long)x * ny]) : T2 { 0, 0 };
+        }
+        xt_kernel_rest<T>(v, hcol, ny, kx, live, t);
+        xt_kernel_fft<T, W>(v, kx, xl, tqf, t);
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             const T2 e = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
