@@ -788,11 +788,11 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
 #pragma unroll
                 for (int q = 0; q < R; ++q) b[q] = nt_load(&p1[P * q]);
             };
-            reg_fft_lean<T, N, R, false, decltype(load_b), kYzRow>(a, pen, tws, t, false, load_b);
+            reg_fft_lean<T, N, R, false, decltype(load_b), kYzRow, false>(a, pen, tws, t, false, load_b);
             // (the two transforms are not interleaved by the scheduler: their
             // working sets together do not fit the registers)
             __builtin_amdgcn_sched_barrier(0);
-            reg_fft_lean<T, N, R, false, NoWork, kYzRow>(b, pen, tws, t, false);
+            reg_fft_lean<T, N, R, false, NoWork, kYzRow, false>(b, pen, tws, t, false);
         }
         int o2 = o, x2 = x;
         const int c2 = next_col(c + int(gridDim.x), o2, x2);
@@ -854,13 +854,13 @@ __global__ void __launch_bounds__(kYzThreads) rf_yz_argmin(const cx<T>* __restri
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 }
-                dft_reg<T, R>(v, false);
+                dft_reg<T, R, false>(v, false);
             }
             // the exchange regions alias the staged lines other waves read
             __syncthreads();
             unsigned int zeros = 0;
             if (live) {
-                reg_fft_lean_second<T, N, R, NoWork, kYzRow>(v, pen, tws, t, false);
+                reg_fft_lean_second<T, N, R, NoWork, kYzRow, false>(v, pen, tws, t, false);
                 // checks over every value of the wave's rows, kept as running
                 // extremes.  Every circular value, legal or not, is in exact
                 // arithmetic an integer in [0, min(masses)] and lies under the

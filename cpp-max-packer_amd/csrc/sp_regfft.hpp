@@ -14,8 +14,77 @@
 
 namespace sphip {
 
-// Radix-16 in registers as 4 x 4 with the w16 twiddles in between.
+// A + W B and A - W B for W = c + i s written as c (1 + i tau), tau = s / c:
+// u = B (1 + i tau) takes two fused multiply-adds, and each output two more,
+// six operations where a complex product and two complex additions take
+// eight.
+template <typename T>
+__device__ __forceinline__ void bfly_tan(cx<T> A, cx<T> B, T c, T tau, cx<T>& plus, cx<T>& minus)
+{
+    const T ux = fma(-tau, B.y, B.x), uy = fma(tau, B.x, B.y);
+    plus = cx<T> { fma(c, ux, A.x), fma(c, uy, A.y) };
+    minus = cx<T> { fma(-c, ux, A.x), fma(-c, uy, A.y) };
+}
+
+// Radix-16 in registers as 4 x 4.  The first radix-4 stage runs on the four
+// columns n2; the w16^(n2 k1) twiddles of the second are folded into its
+// butterflies: with y_n the column k1 inputs and W = w16^k1, the radix-4
+// DFT of (y0, W y1, W^2 y2, W^3 y3) is
+//   b = y0 +- W^2 y2, c = y1 +- W^2 y3,
+//   X0, X2 = b+ +- W c+,  X1, X3 = b- +- (w4 W) c-,
+// each "+-" pair one bfly_tan (w4 = -+i, the radix-4 root).
 template <typename T> __device__ __forceinline__ void dft16(cx<T>* v, bool fwd)
+{
+    using T2 = cx<T>;
+    // cos and tan of m pi / 8; the sine carries the direction's sign sg
+    const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173), h = T(0.70710678118654752440);
+    const T t1 = T(0.41421356237309504880), t3 = T(2.41421356237309504880);
+    const T sg = fwd ? T(-1) : T(1);
+    T2 y[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        T2 a[4] = { v[n2], v[4 + n2], v[8 + n2], v[12 + n2] };
+        Dft<T, 4>::run(a, fwd);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) y[n2 * 4 + k1] = a[k1];
+    }
+    {
+        T2 a[4] = { y[0], y[4], y[8], y[12] };
+        Dft<T, 4>::run(a, fwd);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[4 * k2] = a[k2];
+    }
+    // (c, tan) of w16^m = cos(m pi / 8) + i sg sin(m pi / 8)
+    auto col = [&](int k1, T c2, T tau2, T c1_, T tau1, T cv, T tauv) {
+        const T2 y0 = y[k1], y1 = y[4 + k1], y2 = y[8 + k1], y3 = y[12 + k1];
+        T2 bp, bm, cp, cm;
+        if (k1 == 2) {  // W^2 = w16^4 = w4
+            const T2 r2 = mul_mi(y2, fwd), r3 = mul_mi(y3, fwd);
+            bp = cadd(y0, r2);
+            bm = csub(y0, r2);
+            cp = cadd(y1, r3);
+            cm = csub(y1, r3);
+        }
+        else {
+            bfly_tan(y0, y2, c2, tau2, bp, bm);
+            bfly_tan(y1, y3, c2, tau2, cp, cm);
+        }
+        bfly_tan(bp, cp, c1_, tau1, v[k1], v[k1 + 8]);
+        bfly_tan(bm, cm, cv, tauv, v[k1 + 4], v[k1 + 12]);
+    };
+    // k1 = 1: W^2 = w16^2, W = w16^1, w4 W = w16^5
+    col(1, h, sg, c1, sg * t1, -s1, -sg * t3);
+    // k1 = 2: W = w16^2, w4 W = w16^6
+    col(2, T(0), T(0), h, sg, -h, -sg);
+    // k1 = 3: W^2 = w16^6, W = w16^3, w4 W = w16^7
+    col(3, -h, -sg, s1, sg * t3, -c1, -sg * t1);
+}
+
+// Radix-16 in registers as 4 x 4 with the w16 twiddles applied between the
+// two radix-4 stages (a complex product each): fewer values live at once
+// than in dft16's folded butterflies, for kernels at the edge of their
+// register budget.
+template <typename T> __device__ __forceinline__ void dft16_twiddled(cx<T>* v, bool fwd)
 {
     using T2 = cx<T>;
     const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173), h = T(0.70710678118654752440);
@@ -94,9 +163,13 @@ template <typename T, int NQ> __device__ __forceinline__ void dft16_lead(cx<T>* 
     }
 }
 
-template <typename T, int R> __device__ __forceinline__ void dft_reg(cx<T>* v, bool fwd)
+// FOLD (radix 16): dft16's folded butterflies, else dft16_twiddled.
+template <typename T, int R, bool FOLD = true> __device__ __forceinline__ void dft_reg(cx<T>* v, bool fwd)
 {
-    if constexpr (R == 16) dft16<T>(v, fwd);
+    if constexpr (R == 16) {
+        if constexpr (FOLD) dft16<T>(v, fwd);
+        else dft16_twiddled<T>(v, fwd);
+    }
     else Dft<T, R>::run(v, fwd);
 }
 
@@ -172,7 +245,7 @@ struct NoWork {
 // exchange and the second pass.  A caller that splits the transform around a
 // block barrier, some waves taking only the barrier, runs dft_reg, the
 // barrier, then this.
-template <typename T, int N, int R, typename Mid = NoWork, int ROW = R + 1>
+template <typename T, int N, int R, typename Mid = NoWork, int ROW = R + 1, bool FOLD = true>
 __device__ __forceinline__ void reg_fft_lean_second(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd,
                                                     Mid mid = {})
 {
@@ -203,7 +276,7 @@ __device__ __forceinline__ void reg_fft_lean_second(cx<T>* v, cx<T>* lds, const 
         }
         __builtin_amdgcn_sched_barrier(0);
     }
-    dft_reg<T, R>(v, fwd);
+    dft_reg<T, R, FOLD>(v, fwd);
 }
 
 // reg_fft for N = R * R (one exchange) with the exchange twiddles taken from
@@ -221,18 +294,19 @@ __device__ __forceinline__ void reg_fft_lean_second(cx<T>* v, cx<T>* lds, const 
 // registers hold nothing: a caller can issue loads for later work there.
 // ROW: LDS values per exchange row (17: one pad value; 18 in FP32: rows
 // start on 16-byte boundaries and each thread writes its row as eight
-// 16-byte stores, the region itself being 16-byte aligned).
-template <typename T, int N, int R, bool SYNC = true, typename Mid = NoWork, int ROW = R + 1>
+// 16-byte stores, the region itself being 16-byte aligned).  FOLD: as in
+// dft_reg.
+template <typename T, int N, int R, bool SYNC = true, typename Mid = NoWork, int ROW = R + 1, bool FOLD = true>
 __device__ __forceinline__ void reg_fft_lean(cx<T>* v, cx<T>* lds, const cx<T>* tq, int t, bool fwd, Mid mid = {})
 {
     // a fresh copy of t per call: the twiddles are then read again here
     // rather than kept in registers from an earlier call
     asm volatile("" : "+v"(t));
     __builtin_assume(t >= 0 && t < R);
-    dft_reg<T, R>(v, fwd);
+    dft_reg<T, R, FOLD>(v, fwd);
     if (SYNC) __syncthreads();
     else __builtin_amdgcn_wave_barrier();
-    reg_fft_lean_second<T, N, R, Mid, ROW>(v, lds, tq, t, fwd, mid);
+    reg_fft_lean_second<T, N, R, Mid, ROW, FOLD>(v, lds, tq, t, fwd, mid);
 }
 
 }  // namespace sphip
