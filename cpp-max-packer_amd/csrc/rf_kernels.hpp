@@ -243,6 +243,10 @@ __device__ __forceinline__ void xt_kernel_rest(cx<T>* v, const cx<T>* __restrict
             const int x = t + P * q;
             v[q] = (live && x < kx) ? nt_load(&hcol[(long long)x * ny]) : T2 { 0, 0 };
         }
+        // the loads complete within this branch: the waits after it would
+        // otherwise count them as possibly in flight on every path
+#pragma unroll
+        for (int q = kXtAhead; q < R; ++q) asm volatile("" : "+v"(v[q].x), "+v"(v[q].y));
     }
     else {
 #pragma unroll
@@ -334,7 +338,12 @@ rf_mul_xt(const cx<T>* __restrict__ H, long long h_o, int h_lines, const cx<T>* 
         };
         auto take = [&] {
 #pragma unroll
-            for (int q = 0; q < kXtAhead; ++q) first[q] = (live && t + P * q < kxn) ? ahead[q] : T2 { 0, 0 };
+            for (int q = 0; q < kXtAhead; ++q) {
+                first[q] = (live && t + P * q < kxn) ? ahead[q] : T2 { 0, 0 };
+                // taken here, not moved past the stores that follow (the wait
+                // for the loads would then include those stores)
+                asm volatile("" : "+v"(first[q].x), "+v"(first[q].y));
+            }
         };
         fetch(i0);
         take();
