@@ -749,7 +749,7 @@ __global__ void k_lazy_init(const unsigned long long* __restrict__ orient_best, 
 // window) is no longer the minimum.  Blocks (orientation, slice): each wave
 // takes kernel rows (z, y) in turn, its lanes the row's x cells, so no cell
 // index is divided; every block that finds an overlap clears the flag.
-constexpr int kLazySlices = 8;
+constexpr int kLazySlices = 32;
 
 __global__ __launch_bounds__(256) void k_lazy_check(const u8* __restrict__ env, int ex, int ey, int ez,
                                                     const u8* __restrict__ kcells,

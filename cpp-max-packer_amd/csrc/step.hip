@@ -26,6 +26,10 @@ namespace sphip {
 namespace {
 
 constexpr int kThreads = 256;
+// Blocks of the single-pose kernels (recheck, stamp): one wave per kernel row
+// or so at typical kernel sizes, so that a row's dependent loads are not
+// waited for one after another
+constexpr int kSmallGrid = 256;
 constexpr std::size_t kMaxExcluded = 65536;
 constexpr unsigned int kCopyLogCap = 1u << 20;  // copies whose footprint a session keeps (removable copies)
 constexpr std::size_t kLdsBudget = 64 * 1024;
@@ -125,21 +129,29 @@ __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsign
     const int o = int(key & 0xffffULL);
     const int ux = int((key >> 16) & 0xffffULL), uy = int((key >> 32) & 0xffffULL), uz = int(key >> 48);
     const OrientDesc d = descs[o];
-    const long long n = (long long)d.kx * d.ky * d.kz;
     unsigned long long added = 0;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        if (!kcells[d.cells_offset + i]) continue;
-        const int x = int(i % d.kx), y = int((i / d.kx) % d.ky), z = int(i / ((long long)d.kx * d.ky));
-        const int gx = ux - shift + x, gy = uy - shift + y, gz = uz - shift + z;
-        if (gx < 0 || gy < 0 || gz < 0 || gx >= ex || gy >= ey || gz >= ez) continue;
-        const long long idx = gx + (long long)ex * (gy + (long long)ey * gz);
-        if (track.counts) atomicAdd(track.counts + idx, 1u);
-        unsigned char* c = env + idx;
-        if (*c == 0) {
-            *c = 1;
-            ++added;
-            // newly set cell, at kernel cell + margin in the update window
-            if (collect) collect[(x + mx) + (long long)cx * ((y + my) + (long long)cy * (z + mz))] = 1;
+    // each wave takes kernel rows (z, y) in turn and its lanes the row's x
+    // cells (distinct kernel cells land on distinct environment cells)
+    const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    const int rows = d.ky * d.kz;
+    for (int rw = blockIdx.x * waves + (threadIdx.x >> 6); rw < rows; rw += waves * gridDim.x) {
+        const int z = rw / d.ky, y = rw - z * d.ky;
+        const int gy = uy - shift + y, gz = uz - shift + z;
+        if (gy < 0 || gz < 0 || gy >= ey || gz >= ez) continue;
+        const unsigned char* krow = kcells + d.cells_offset + (long long)rw * d.kx;
+        const long long erow = (long long)ex * (gy + (long long)ey * gz);
+        for (int x = lane; x < d.kx; x += 64) {
+            const int gx = ux - shift + x;
+            if (!krow[x] || gx < 0 || gx >= ex) continue;
+            const long long idx = erow + gx;
+            if (track.counts) atomicAdd(track.counts + idx, 1u);
+            unsigned char* c = env + idx;
+            if (*c == 0) {
+                *c = 1;
+                ++added;
+                // newly set cell, at kernel cell + margin in the update window
+                if (collect) collect[(x + mx) + (long long)cx * ((y + my) + (long long)cy * (z + mz))] = 1;
+            }
         }
     }
     if (added) {
@@ -807,7 +819,7 @@ public:
     void enqueue_verify()
     {
         if (circular_ || real_env_) return;
-        hipLaunchKernelGGL(k_verify_choice, dim3(16), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0],
+        hipLaunchKernelGGL(k_verify_choice, dim3(kSmallGrid), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0],
                            ext_[1], ext_[2], kcells_.as<unsigned char>(), d_descs_.as<OrientDesc>(),
                            result_.as<StepDeviceResult>());
         mark("verify");
@@ -933,7 +945,7 @@ public:
     void enqueue_stamp(unsigned char* collect, int cx, int cy, unsigned long long* cmass)
     {
         const bool dil = halo_ > 0;
-        hipLaunchKernelGGL(k_stamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
+        hipLaunchKernelGGL(k_stamp, dim3(kSmallGrid), dim3(kThreads), 0, stream_, env_.as<unsigned char>(), ext_[0], ext_[1],
                            ext_[2], dil ? scells_.as<unsigned char>() : kcells_.as<unsigned char>(),
                            dil ? sdescs_.as<OrientDesc>() : d_descs_.as<OrientDesc>(), result_.as<StepDeviceResult>(),
                            track(), collect, cx, cy, 0, 0, 0, cmass, halo_);
