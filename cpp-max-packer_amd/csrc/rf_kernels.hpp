@@ -180,8 +180,11 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
 
 // The column-major product pass of the fused path (FP32, N = 256, radix 16).
 // Same work as rf_mul_x (kernel x transform, product with conj, inverse x)
-// with the output out[o][x][plane][y] (x stride (nh - 1) ny) that
-// rf_yz_argmin reads:
+// plus the environment's own x transform: the environment spectrum SB comes
+// transformed along z and y only, and each row tile transforms its rows
+// along x once for all of its orientations (the step then has no separate
+// environment x pass).  The output is out[o][x][plane][y] (x stride
+// (nh - 1) ny), which rf_yz_argmin reads:
 //  * lanes take the pencil fastest (w = lane % W rows of a tile, t = lane / W
 //    the 16 threads of a pencil), so that a store covers W consecutive y;
 //  * the kernel spectra are kept y-fastest, H[o][kz][x < h_lines][y], so a
@@ -312,12 +315,14 @@ rf_mul_xt(const cx<T>* __restrict__ H, long long h_o, int h_lines, const cx<T>* 
         // byte offset of (t, y) within an x plane group of the output (below
         // 2^31: at most 15 (nh - 1) ny + ny values)
         const unsigned int lane_off = unsigned(y + t * planes * ny) * unsigned(sizeof(T2));
+        // the tile's environment rows, transformed along x here (once per
+        // tile, for every orientation of it)
         T2 sb[R];
 #pragma unroll
-        for (int s = 0; s < R; ++s) {
-            const T2 e = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
-            sb[s] = T2 { e.x * sb_scale, e.y * sb_scale };
-        }
+        for (int s = 0; s < R; ++s) sb[s] = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
+        xfft256<T, W, 0>(sb, xl, tqf, t, true);
+#pragma unroll
+        for (int s = 0; s < R; ++s) sb[s] = T2 { sb[s].x * sb_scale, sb[s].y * sb_scale };
         // The next orientation's descriptor and the first values of its
         // kernel row are loaded while the current one is transformed.  The
         // loads are issued every iteration from clamped addresses (values
@@ -411,6 +416,11 @@ rf_mul_xt_plane0(const cx<T>* __restrict__ H, long long h_o, int h_lines, const 
     for (int h = 0; h < 2; ++h) {
         const int kz = h == 0 ? 0 : nh - 1;
         const long long row = (long long)kz * ny + y;
+        // the environment row, transformed along x here
+        T2 e[R];
+#pragma unroll
+        for (int s = 0; s < R; ++s) e[s] = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
+        xfft256<T, W, 0>(e, xl, tqf, t, true);
         T2 v[R];
         const T2* hcol = H + o * h_o + (long long)kz * h_lines * ny + y;
 #pragma unroll
@@ -422,8 +432,7 @@ rf_mul_xt_plane0(const cx<T>* __restrict__ H, long long h_o, int h_lines, const 
         xt_kernel_fft<T, W>(v, kx, xl, tqf, t);
 #pragma unroll
         for (int s = 0; s < R; ++s) {
-            const T2 e = live ? SB[row * N + t + P * s] : T2 { 0, 0 };
-            const T2 q = cmulc(T2 { e.x * sb_scale, e.y * sb_scale }, v[s]);
+            const T2 q = cmulc(T2 { e[s].x * sb_scale, e[s].y * sb_scale }, v[s]);
             p[s] = h == 0 ? q : T2 { p[s].x - q.y, p[s].y + q.x };
         }
         // the second row's loads are not issued while the first is live

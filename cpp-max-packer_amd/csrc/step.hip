@@ -1703,16 +1703,19 @@ public:
                                (long long)nx * ny, nx, ny, ny, nullptr, 0, nullptr, twy, plans_[1], W, 1);
             mark("env_c2c_y");
         }
-        // E3: C2C along x (rows, batch kz)
-        if (rf_supported(nx) && use_rf_) {
-            rf_rows_launch<T>(nx, stream_, SB, ny, (long long)nx, nh, (long long)nx * ny, twx, 1);
-            mark("env_c2c_x");
-        }
-        else {
-            const int W = choose_w(nx, e, 2, false, 16, 1);
-            hipLaunchKernelGGL((k_c2c<T>), dim3((ny + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(nx, 2, W),
-                               stream_, SB, 0LL, 1LL, (long long)nx, (long long)nx * ny, SB, 0LL, 1LL, (long long)nx,
-                               (long long)nx * ny, ny, nx, nx, nullptr, 0, nullptr, twx, plans_[0], W, 1);
+        // E3: C2C along x (rows, batch kz); the fused path's product pass
+        // transforms each row tile along x itself
+        if (!fused_yz()) {
+            if (rf_supported(nx) && use_rf_) {
+                rf_rows_launch<T>(nx, stream_, SB, ny, (long long)nx, nh, (long long)nx * ny, twx, 1);
+            }
+            else {
+                const int W = choose_w(nx, e, 2, false, 16, 1);
+                hipLaunchKernelGGL((k_c2c<T>), dim3((ny + W - 1) / W, nh, 1), dim3(kThreads), lds_bytes<T>(nx, 2, W),
+                                   stream_, SB, 0LL, 1LL, (long long)nx, (long long)nx * ny, SB, 0LL, 1LL,
+                                   (long long)nx, (long long)nx * ny, ny, nx, nx, nullptr, 0, nullptr, twx, plans_[0],
+                                   W, 1);
+            }
             mark("env_c2c_x");
         }
         const long long h2o = (long long)nh * ny * h2p_;
