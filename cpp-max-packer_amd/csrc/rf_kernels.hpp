@@ -325,10 +325,10 @@ rf_mul_xt(const cx<T>* __restrict__ H, long long h_o, int h_lines, const cx<T>* 
         for (int s = 0; s < R; ++s) sb[s] = T2 { sb[s].x * sb_scale, sb[s].y * sb_scale };
         // The next orientation's descriptor and the first values of its
         // kernel row are loaded while the current one is transformed.  The
-        // loads are issued every iteration from clamped addresses (values
-        // past kx are replaced by zero when taken), and taken after the
-        // inverse transform, before the stores: the wait for them then falls
-        // where they are the only memory operations in flight.
+        // loads use clamped addresses (values past kx are replaced by zero
+        // when taken) and are taken after the inverse transform, before the
+        // stores: the wait for them then falls where they are the only
+        // memory operations in flight.
         int on = 0, kxn = 0, lxn = 0;
         T2 ahead[kXtAhead], first[kXtAhead];
         const int yc = live ? y : ny - 1;
@@ -358,7 +358,7 @@ rf_mul_xt(const cx<T>* __restrict__ H, long long h_o, int h_lines, const cx<T>* 
 #pragma unroll
             for (int q = 0; q < kXtAhead; ++q) v[q] = first[q];
             xt_kernel_rest<T>(v, H + o * h_o + (long long)kz * h_lines * ny + y, ny, kx, live, t);
-            fetch(min(i + 1, i1 - 1));
+            if (i + 1 < i1) fetch(i + 1);  // uniform across the block
             if (lx <= 0) {  // uniform across the block
                 take();
                 continue;
