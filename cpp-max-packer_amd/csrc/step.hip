@@ -154,9 +154,12 @@ __global__ void k_stamp(unsigned char* env, int ex, int ey, int ez, const unsign
             }
         }
     }
-    if (added) {
-        atomicAdd(&r->env_mass, added);
-        if (collect_mass) atomicAdd(collect_mass, added);
+    // one atomic per wave (a wave's sum is at most the kernel's cell count)
+    unsigned int wadded = unsigned(added);
+    for (int off = 32; off > 0; off >>= 1) wadded += __shfl_xor(wadded, off, 64);
+    if (lane == 0 && wadded) {
+        atomicAdd(&r->env_mass, (unsigned long long)wadded);
+        if (collect_mass) atomicAdd(collect_mass, (unsigned long long)wadded);
     }
 }
 
