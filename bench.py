@@ -76,6 +76,48 @@ def kernel_bytes(name, n, kernels, ext, elem, fused=False):
     return 0
 
 
+def incremental_timing(sp, ctx, w, env, kernels, prec, args, ms_step=None):
+    """Incremental steps (fields updated by each step's new cells) on a
+    session from the container occupancy, timed over args.steps steps and
+    checked against full steps from the same occupancy (ms_step: the full
+    step's time, for the speedup)."""
+    import numpy as np
+    import torch
+    inc = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=prec, capture_graph=not args.no_graph)
+    inc.set_environment(env)
+    for i, k in enumerate(kernels):
+        inc.set_kernel(i, k["cells"], k["shape"], k["first"])
+    inc.set_incremental(True)
+    ref = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=prec, capture_graph=not args.no_graph)
+    ref.set_environment(env)
+    for i, k in enumerate(kernels):
+        ref.set_kernel(i, k["cells"], k["shape"], k["first"])
+    for _ in range(1 + args.warmup):
+        inc.step(stamp=True)
+        ref.step(stamp=True)
+    istream = torch.cuda.ExternalStream(inc.stream())
+    g0 = torch.cuda.Event(enable_timing=True)
+    g1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    g0.record(istream)
+    inc.run(args.steps, stamp=True)
+    g1.record(istream)
+    g1.synchronize()
+    ms_inc = g0.elapsed_time(g1) / args.steps
+    ref.run(args.steps, stamp=True)
+    same = bool(np.array_equal(inc.environment(), ref.environment())) and \
+        inc.last_result()["translation"] == ref.last_result()["translation"]
+    out = {"ms_per_step": ms_inc, "steps_per_s": 1000.0 / ms_inc,
+           "speedup_vs_full": (ms_step / ms_inc) if ms_step else None,
+           "same_occupancy_and_choice_as_full_steps": same,
+           "note": "overlap fields updated by the correlation of each step's new cells "
+                   "(gap_fill.cpp:426-451,524-584); first step full"}
+    log(f"[bench] incremental steps: {ms_inc:.3f} ms/step ({1000.0 / ms_inc:.1f} steps/s), same as full: {same}")
+    inc.close()
+    ref.close()
+    return out
+
+
 def fp_deviation(sp, ctx, w, env, kernels, bound):
     """One step from the container occupancy in FP32 and in FP64: the largest
     |FP32 - FP64| over every legal overlap value of every orientation."""
@@ -179,6 +221,8 @@ def main():
                     help="run only learned-proposal episodes (no plain step timing, no plain episodes); for profiling")
     ap.add_argument("--no-ranked", action="store_true", help="skip the ranked-page (proximity) timing")
     ap.add_argument("--only-ranked", action="store_true", help="run only the ranked-page timing; for profiling")
+    ap.add_argument("--only-incremental", action="store_true",
+                    help="run only the incremental-step timing (with its check against full steps); for profiling")
     ap.add_argument("--ranked-placements", type=int, default=10, help="greedy placements timed on ranked pages")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
@@ -225,6 +269,10 @@ def main():
     if args.only_ranked:
         print(json.dumps({"ranked_pages": ranked_timing(sp, ctx, w, env, kernels, args.precision or 0, args)}),
               flush=True)
+        return
+    if args.only_incremental:
+        prec = args.precision or (32 if w.precision == "fp32" else 0)
+        print(json.dumps({"incremental": incremental_timing(sp, ctx, w, env, kernels, prec, args)}), flush=True)
         return
     s = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=args.precision or
                    (32 if w.precision == "fp32" else 0), capture_graph=not args.no_graph)
@@ -280,37 +328,7 @@ def main():
     # session from the same starting occupancy, checked against this one
     incremental = None
     if not args.no_incremental:
-        inc = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=prec, capture_graph=not args.no_graph)
-        inc.set_environment(env)
-        for i, k in enumerate(kernels):
-            inc.set_kernel(i, k["cells"], k["shape"], k["first"])
-        inc.set_incremental(True)
-        ref = sp.Session(ctx, w.env_shape, w.env_first, len(kernels), precision=prec, capture_graph=not args.no_graph)
-        ref.set_environment(env)
-        for i, k in enumerate(kernels):
-            ref.set_kernel(i, k["cells"], k["shape"], k["first"])
-        for _ in range(1 + args.warmup):
-            inc.step(stamp=True)
-            ref.step(stamp=True)
-        istream = torch.cuda.ExternalStream(inc.stream())
-        g0 = torch.cuda.Event(enable_timing=True)
-        g1 = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        g0.record(istream)
-        inc.run(args.steps, stamp=True)
-        g1.record(istream)
-        g1.synchronize()
-        ms_inc = g0.elapsed_time(g1) / args.steps
-        ref.run(args.steps, stamp=True)
-        same = bool(np.array_equal(inc.environment(), ref.environment())) and \
-            inc.last_result()["translation"] == ref.last_result()["translation"]
-        incremental = {"ms_per_step": ms_inc, "steps_per_s": 1000.0 / ms_inc, "speedup_vs_full": ms_step / ms_inc,
-                       "same_occupancy_and_choice_as_full_steps": same,
-                       "note": "overlap fields updated by the correlation of each step's new cells "
-                               "(gap_fill.cpp:426-451,524-584); first step full"}
-        log(f"[bench] incremental steps: {ms_inc:.3f} ms/step ({1000.0 / ms_inc:.1f} steps/s), same as full: {same}")
-        inc.close()
-        ref.close()
+        incremental = incremental_timing(sp, ctx, w, env, kernels, prec, args, ms_step)
 
     ranked = None
     if not args.no_ranked:
