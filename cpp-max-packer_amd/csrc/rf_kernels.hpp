@@ -327,8 +327,8 @@ rf_mul_xt(const cx<T>* __restrict__ H, long long h_o, int h_lines, const cx<T>* 
         // kernel row are loaded while the current one is transformed.  The
         // loads use clamped addresses (values past kx are replaced by zero
         // when taken) and are taken after the inverse transform, before the
-        // stores: the wait for them then falls where they are the only
-        // memory operations in flight.
+        // stores: the wait for them then covers no store of the same
+        // iteration, only memory operations an iteration old.
         int on = 0, kxn = 0, lxn = 0;
         T2 ahead[kXtAhead], first[kXtAhead];
         const int yc = live ? y : ny - 1;
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
 // Fused inverse y + C2R along z + checks + argmin for N = ny = nz = 256 at
 // radix 16.  A block of 1024 threads owns one x column of one orientation at
 // a time: the column's half spectrum in[o][x][kz][y] (128 x 256 values,
-// written by rf_mul_x<..., true> with plane 0 holding kz 0 + i kz 128, and
+// written by rf_mul_xt with plane 0 holding kz 0 + i kz 128, and
 // already scaled by 1/(nx ny nz)) is read once, transformed along y in
 // registers, and transformed along z in two chunks of up to 128 legal rows
 // through LDS, so the inverse y output never goes back to memory.
@@ -665,7 +665,8 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
 //    the packed kz0 + i kz128 values); group g takes rows (2g, 2g + 1) of the
 //    chunk as one complex pencil through the Hermitian extension (as
 //    rf_c2r_z_argmin does for column pairs), and the real and imaginary
-//    outputs are the two rows' values at z = t + 16 s.
+//    outputs are the two rows' values at z = t + 16 s.  A wave whose eight
+//    rows of the chunk all lie past the legal range skips them.
 // The whole column is in registers between the two passes (64 VGPRs of the
 // 128 a thread has at 1024 threads), so the transforms use reg_fft_lean
 // (exchange twiddles from a [q][t] table, reads four at a time).
