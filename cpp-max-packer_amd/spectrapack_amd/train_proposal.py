@@ -42,7 +42,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--seed-base", type=int, default=1000, help="episode seeds are seed-base + 0 .. episodes - 1")
     ap.add_argument("--skip-packing", action="store_true", help="train only (no packing comparison)")
-    ap.add_argument("--out", default="gpurun_out/proposal")
+    ap.add_argument("--out", default="proposal_run", help="directory for the weights, the sample bank and the report")
     ap.add_argument("--init", default="", help="start from these weights (.npy)")
     ap.add_argument("--labels", choices=("plain", "lazy"), default="plain")
     ap.add_argument("--prior-weight", type=float, default=2.0, help="lazy packing: weight of the orientation prior")
