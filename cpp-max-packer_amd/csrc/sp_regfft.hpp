@@ -175,13 +175,70 @@ template <typename T, int R, bool FOLD = true> __device__ __forceinline__ void d
 
 template <int N, int R> struct RegShape {
     static constexpr int P = N / R;
-    static constexpr int passes = (N == R) ? 1 : (N == R * R) ? 2 : (N == R * R * R) ? 3 : (N == R * R * R * R) ? 4 : -1;
-    static_assert(passes > 0, "N must be a power of R");
+    // N = 128 at R = 16 is the mixed 16 x 8 transform (reg_fft_16x8): eight
+    // threads a pencil, sixteen values each, as for a power of R
+    static constexpr bool mixed = N == 128 && R == 16;
+    static constexpr int passes = (N == R) ? 1 : (N == R * R) ? 2 : (N == R * R * R) ? 3 : (N == R * R * R * R) ? 4
+                                : mixed ? 2 : -1;
+    static_assert(passes > 0, "N must be a power of R, or 128 at R = 16");
     // LDS elements of one pencil: one pad slot every 16 plus one, so lanes of
     // consecutive pencils and stride-16 accesses land in distinct banks.
     static constexpr int stride = N + N / 16 + 1;
     __device__ static constexpr int pad(int i) { return i + (i >> 4); }
 };
+
+// The 128-point transform as 16 x 8 (decimation in time, one exchange), in
+// the distribution of a power-of-R transform: eight threads a pencil, thread
+// t holding x[t + 8 q] (q < 16) in and X[t + 8 s] (s < 16) out.  With
+// n = n2 + 8 n1 and k = k1 + 16 k2,
+//   X[k1 + 16 k2] = sum_n2 w128^(n2 k1) w8^(n2 k2) Y[n2][k1],
+//   Y[n2][k1]     = sum_n1 w16^(n1 k1) x[n2 + 8 n1]:
+// thread t computes Y[t][.] (a radix 16 over its own values), writes it as
+// row t of the exchange, and takes the two columns k1 = t and k1 = t + 8,
+// whose twiddled radix 8s give X[t + 16 k2] and X[t + 8 + 16 k2], i.e.
+// X[t + 8 s] for even and odd s.  LEAD > 0: x[t + 8 q] is zero for
+// q >= LEAD (dft16_lead).  lds: the pencil's region (RegShape::stride
+// elements); tw: exp(-2 pi i k / 128).
+template <typename T, int LEAD, bool WAVE>
+__device__ __forceinline__ void reg_fft_16x8(cx<T>* v, cx<T>* lds, const cx<T>* tw, int t, bool fwd)
+{
+    using T2 = cx<T>;
+    using S = RegShape<128, 16>;
+    static_assert(!WAVE || 64 % S::P == 0, "a wave-local pencil must lie within one wave");
+    auto sync = [] {
+        if constexpr (WAVE) __builtin_amdgcn_wave_barrier();
+        else __syncthreads();
+    };
+    __builtin_assume(t >= 0 && t < 8);
+    if constexpr (LEAD > 0) dft16_lead<T, LEAD>(v, fwd);
+    else dft16<T>(v, fwd);
+    sync();  // earlier reads of this region are done
+#pragma unroll
+    for (int s = 0; s < 16; ++s) lds[S::pad(16 * t + s)] = v[s];
+    sync();
+    T2 a[8], b[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        a[q] = lds[S::pad(16 * q + t)];
+        b[q] = lds[S::pad(16 * q + t + 8)];
+        if (q > 0) {
+            T2 wa = tw[(q * t) & 127], wb = tw[(q * (t + 8)) & 127];
+            if (!fwd) {
+                wa.y = -wa.y;
+                wb.y = -wb.y;
+            }
+            a[q] = cmul(a[q], wa);
+            b[q] = cmul(b[q], wb);
+        }
+    }
+    Dft<T, 8>::run(a, fwd);
+    Dft<T, 8>::run(b, fwd);
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) {
+        v[2 * k2] = a[k2];
+        v[2 * k2 + 1] = b[k2];
+    }
+}
 
 // Runs the transform for one pencil.  v: in = x[t + P*q], out = X[t + P*s].
 // lds: this pencil's region (RegShape::stride elements); tw: exp(-2 pi i k/N)
@@ -198,42 +255,47 @@ __device__ __forceinline__ void reg_fft(cx<T>* v, cx<T>* lds, const cx<T>* tw, i
     using T2 = cx<T>;
     using S = RegShape<N, R>;
     static_assert(!WAVE || 64 % S::P == 0, "a wave-local pencil must lie within one wave");
-    auto sync = [] {
-        if constexpr (WAVE) __builtin_amdgcn_wave_barrier();
-        else __syncthreads();
-    };
-    // t < P lets the compiler drop the wrap of every twiddle index and fold
-    // the padded LDS offsets into constants
-    __builtin_assume(t >= 0 && t < S::P);
-    int l = 1;
+    if constexpr (S::mixed) {
+        reg_fft_16x8<T, LEAD, WAVE>(v, lds, tw, t, fwd);
+    }
+    else {
+        auto sync = [] {
+            if constexpr (WAVE) __builtin_amdgcn_wave_barrier();
+            else __syncthreads();
+        };
+        // t < P lets the compiler drop the wrap of every twiddle index and fold
+        // the padded LDS offsets into constants
+        __builtin_assume(t >= 0 && t < S::P);
+        int l = 1;
 #pragma unroll
-    for (int p = 0; p < S::passes; ++p) {
-        const int m = N / (l * R);
-        const int j = t % l, k = t / l;
-        if (p > 0) {
-            // gather this pass's inputs: src[(q*m + k)*l + j]
-            sync();
+        for (int p = 0; p < S::passes; ++p) {
+            const int m = N / (l * R);
+            const int j = t % l, k = t / l;
+            if (p > 0) {
+                // gather this pass's inputs: src[(q*m + k)*l + j]
+                sync();
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[q] = lds[S::pad((q * m + k) * l + j)];
+                for (int q = 0; q < R; ++q) v[q] = lds[S::pad((q * m + k) * l + j)];
 #pragma unroll
-            for (int q = 1; q < R; ++q) {
-                T2 w = tw[(j * q * m) & (N - 1)];
-                if (!fwd) w.y = -w.y;
-                v[q] = cmul(v[q], w);
+                for (int q = 1; q < R; ++q) {
+                    T2 w = tw[(j * q * m) & (N - 1)];
+                    if (!fwd) w.y = -w.y;
+                    v[q] = cmul(v[q], w);
+                }
             }
-        }
-        if constexpr (LEAD > 0 && R == 16) {
-            if (p == 0) dft16_lead<T, LEAD>(v, fwd);
-            else dft_reg<T, R>(v, fwd);
-        }
-        else
-            dft_reg<T, R>(v, fwd);
-        if (p + 1 < S::passes) {
-            sync();
+            if constexpr (LEAD > 0 && R == 16) {
+                if (p == 0) dft16_lead<T, LEAD>(v, fwd);
+                else dft_reg<T, R>(v, fwd);
+            }
+            else
+                dft_reg<T, R>(v, fwd);
+            if (p + 1 < S::passes) {
+                sync();
 #pragma unroll
-            for (int s = 0; s < R; ++s) lds[S::pad((k * R + s) * l + j)] = v[s];
+                for (int s = 0; s < R; ++s) lds[S::pad((k * R + s) * l + j)] = v[s];
+            }
+            l *= R;
         }
-        l *= R;
     }
 }
 

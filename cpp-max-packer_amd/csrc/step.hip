@@ -256,12 +256,13 @@ __device__ __forceinline__ unsigned long long pack_key(int ux, int uy, int uz, i
 // s < n' - (K_o - 1) and s - n' otherwise.  F_o(u_p + shift) += U_o(s) for
 // corners inside the main legal range, u_p the pending placement's corner;
 // flags orientations whose current lowest feasible corner received overlap.
-__device__ __forceinline__ int unwrap(int s, int n, int k) { return s < n - (k - 1) ? s : s - n; }
+// Only the shifts where U_o can be nonzero are visited (a >= Kp bounds the
+// new cells' extent per axis).
 
 __global__ void k_apply_update(int* __restrict__ F, const OrientDesc* __restrict__ md, const long long* __restrict__ moff,
                                const int* __restrict__ U, const OrientDesc* __restrict__ ud,
                                const long long* __restrict__ uoff, const unsigned long long* __restrict__ origin,
-                               int nxu, int nyu, int nzu, const unsigned long long* __restrict__ best,
+                               int nxu, int nyu, int nzu, int ax, int ay, int az, const unsigned long long* __restrict__ best,
                                int* __restrict__ rescan, int shift, int sign)
 {
     const int o = blockIdx.y;
@@ -270,18 +271,35 @@ __global__ void k_apply_update(int* __restrict__ F, const OrientDesc* __restrict
     const unsigned long long key = *origin;
     const int ox = int((key >> 16) & 0xffffULL) - shift, oy = int((key >> 32) & 0xffffULL) - shift,
               oz = int(key >> 48) - shift;
-    const long long n = (long long)e.lx * e.ly * e.lz;
+    // The update's support: the changed cells lie in [0, a) of the window per
+    // axis, so their correlation with this kernel is nonzero only at shifts
+    // [-(k - 1), a - 1], i.e. at circular indices [0, a) and [n - (k - 1), n).
+    // Support index j < a is shift j; j >= a is shift j - s (s = a + k - 1).
+    const int sx = min(ax + d.kx - 1, nxu), sy = min(ay + d.ky - 1, nyu), sz = min(az + d.kz - 1, nzu);
+    const int axx = min(ax, nxu), ayy = min(ay, nyu), azz = min(az, nzu);
     const int* u = U + uoff[o];
     int* f = F + moff[o];
     const unsigned long long b = best[o];
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const int v = u[i];
-        if (v == 0) continue;
-        const int tx = ox + unwrap(int(i % e.lx), nxu, d.kx), ty = oy + unwrap(int((i / e.lx) % e.ly), nyu, d.ky),
-                  tz = oz + unwrap(int(i / ((long long)e.lx * e.ly)), nzu, d.kz);
-        if (tx < 0 || ty < 0 || tz < 0 || tx >= d.lx || ty >= d.ly || tz >= d.lz) continue;
-        f[tx + (long long)d.lx * (ty + (long long)d.ly * tz)] += sign * v;
-        if (pack_key(tx, ty, tz, o) == b) rescan[o] = 1;
+    // each wave takes support rows (z, y) in turn and its lanes the row's x
+    const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    const int rows = sy * sz;
+    for (int rw = blockIdx.x * waves + (threadIdx.x >> 6); rw < rows; rw += waves * gridDim.x) {
+        const int jz = rw / sy, jy = rw - jz * sy;
+        const int dz = jz < azz ? jz : jz - sz, dy = jy < ayy ? jy : jy - sy;
+        const int tz = oz + dz, ty = oy + dy;
+        if (ty < 0 || tz < 0 || ty >= d.ly || tz >= d.lz) continue;
+        const int cz = dz < 0 ? nzu + dz : dz, cy = dy < 0 ? nyu + dy : dy;
+        const int* urow = u + (long long)e.lx * (cy + (long long)e.ly * cz);
+        int* frow = f + (long long)d.lx * (ty + (long long)d.ly * tz);
+        for (int jx = lane; jx < sx; jx += 64) {
+            const int dx = jx < axx ? jx : jx - sx;
+            const int tx = ox + dx;
+            if (tx < 0 || tx >= d.lx) continue;
+            const int v = urow[dx < 0 ? nxu + dx : dx];
+            if (v == 0) continue;
+            frow[tx] += sign * v;
+            if (pack_key(tx, ty, tz, o) == b) rescan[o] = 1;
+        }
     }
 }
 
@@ -362,7 +380,7 @@ template <typename T, int N, int R> constexpr std::size_t rf_lds(int pencils)
     return sizeof(cx<T>) * (std::size_t(N) + std::size_t(pencils) * RegShape<N, R>::stride);
 }
 
-inline bool rf_supported(int n) { return n == 256 || n == 512; }
+inline bool rf_supported(int n) { return n == 128 || n == 256 || n == 512; }
 
 // ---- tile candidates (radix R, tile width W) for the register-FFT passes ---
 // Every candidate of a list is compiled; the autotuner times the ones whose
@@ -388,20 +406,24 @@ template <typename... Ts> std::vector<Tile> tiles_for(TileList<Ts...>, int n)
 }
 
 // P1 (kernel x pass + product + inverse x), P2 (inverse y), P3 (C2R z + argmin)
-using MulTiles = TileList<TileT<256, 16, 8>, TileT<256, 16, 16>, TileT<256, 16, 32>, TileT<256, 4, 4>, TileT<256, 4, 8>,
-                          TileT<256, 4, 16>, TileT<512, 8, 2>, TileT<512, 8, 4>, TileT<512, 8, 8>, TileT<512, 8, 16>>;
-using ColTiles = TileList<TileT<256, 16, 16>, TileT<256, 16, 32>, TileT<256, 16, 64>, TileT<256, 4, 8>, TileT<256, 4, 16>,
-                          TileT<512, 8, 4>, TileT<512, 8, 8>, TileT<512, 8, 16>>;
-using C2rTiles = TileList<TileT<256, 16, 32>, TileT<256, 16, 64>, TileT<256, 16, 128>, TileT<256, 4, 16>, TileT<256, 4, 32>,
-                          TileT<512, 8, 8>, TileT<512, 8, 16>, TileT<512, 8, 32>>;
+// (128 runs at R = 16 as the mixed 16 x 8 transform, eight threads a pencil)
+using MulTiles = TileList<TileT<128, 16, 16>, TileT<128, 16, 32>, TileT<128, 16, 64>, TileT<256, 16, 8>,
+                          TileT<256, 16, 16>, TileT<256, 16, 32>, TileT<256, 4, 4>, TileT<256, 4, 8>, TileT<256, 4, 16>,
+                          TileT<512, 8, 2>, TileT<512, 8, 4>, TileT<512, 8, 8>, TileT<512, 8, 16>>;
+using ColTiles = TileList<TileT<128, 16, 32>, TileT<128, 16, 64>, TileT<128, 16, 128>, TileT<256, 16, 16>,
+                          TileT<256, 16, 32>, TileT<256, 16, 64>, TileT<256, 4, 8>, TileT<256, 4, 16>, TileT<512, 8, 4>,
+                          TileT<512, 8, 8>, TileT<512, 8, 16>>;
+using C2rTiles = TileList<TileT<128, 16, 64>, TileT<128, 16, 128>, TileT<128, 16, 256>, TileT<256, 16, 32>,
+                          TileT<256, 16, 64>, TileT<256, 16, 128>, TileT<256, 4, 16>, TileT<256, 4, 32>, TileT<512, 8, 8>,
+                          TileT<512, 8, 16>, TileT<512, 8, 32>>;
 
 enum { kPassMul = 0, kPassInvY = 1, kPassC2r = 2, kPasses = 3 };
 
 inline Tile default_tile(int pass, int n)
 {
-    if (pass == kPassMul) return n == 256 ? Tile { 16, 16 } : Tile { 8, 4 };
-    if (pass == kPassInvY) return n == 256 ? Tile { 16, 32 } : Tile { 8, 16 };
-    return n == 256 ? Tile { 16, 64 } : Tile { 8, 16 };
+    if (pass == kPassMul) return n == 128 ? Tile { 16, 32 } : n == 256 ? Tile { 16, 16 } : Tile { 8, 4 };
+    if (pass == kPassInvY) return n == 128 ? Tile { 16, 64 } : n == 256 ? Tile { 16, 32 } : Tile { 8, 16 };
+    return n == 128 ? Tile { 16, 128 } : n == 256 ? Tile { 16, 64 } : Tile { 8, 16 };
 }
 
 inline std::vector<Tile> tile_candidates(int pass, int n)
@@ -444,7 +466,12 @@ void rf_rows_launch(int n, hipStream_t st, cx<T>* data, int rows, long long row_
                     const cx<T>* tw, int fwd)
 {
     if (!rf_supported(n)) throw DeviceError { "register FFT length" };
-    if (n == 256) {
+    if (n == 128) {
+        constexpr int N = 128, R = 16, W = 32;
+        hipLaunchKernelGGL((rf_rows<T, N, R, W>), dim3((rows + W - 1) / W, batches), dim3(W * (N / R)), (rf_lds<T, N, R>(W)),
+                           st, data, rows, row_pitch, batch_pitch, tw, fwd);
+    }
+    else if (n == 256) {
         constexpr int N = 256, R = 16, W = 16;
         hipLaunchKernelGGL((rf_rows<T, N, R, W>), dim3((rows + W - 1) / W, batches), dim3(W * (N / R)), (rf_lds<T, N, R>(W)),
                            st, data, rows, row_pitch, batch_pitch, tw, fwd);
@@ -528,7 +555,13 @@ void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batche
                    const OrientDesc* descs, const int* act, const cx<T>* tw)
 {
     if (!rf_supported(n)) throw DeviceError { "register FFT length" };
-    if (n == 256) {
+    if (n == 128) {
+        constexpr int N = 128, R = 16, W = 64;
+        hipLaunchKernelGGL((rf_r2c_z<T, N, R, W, In>), dim3((cols_max + W - 1) / W, rows_max, batches),
+                           dim3((W / 2) * (N / R)), (rf_lds<T, N, R>(W / 2)), st, in, in_px, in_py, ex, ey, ez, out,
+                           out_bs, out_px, out_py, cx_, cy_, descs, act, tw);
+    }
+    else if (n == 256) {
         constexpr int N = 256, R = 16, W = 32;
         hipLaunchKernelGGL((rf_r2c_z<T, N, R, W, In>), dim3((cols_max + W - 1) / W, rows_max, batches),
                            dim3((W / 2) * (N / R)), (rf_lds<T, N, R>(W / 2)), st, in, in_px, in_py, ex, ey, ez, out,
@@ -693,6 +726,7 @@ public:
             tuned_ = false;
             drop_graphs();
         }
+        if (upd_) upd_->set_fast_path(on);  // the incremental window's passes too
     }
 
     // Restricts the next steps to a list of orientations and, optionally, to
@@ -908,8 +942,9 @@ public:
             hipLaunchKernelGGL(k_apply_update, dim3(64, O_), dim3(kThreads), 0, stream_, fields_.as<int>(),
                                d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), u.fields_.as<int>(),
                                u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(),
-                               rm_key_.as<unsigned long long>(), u.n_[0], u.n_[1], u.n_[2],
-                               orient_best_.as<unsigned long long>(), rescan_.as<int>(), halo_, -1);
+                               rm_key_.as<unsigned long long>(), u.n_[0], u.n_[1], u.n_[2], kmax_[0] + 2 * halo_,
+                               kmax_[1] + 2 * halo_, kmax_[2] + 2 * halo_, orient_best_.as<unsigned long long>(),
+                               rescan_.as<int>(), halo_, -1);
             hipLaunchKernelGGL(k_rescan_all, dim3(1), dim3(256), 0, stream_, orient_best_.as<unsigned long long>(),
                                rescan_.as<int>(), O_);
             hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
@@ -1031,8 +1066,12 @@ public:
         double vol_u = 1.0, vol_m = 1.0;
         for (int a = 0; a != 3; ++a) {
             margin_[a] = 0;
-            const int g = good_length(2 * kmax_[a] + 2 * halo_ - 1);
+            const int need = 2 * kmax_[a] + 2 * halo_ - 1;
+            int g = good_length(need);
             if (g == 0) return;
+            // past 80 points the 128-point register transform (at most 2.6x
+            // the points) beats the LDS Stockham passes at the shorter length
+            if (use_rf_ && g > 80 && need <= 128) g = 128;
             vol_u *= g;
             vol_m *= n_[a];
             d.environment_shape[a] = std::uint32_t(g);
@@ -1044,6 +1083,7 @@ public:
         d.capture_graph = desc_.capture_graph;
         upd_ = std::make_unique<Session>(ctx_, d, stream_);
         upd_->circular_ = true;
+        upd_->set_fast_path(use_rf_);
         for (int o = 0; o != O_; ++o) {
             const Kernel& k = kernels_[o];
             const std::uint32_t sh[3] = { std::uint32_t(k.shape[0]), std::uint32_t(k.shape[1]), std::uint32_t(k.shape[2]) };
@@ -1078,8 +1118,9 @@ public:
         hipLaunchKernelGGL(k_apply_update, dim3(64, O_), dim3(kThreads), 0, stream_, fields_.as<int>(),
                            d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), u.fields_.as<int>(),
                            u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(),
-                           &result_.as<StepDeviceResult>()->key, u.n_[0], u.n_[1], u.n_[2],
-                           orient_best_.as<unsigned long long>(), rescan_.as<int>(), halo_, 1);
+                           &result_.as<StepDeviceResult>()->key, u.n_[0], u.n_[1], u.n_[2], kmax_[0] + 2 * halo_,
+                           kmax_[1] + 2 * halo_, kmax_[2] + 2 * halo_, orient_best_.as<unsigned long long>(),
+                           rescan_.as<int>(), halo_, 1);
         hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
                            d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(), rescan_.as<int>());
         SP_HIP_CHECK(hipGetLastError());

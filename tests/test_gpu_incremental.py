@@ -114,3 +114,56 @@ def test_incremental_refuses_restrictions(ctx):
     s = make_session(ctx, env, first, kernels, 32)
     with pytest.raises(sp.Failure):
         s.set_active([0, 1])
+
+
+@pytest.mark.parametrize("sides", [(44, 46, 50), (47, 6, 42)])
+def test_incremental_window_128_equals_full(ctx, sides):
+    """Kernels of 41..64 cells on an axis (2 Kmax - 1 in 81..127) put that axis
+    of the update window on the 128-point register transforms (16 x 8); an
+    axis with short kernels keeps its LDS Stockham length.  On a 200^3 grid,
+    with stamped and rejected corners: every orientation's field (rejected
+    corners held blocked), the choices and the occupancy equal full steps'."""
+    rs = np.random.default_rng(sum(sides))
+    shape = (200, 200, 200)
+    first = (0, 0, 0)
+    env = (rs.uniform(0, 1, shape[::-1]) < 0.004).astype(np.uint8)
+    env[:2] = 1
+    kernels = []
+    for i in range(4):
+        s = tuple(int(v) if i == 0 else int(rs.integers(max(1, v // 2), v + 1)) for v in sides)
+        c = (rs.uniform(0, 1, s[::-1]) < 0.6).astype(np.uint8)
+        c.flat[0] = 1
+        kernels.append(dict(first=tuple(int(v) for v in rs.integers(-3, 3, 3)), shape=s, cells=c))
+    inc = make_session(ctx, env, first, kernels, 32, True)
+    full = make_session(ctx, env, first, kernels, 32, False)
+    full.keep_fields(True)
+    excluded = []
+    for step in range(10):
+        print(f"[incremental] window 128 sides={sides} step={step}", flush=True)
+        a = inc.step(stamp=False)
+        b = full.step(stamp=False)
+        assert (a["orientation"], a["translation"]) == (b["orientation"], b["translation"]), step
+        assert a["numeric_flags"] == 0
+        for i in range(len(kernels)):
+            want = full.overlap_field(i)
+            got = inc.overlap_field(i)
+            if want is None:
+                assert got is None
+                continue
+            for eo, et in excluded:
+                if eo == i:
+                    u = [int(et[k]) + kernels[i]["first"][k] - first[k] for k in range(3)]
+                    want[u[2], u[1], u[0]] += 1 << 30
+            assert np.array_equal(got, want), (step, i)
+        o = a["orientation"]
+        if o < 0:
+            break
+        t = tuple(a["translation"])
+        if step % 3 == 2:
+            excluded.append((o, t))
+            inc.exclude(o, t)
+            full.exclude(o, t)
+        else:
+            inc.stamp(o, t)
+            full.stamp(o, t)
+    assert np.array_equal(inc.environment(), full.environment())

@@ -1,4 +1,4 @@
-"""The register-FFT passes (256- and 512-point axes) against the oracle, and
+"""The register-FFT passes (128-, 256- and 512-point axes) against the oracle, and
 the full bench workload checked at scale by size-independent properties."""
 import numpy as np
 import pytest
@@ -44,6 +44,12 @@ def run_against_oracle(ctx, oracle, shape, seed, precision, steps=2, density=0.0
 @pytest.mark.parametrize("precision", [32, 64])
 def test_256_cubed(ctx, oracle, precision):
     run_against_oracle(ctx, oracle, (256, 256, 256), 1, precision, steps=2)
+
+
+@pytest.mark.parametrize("shape,precision", [((128, 128, 128), 32), ((128, 20, 128), 64), ((20, 128, 24), 32)])
+def test_128_axes(ctx, oracle, shape, precision):
+    """128-point axes: the mixed-radix (16 x 8) register transforms."""
+    run_against_oracle(ctx, oracle, shape, 3, precision, steps=2)
 
 
 @pytest.mark.parametrize("shape", [(512, 24, 512), (24, 512, 20)])
