@@ -315,22 +315,31 @@ __global__ void k_block_corner(int* __restrict__ F, const OrientDesc* __restrict
     if (pack_key(ux, uy, uz, o) == best[o]) rescan[o] = 1;
 }
 
-// One block per orientation: when flagged, the lowest (z, y, x) corner with
-// F == 0 at or after the old minimum (the feasible set only shrinks).
+// When flagged, the lowest (z, y, x) corner of an orientation with F == 0 at
+// or after its old minimum (the feasible set only shrinks).  kRescanBlocks
+// blocks share an orientation: block j takes the planes z0 + j, z0 + j + B,
+// ... in order and stops before a plane past one where a zero was already
+// found (found[o] only decreases, and each plane has one owner, so every
+// plane below the final result is scanned in full).  k_rescan_finish then
+// publishes found[o] as the minimum and resets it to ~0 for the next rescan.
+constexpr int kRescanBlocks = 8;
 __global__ __launch_bounds__(1024) void k_rescan(const int* __restrict__ F, const OrientDesc* __restrict__ md,
                                                  const long long* __restrict__ moff,
-                                                 unsigned long long* __restrict__ best, int* __restrict__ rescan)
+                                                 const unsigned long long* __restrict__ best,
+                                                 const int* __restrict__ rescan, unsigned long long* __restrict__ found)
 {
-    __shared__ unsigned long long found;
+    __shared__ int stop;
     const int o = blockIdx.x;
     if (!rescan[o]) return;
     const OrientDesc d = md[o];
     const unsigned long long b0 = best[o];
-    if (threadIdx.x == 0) found = ~0ULL;
-    __syncthreads();
     const int* f = F + moff[o];
     const long long plane = (long long)d.lx * d.ly;
-    for (int z = int(b0 >> 48); z < d.lz; ++z) {
+    for (int z = int(b0 >> 48) + int(blockIdx.y); z < d.lz; z += int(gridDim.y)) {
+        if (threadIdx.x == 0)
+            stop = int(__hip_atomic_load(&found[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 48) < z;
+        __syncthreads();
+        if (stop) break;
         unsigned long long mine = ~0ULL;
         for (long long i = threadIdx.x; i < plane; i += blockDim.x) {
             if (f[z * plane + i] != 0) continue;
@@ -341,13 +350,18 @@ __global__ __launch_bounds__(1024) void k_rescan(const int* __restrict__ F, cons
             const unsigned long long om = __shfl_down(mine, off, 64);
             mine = om < mine ? om : mine;
         }
-        if ((threadIdx.x & 63) == 0 && mine != ~0ULL) atomicMin(&found, mine);
-        __syncthreads();
-        if (found != ~0ULL) break;
-        __syncthreads();
+        if ((threadIdx.x & 63) == 0 && mine != ~0ULL) atomicMin(&found[o], mine);
+        __syncthreads();  // every wave has read `stop` before it is written again
     }
-    if (threadIdx.x == 0) {
-        best[o] = found;
+}
+
+__global__ void k_rescan_finish(unsigned long long* __restrict__ best, int* __restrict__ rescan,
+                                unsigned long long* __restrict__ found, int orientations)
+{
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < orientations; o += gridDim.x * blockDim.x) {
+        if (!rescan[o]) continue;
+        best[o] = found[o];
+        found[o] = ~0ULL;
         rescan[o] = 0;
     }
 }
@@ -501,17 +515,28 @@ inline int mul_grid(const void* fn, int threads, std::size_t lds, int cus, long 
     return int(std::max(1LL, std::min(cap, tiles * slots)));
 }
 
-// P1 over the active orientations, one block per row tile.
+// P1 over the active orientations.  One block per row tile when the tiles
+// fill at least one round of resident blocks; fewer tiles (the 128-point
+// window: 520 tiles of 128 threads) share their (tile, orientation) units
+// evenly over one round, which would otherwise leave most of each CU idle.
 template <typename T>
-void rf_mul_launch(int n, Tile tile, hipStream_t st, int ny, int nh, const cx<T>* H, long long h_o, int h_px,
-                   const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act, const cx<T>* tw)
+void rf_mul_launch(int n, Tile tile, hipStream_t st, int cus, int slots, int ny, int nh, const cx<T>* H, long long h_o,
+                   int h_px, const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
+                   const cx<T>* tw)
 {
     const bool ok = dispatch_tile(MulTiles {}, n, tile, [&](auto tt) {
         using TT = decltype(tt);
         constexpr int N = TT::n, R = TT::r, W = TT::w;
-        const int tiles = (ny + W - 1) / W * nh;
-        hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3(tiles), dim3(W * (N / R)), (rf_lds<T, N, R>(W)), st, H, h_o, h_px,
-                           SB, ny, nh, out, out_o, descs, act, tw);
+        const long long tiles = (long long)(ny + W - 1) / W * nh;
+        const int threads = W * (N / R);
+        const std::size_t lds = rf_lds<T, N, R>(W);
+        int per = 0;
+        SP_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&rf_mul_x<T, N, R, W>),
+                                                                  threads, lds));
+        const long long cap = (long long)std::max(per, 1) * std::max(cus, 1);
+        const long long grid = tiles >= cap ? tiles : std::min(cap, tiles * std::max(slots, 1));
+        hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3(unsigned(grid)), dim3(threads), lds, st, H, h_o, h_px, SB, ny, nh,
+                           out, out_o, descs, act, tw);
     });
     if (!ok) throw DeviceError { "no such FFT tile" };
 }
@@ -820,8 +845,7 @@ public:
             hipLaunchKernelGGL(k_block_corner, dim3(1), dim3(64), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
                                d_offsets_.as<long long>(), int(o), int((key >> 16) & 0xffff), int((key >> 32) & 0xffff),
                                int(key >> 48), orient_best_.as<unsigned long long>(), rescan_.as<int>());
-            hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
-                               d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(), rescan_.as<int>());
+            launch_rescan();
             SP_HIP_CHECK(hipGetLastError());
         }
         SP_HIP_CHECK(hipStreamSynchronize(stream_));
@@ -947,8 +971,7 @@ public:
                                rescan_.as<int>(), halo_, -1);
             hipLaunchKernelGGL(k_rescan_all, dim3(1), dim3(256), 0, stream_, orient_best_.as<unsigned long long>(),
                                rescan_.as<int>(), O_);
-            hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
-                               d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(), rescan_.as<int>());
+            launch_rescan();
         }
         else
             hipLaunchKernelGGL(k_unstamp, dim3(64), dim3(kThreads), 0, stream_, env_.as<unsigned char>(),
@@ -1094,6 +1117,8 @@ public:
         upd_->prepare();
         rescan_.allocate(sizeof(int) * O_);
         SP_HIP_CHECK(hipMemsetAsync(rescan_.get(), 0, sizeof(int) * O_, stream_));
+        rfound_.allocate(sizeof(unsigned long long) * O_);
+        SP_HIP_CHECK(hipMemsetAsync(rfound_.get(), 0xff, sizeof(unsigned long long) * O_, stream_));
         inc_effective_ = true;
     }
 
@@ -1121,10 +1146,19 @@ public:
                            &result_.as<StepDeviceResult>()->key, u.n_[0], u.n_[1], u.n_[2], kmax_[0] + 2 * halo_,
                            kmax_[1] + 2 * halo_, kmax_[2] + 2 * halo_, orient_best_.as<unsigned long long>(),
                            rescan_.as<int>(), halo_, 1);
-        hipLaunchKernelGGL(k_rescan, dim3(O_), dim3(1024), 0, stream_, fields_.as<int>(), d_descs_.as<OrientDesc>(),
-                           d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(), rescan_.as<int>());
+        launch_rescan();
         SP_HIP_CHECK(hipGetLastError());
         pending_ = false;
+    }
+
+    // Searches again the minima of the orientations flagged in rescan_.
+    void launch_rescan()
+    {
+        hipLaunchKernelGGL(k_rescan, dim3(O_, kRescanBlocks), dim3(1024), 0, stream_, fields_.as<int>(),
+                           d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(),
+                           rescan_.as<int>(), rfound_.as<unsigned long long>());
+        hipLaunchKernelGGL(k_rescan_finish, dim3((O_ + 255) / 256), dim3(256), 0, stream_,
+                           orient_best_.as<unsigned long long>(), rescan_.as<int>(), rfound_.as<unsigned long long>(), O_);
     }
 
     void launch_incremental(bool stamp)
@@ -1391,7 +1425,7 @@ public:
                     return;
                 }
             }
-            rf_mul_launch<T>(nx, tile, stream_, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx));
+            rf_mul_launch<T>(nx, tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx));
         }
         else if (pass == kPassInvY) {
             rf_cols_launch<T>(ny, tile, stream_, lmax_[0], nh, slots, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
@@ -2479,6 +2513,7 @@ private:
     int margin_[3] {};
     std::unique_ptr<Session> upd_;
     DeviceBuffer rescan_;
+    DeviceBuffer rfound_;   // per orientation: lowest zero key found by the running rescan (~0 between rescans)
     bool autotune_ { true }, tuned_ {};
     Tile tile_[kPasses] {};
     float tile_ms_[kPasses] {};

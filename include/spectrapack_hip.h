@@ -257,7 +257,7 @@ int sp_hip_session_overlap_field(sp_hip_session* s, uint32_t orientation, int32_
  * receives the number of kernels. */
 int sp_hip_session_profile_step(sp_hip_session* s, int stamp, float* kernel_ms, char* names, uint32_t capacity,
                                 uint32_t* count);
-/* 1 (default): register-resident FFT passes for 256- and 512-point axes;
+/* 1 (default): register-resident FFT passes for 128-, 256- and 512-point axes;
  * 0: the generic LDS Stockham passes for every axis (same results). */
 int sp_hip_session_set_fast_path(sp_hip_session* s, int on);
 /* 1 (default): on FP32 sessions over a 256^3 FFT grid whose steps keep no

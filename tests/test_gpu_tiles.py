@@ -1,6 +1,6 @@
 """Every tile the autotuner may pick gives the same overlap fields and choice
 as the default tiles (which the other step tests hold to the oracle): every
-(radix, width) of the register-FFT passes at 256- and 512-point axes and
+(radix, width) of the register-FFT passes at 128-, 256- and 512-point axes and
 every width of the generic passes at other lengths, in FP32 and FP64; and the
 autotuned session matches too."""
 import numpy as np
@@ -38,7 +38,7 @@ def run(ctx, env, kernels, precision, tiles=None, autotune=False):
     return r, fields, used
 
 
-@pytest.mark.parametrize("shape", [(256, 256, 256), (512, 256, 256), (256, 512, 256), (256, 256, 512), (96, 90, 100)])
+@pytest.mark.parametrize("shape", [(128, 128, 128), (256, 256, 256), (512, 256, 256), (256, 512, 256), (256, 256, 512), (96, 90, 100)])
 @pytest.mark.parametrize("precision", [32, 64])
 def test_every_tile_matches_the_default(ctx, shape, precision):
     import spectrapack_amd as sp
