@@ -516,10 +516,41 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_r2c_z(const In* __restri
     }
 }
 
+// An incremental update's target.  The C2R pass of a circular update window
+// (period nx, ny, N; the new cells at offsets [0, a) per axis) adds each
+// count of the new cells' correlation support into the main session's fields
+// instead of writing a window field: window index j stands for the linear
+// shift j when j < a and j - n otherwise, the support is a + k - 1 shifts
+// per axis (k the kernel's extent), and F_o(u + shift) += sign * count for
+// main corners inside the legal range, u the placement's corner (`origin`)
+// less `shift`.  An orientation whose lowest feasible corner receives a
+// count is flagged for a rescan.
+struct FieldUpdate {
+    int* F;
+    const OrientDesc* md;
+    const long long* moff;
+    const unsigned long long* origin;
+    const unsigned long long* best;
+    int* rescan;
+    int ax, ay, az;
+    int shift, sign;
+};
+
+// Window index j (period n) as the linear shift it stands for, or -2^30
+// (no corner of a grid of at most 2^16 cells per axis plus it is legal) when
+// j lies outside the support of a new-cell extent a and a kernel extent k.
+__device__ __forceinline__ int window_shift(int j, int n, int a, int k)
+{
+    const int aa = min(a, n), neg = min(a + k - 1, n) - aa;
+    return j < aa ? j : j >= n - neg ? j - n : -(1 << 30);
+}
+
 // C2R along z of column pairs + rounding, the reference's per-value checks,
 // the feasible mask and the lowest (z, y, x) key (see k_c2r_z_argmin; OUT and
-// the rounding as there).
-template <typename T, int N, int R, int W, bool OUT>
+// the rounding as there).  APPLY: the counts go into another session's
+// fields by `fu` (no key, no field output); rows outside the support are
+// skipped whole.
+template <typename T, int N, int R, int W, bool OUT, bool APPLY = false>
 __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>* __restrict__ in, long long in_o,
                                                                       int nx, int ny, const OrientDesc* __restrict__ descs,
                                                                       T inv_total, StepDeviceResult* __restrict__ result,
@@ -527,7 +558,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
                                                                       const long long* __restrict__ field_offsets,
                                                                       const unsigned long long* __restrict__ excl,
                                                                       double* __restrict__ raw, StepFilter filt,
-                                                                      const cx<T>* __restrict__ tw)
+                                                                      const cx<T>* __restrict__ tw, FieldUpdate fu = {})
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
@@ -544,6 +575,20 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
     const int y = blockIdx.y;
     const int x0 = blockIdx.x * W;
     if (d.lx <= 0 || d.ly <= 0 || d.lz <= 0 || y >= d.ly || x0 >= d.lx) return;
+    // APPLY: the main orientation, the target row and the placement's corner
+    OrientDesc m {};
+    int ty = 0, ox = 0, oz = 0;
+    int* fm = nullptr;
+    if constexpr (APPLY) {
+        m = fu.md[o];
+        if (m.lx <= 0) return;
+        const unsigned long long key = *fu.origin;
+        ox = int((key >> 16) & 0xffffULL) - fu.shift;
+        oz = int(key >> 48) - fu.shift;
+        ty = int((key >> 32) & 0xffffULL) - fu.shift + window_shift(y, ny, fu.ay, m.ky);
+        if (ty < 0 || ty >= m.ly) return;  // outside the support or the legal range: no count lands
+        fm = fu.F + fu.moff[o];
+    }
     if (threadIdx.x == 0) {
         best = ~0ULL;
         res_bits = 0;
@@ -600,6 +645,8 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
     int* fo = OUT && fields ? fields + field_offsets[o] : nullptr;
     const bool any_excluded = excl[0] != 0;
     const bool regions_on = filt.act && filt.act[1];
+    const unsigned long long b_apply = APPLY ? fu.best[o] : 0ULL;
+    bool hit = false;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
         const int z = t + P * s;
@@ -618,6 +665,18 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
                 if (fo) fo[x + (long long)d.lx * (y + (long long)d.ly * z)] = int(rv);
                 if (raw) raw[field_offsets[o] + x + (long long)d.lx * (y + (long long)d.ly * z)] = double(val);
             }
+            if constexpr (APPLY) {
+                const int tz = oz + window_shift(z, N, fu.az, m.kz);
+                const int tx = ox + window_shift(x, nx, fu.ax, m.kx);
+                const int c = int(rv);
+                if (c != 0 && tx >= 0 && tx < m.lx && tz >= 0 && tz < m.lz) {
+                    fm[tx + (long long)m.lx * (ty + (long long)m.ly * tz)] += fu.sign * c;
+                    const unsigned long long key = ((unsigned long long)tz << 48) | ((unsigned long long)ty << 32) |
+                                                   ((unsigned long long)tx << 16) | (unsigned long long)o;
+                    hit = hit || key == b_apply;
+                }
+                continue;
+            }
             if (rv == T(0)) {
                 const unsigned long long key = ((unsigned long long)z << 48) | ((unsigned long long)y << 32) |
                                                ((unsigned long long)x << 16) | (unsigned long long)o;
@@ -626,6 +685,7 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
             }
         }
     }
+    if (APPLY && hit) fu.rescan[o] = 1;
     // wave-level reduction, then one LDS atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long om = __shfl_down(mine, off, 64);

@@ -600,17 +600,28 @@ void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batche
     }
 }
 
+// Windows up to this length per axis apply their counts in the C2R pass.
+constexpr int kApplyMaxN = 256;
+
 template <typename T>
 void rf_c2r_launch(int n, Tile tile, hipStream_t st, int lx_max, int ly_max, int orients, const cx<T>* in, long long in_o,
                    int nx, int ny, const OrientDesc* descs, T inv_total, StepDeviceResult* res, int* fields,
                    const long long* offsets, const unsigned long long* excl, double* raw, StepFilter filt,
-                   const cx<T>* tw)
+                   const cx<T>* tw, const FieldUpdate* fu = nullptr)
 {
     const bool ok = dispatch_tile(C2rTiles {}, n, tile, [&](auto tt) {
         using TT = decltype(tt);
         constexpr int N = TT::n, R = TT::r, W = TT::w;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
-        if (fields || raw)
+        if (fu) {
+            if constexpr (N <= kApplyMaxN)
+                hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W, false, true>), dim3((lx_max + W - 1) / W, ly_max, orients),
+                                   dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, nullptr,
+                                   offsets, excl, nullptr, filt, tw, *fu);
+            else
+                throw DeviceError { "update window length" };
+        }
+        else if (fields || raw)
             hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W, true>), dim3((lx_max + W - 1) / W, ly_max, orients),
                                dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, fields,
                                offsets, excl, raw, filt, tw);
@@ -962,13 +973,7 @@ public:
             // the removed copy's key as the update window's origin
             rm_key_.ensure(sizeof(unsigned long long));
             SP_HIP_CHECK(hipMemcpyAsync(rm_key_.get(), &key, sizeof key, hipMemcpyHostToDevice, stream_));
-            u.launch_mode(false, false);
-            hipLaunchKernelGGL(k_apply_update, dim3(64, O_), dim3(kThreads), 0, stream_, fields_.as<int>(),
-                               d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), u.fields_.as<int>(),
-                               u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(),
-                               rm_key_.as<unsigned long long>(), u.n_[0], u.n_[1], u.n_[2], kmax_[0] + 2 * halo_,
-                               kmax_[1] + 2 * halo_, kmax_[2] + 2 * halo_, orient_best_.as<unsigned long long>(),
-                               rescan_.as<int>(), halo_, -1);
+            apply_window(rm_key_.as<unsigned long long>(), -1);
             hipLaunchKernelGGL(k_rescan_all, dim3(1), dim3(256), 0, stream_, orient_best_.as<unsigned long long>(),
                                rescan_.as<int>(), O_);
             launch_rescan();
@@ -1138,17 +1143,60 @@ public:
     // the fields and rescans orientations whose minimum was blocked.
     void apply_pending()
     {
-        Session& u = *upd_;
-        u.launch_mode(false, false);
-        hipLaunchKernelGGL(k_apply_update, dim3(64, O_), dim3(kThreads), 0, stream_, fields_.as<int>(),
-                           d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), u.fields_.as<int>(),
-                           u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(),
-                           &result_.as<StepDeviceResult>()->key, u.n_[0], u.n_[1], u.n_[2], kmax_[0] + 2 * halo_,
-                           kmax_[1] + 2 * halo_, kmax_[2] + 2 * halo_, orient_best_.as<unsigned long long>(),
-                           rescan_.as<int>(), halo_, 1);
+        apply_window(&result_.as<StepDeviceResult>()->key, 1);
         launch_rescan();
         SP_HIP_CHECK(hipGetLastError());
         pending_ = false;
+    }
+
+    // Adds sign times the update window's correlation (its occupancy holds
+    // the changed cells, placed at `origin`) into the fields: inside the
+    // window's C2R pass where that pass is the register one, else by
+    // k_apply_update from the window's own fields.
+    void apply_window(const unsigned long long* origin, int sign)
+    {
+        Session& u = *upd_;
+        const int ax = kmax_[0] + 2 * halo_, ay = kmax_[1] + 2 * halo_, az = kmax_[2] + 2 * halo_;
+        if (u.can_apply()) {
+            const FieldUpdate fu { fields_.as<int>(), d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), origin,
+                                   orient_best_.as<unsigned long long>(), rescan_.as<int>(), ax, ay, az, halo_, sign };
+            u.launch_apply(fu, sign > 0 ? 0 : 1);
+            return;
+        }
+        u.launch_mode(false, false);
+        hipLaunchKernelGGL(k_apply_update, dim3(64, O_), dim3(kThreads), 0, stream_, fields_.as<int>(),
+                           d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), u.fields_.as<int>(),
+                           u.d_descs_.as<OrientDesc>(), u.d_offsets_.as<long long>(), origin, u.n_[0], u.n_[1], u.n_[2],
+                           ax, ay, az, orient_best_.as<unsigned long long>(), rescan_.as<int>(), halo_, sign);
+    }
+
+    bool can_apply() const { return pass_uses_rf(kPassC2r) && n_[2] <= kApplyMaxN; }
+
+    // Window sessions: one step whose C2R pass adds its counts into another
+    // session's fields by `fu` (no stamp, no field output).  `variant` keys
+    // the cached graph: the pointers in `fu` are fixed per variant until the
+    // owner prepares again, which drops this session.
+    void launch_apply(const FieldUpdate& fu, int variant)
+    {
+        prepare();
+        use_device(ctx_);
+        ensure_kernel_spectra();
+        autotune();
+        apply_ = fu;
+        apply_on_ = true;
+        try {
+            if (!desc_.capture_graph) {
+                enqueue(false, false);
+                SP_HIP_CHECK(hipGetLastError());
+            }
+            else
+                run_graph(4 + variant, false, nullptr, nfit_, [&] { enqueue(false, false); });
+        }
+        catch (...) {
+            apply_on_ = false;
+            throw;
+        }
+        apply_on_ = false;
     }
 
     // Searches again the minima of the orientations flagged in rescan_.
@@ -1438,7 +1486,8 @@ public:
             rf_c2r_launch<T>(nz, tile, stream_, lmax_[0], lmax_[1], slots, TT, to, nx, ny, dd, inv_total,
                              result_.as<StepDeviceResult>(), fields_needed() ? fields_.as<int>() : nullptr,
                              d_offsets_.as<long long>(), excl_.as<unsigned long long>(),
-                             keep_raw_ ? raw_.as<double>() : nullptr, filt, twiddles<T>(nz));
+                             keep_raw_ ? raw_.as<double>() : nullptr, filt, twiddles<T>(nz),
+                             apply_on_ ? &apply_ : nullptr);
         }
     }
 
@@ -2514,6 +2563,8 @@ private:
     std::unique_ptr<Session> upd_;
     DeviceBuffer rescan_;
     DeviceBuffer rfound_;   // per orientation: lowest zero key found by the running rescan (~0 between rescans)
+    FieldUpdate apply_ {};  // set only while launch_apply enqueues
+    bool apply_on_ {};
     bool autotune_ { true }, tuned_ {};
     Tile tile_[kPasses] {};
     float tile_ms_[kPasses] {};

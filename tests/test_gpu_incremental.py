@@ -121,12 +121,14 @@ def test_incremental_window_128_equals_full(ctx, sides):
     """Kernels of 41..64 cells on an axis (2 Kmax - 1 in 81..127) put that axis
     of the update window on the 128-point register transforms (16 x 8); an
     axis with short kernels keeps its LDS Stockham length.  On a 200^3 grid,
-    with stamped and rejected corners: every orientation's field (rejected
-    corners held blocked), the choices and the occupancy equal full steps'."""
+    with stamped and rejected corners and then two removed copies: every
+    orientation's field (rejected corners held blocked), the choices and the
+    occupancy equal full steps'."""
     rs = np.random.default_rng(sum(sides))
     shape = (200, 200, 200)
     first = (0, 0, 0)
-    env = (rs.uniform(0, 1, shape[::-1]) < 0.004).astype(np.uint8)
+    # about 80 scattered cells and a floor: large kernels still find free corners
+    env = (rs.uniform(0, 1, shape[::-1]) < 1e-5).astype(np.uint8)
     env[:2] = 1
     kernels = []
     for i in range(4):
@@ -138,11 +140,9 @@ def test_incremental_window_128_equals_full(ctx, sides):
     full = make_session(ctx, env, first, kernels, 32, False)
     full.keep_fields(True)
     excluded = []
-    for step in range(10):
-        print(f"[incremental] window 128 sides={sides} step={step}", flush=True)
-        a = inc.step(stamp=False)
-        b = full.step(stamp=False)
-        assert (a["orientation"], a["translation"]) == (b["orientation"], b["translation"]), step
+
+    def compare(a, b, label):
+        assert (a["orientation"], a["translation"]) == (b["orientation"], b["translation"]), label
         assert a["numeric_flags"] == 0
         for i in range(len(kernels)):
             want = full.overlap_field(i)
@@ -154,10 +154,15 @@ def test_incremental_window_128_equals_full(ctx, sides):
                 if eo == i:
                     u = [int(et[k]) + kernels[i]["first"][k] - first[k] for k in range(3)]
                     want[u[2], u[1], u[0]] += 1 << 30
-            assert np.array_equal(got, want), (step, i)
+            assert np.array_equal(got, want), (label, i)
+
+    for step in range(10):
+        print(f"[incremental] window 128 sides={sides} step={step}", flush=True)
+        a = inc.step(stamp=False)
+        b = full.step(stamp=False)
+        compare(a, b, step)
         o = a["orientation"]
-        if o < 0:
-            break
+        assert o >= 0, step
         t = tuple(a["translation"])
         if step % 3 == 2:
             excluded.append((o, t))
@@ -166,4 +171,15 @@ def test_incremental_window_128_equals_full(ctx, sides):
         else:
             inc.stamp(o, t)
             full.stamp(o, t)
+    live = [c["copy_id"] for c in inc.copies() if not c["removed"]]
+    assert len(live) >= 3
+    for cid in (live[0], live[2]):
+        inc.remove_copy(cid)
+        full.remove_copy(cid)
+    for step in range(3):
+        print(f"[incremental] window 128 sides={sides} after removals step={step}", flush=True)
+        a = inc.step(stamp=True)
+        b = full.step(stamp=True)
+        compare(a, b, ("removed", step))
+        assert a["orientation"] >= 0
     assert np.array_equal(inc.environment(), full.environment())
