@@ -4,11 +4,12 @@ Run in a container that has the built oracle (oracle/Makefile):
 
     python tests/golden/make_fixtures.py box     # 10x6x4 box in the 64 mm box, to exhaustion
     python tests/golden/make_fixtures.py star    # 512-face star in the 128 mm box, 12 steps
+    python tests/golden/make_fixtures.py bracket # L-bracket in the r=120 mm sphere, 256^3, 96 orientations, 3 steps
 
 Each step of the plain greedy loop (engine/pack_solver/src/gap_fill.cpp:380-509:
 every orientation's legal overlap field, the lowest (z, y, x) feasible corner,
 earliest orientation on ties, then the chosen copy stamped) is computed by the
-oracle's checked FP64 FFT restatement (star) or its direct integer path (box),
+oracle's checked FP64 FFT restatement (star, bracket) or its direct integer path (box),
 orientations spread over worker processes.  A fixture holds, per step, the
 choice (orientation, translation) and the SHA-256 of every orientation's
 int32 legal field; the GPU tests recompute both and compare.
@@ -37,6 +38,7 @@ CONFIGS = {
     # name: (workload, oracle mode, steps (None: to exhaustion))
     "box": (W.box_in_box, True, None),
     "star": (W.star_in_box, False, 12),
+    "bracket": (W.bracket_in_sphere, False, 3),
 }
 
 _STATE = {}
@@ -71,13 +73,16 @@ def choice(fields, env_shape):
 def run(name, workers):
     make, direct, steps = CONFIGS[name]
     w = make()
-    env = O.box_container(w.env_first, w.env_shape, w.pitch, w.box_dims)
+    if w.container is not None:
+        env = O.mesh_container(w.container.vertices, w.container.triangles, w.env_first, w.env_shape, w.pitch)
+    else:
+        env = O.box_container(w.env_first, w.env_shape, w.pitch, w.box_dims)
     kernels = []
     for q in w.catalog:
         f, s, c = O.voxelize_object(w.part.vertices, w.part.triangles, q, w.pitch)
         kernels.append(dict(first=tuple(int(v) for v in f), shape=tuple(int(v) for v in s), cells=c))
     out = dict(workload=w.name, pitch=w.pitch, env_first=list(w.env_first), env_shape=list(w.env_shape),
-               box_dims=list(w.box_dims), orientations=len(kernels),
+               box_dims=list(w.box_dims) if w.box_dims else None, orientations=len(kernels),
                environment_sha256=cells_digest(env),
                kernels=[dict(first=list(k["first"]), shape=list(k["shape"]), sha256=cells_digest(k["cells"]))
                         for k in kernels],
@@ -103,7 +108,7 @@ def run(name, workers):
             out["steps"].append([o] + t + [digest])
             env = O.stamp(env, w.env_first, kernels[o], t)
             step += 1
-            if step % 25 == 0:
+            if step % 25 == 0 or steps is not None:
                 print(f"[{name}] step {step} ({time.time() - t0:.0f} s)", flush=True)
     out["final_environment_sha256"] = cells_digest(env)
     path = os.path.join(HERE, f"steps_{name}.json")

@@ -1,9 +1,11 @@
-"""The box and star workloads step by step against fixtures the CPU oracle
-wrote (tests/golden/make_fixtures.py): every step's choice (orientation,
-translation) and the SHA-256 of every orientation's int32 legal overlap
-field, bit-exact; the box to exhaustion (gap_fill.cpp:380-509 runs until no
-feasible placement remains).  The star also reports max |FP32 - FP64| of the
-unrounded values per step against the a-priori bound."""
+"""The box, star and bracket workloads step by step against fixtures the CPU
+oracle wrote (tests/golden/make_fixtures.py): every step's choice
+(orientation, translation) and the SHA-256 of every orientation's int32 legal
+overlap field, bit-exact; the box to exhaustion (gap_fill.cpp:380-509 runs
+until no feasible placement remains).  The star also reports max |FP32 - FP64|
+of the unrounded values per step against the a-priori bound.  The bracket
+(256^3 sphere container, 96 orientations) holds all 96 fields of its first
+steps, in FP32 and FP64, and the fused default path's choices."""
 import json
 import os
 
@@ -26,7 +28,10 @@ def setup(ctx, fx, workload, precision=0):
     import spectrapack_amd as sp
     w = workload()
     assert w.name == fx["workload"]
-    env = ctx.box_container(w.env_first, w.env_shape, w.pitch, w.box_dims)
+    if w.container is not None:
+        env = ctx.mesh_container(w.container.vertices, w.container.triangles, w.env_first, w.env_shape, w.pitch)
+    else:
+        env = ctx.box_container(w.env_first, w.env_shape, w.pitch, w.box_dims)
     assert cells_digest(env) == fx["environment_sha256"]
     kernels = ctx.voxelize_objects(w.part.vertices, w.part.triangles, w.catalog, w.pitch)
     for k, want in zip(kernels, fx["kernels"]):
@@ -40,7 +45,7 @@ def setup(ctx, fx, workload, precision=0):
 
 
 def step_digest(s, n):
-    return fields_digest([s.overlap_field(o) for o in range(n)])
+    return fields_digest(s.overlap_field(o) for o in range(n))
 
 
 @pytest.mark.parametrize("incremental", [False, True])
@@ -88,3 +93,30 @@ def test_star_steps_match_oracle_fixture_with_fp64_deviation(ctx):
     assert cells_digest(s32.environment()) == fx["final_environment_sha256"]
     s32.close()
     s64.close()
+
+
+def test_bracket_256_steps_match_oracle_fixture(ctx):
+    """The headline workload at full size: every one of the 96 orientations'
+    legal fields (about 9 M corners each) bit-exact to the oracle's checked
+    FP64 FFT, with FP32 (the bound's choice) and FP64 sessions, and the fused
+    inverse y + z path (no field output) making the same choices."""
+    from spectrapack_amd import workloads as W
+    fx = load("bracket")
+    print(f"[fixture] bracket 256^3, {len(fx['steps'])} steps", flush=True)
+    sessions = {}
+    for name, precision, keep in (("fp32", 0, True), ("fp64", 64, True), ("fused", 0, False)):
+        s, kernels = setup(ctx, fx, W.bracket_in_sphere, precision=precision)
+        s.keep_fields(keep)
+        sessions[name] = s
+    assert sessions["fused"].fused_passes()
+    for i, (o, tx, ty, tz, digest) in enumerate(fx["steps"]):
+        for name, s in sessions.items():
+            r = s.step(stamp=True)
+            print(f"[fixture] bracket step {i} {name}: orientation {r['orientation']}", flush=True)
+            assert r["orientation"] == o and tuple(r["translation"]) == (tx, ty, tz), (i, name, r)
+            assert r["precision"] == (64 if name == "fp64" else 32) and r["numeric_flags"] == 0
+            if name != "fused":
+                assert step_digest(s, len(kernels)) == digest, (i, name)
+    for s in sessions.values():
+        assert cells_digest(s.environment()) == fx["final_environment_sha256"]
+        s.close()

@@ -226,3 +226,32 @@ def test_product_pass_shares_break_inside_row_tiles(ctx):
             for o in range(len(kernels)):
                 assert got[o] == (full[o] if o in active else None), (fused, active, o)
         s.close()
+
+
+def test_row_major_product_pass_shares_on_a_small_grid(ctx, oracle):
+    """At 128^3 the row-major product pass (width 16) has 520 row tiles, fewer
+    than one round of resident blocks, so its (row tile, orientation) units
+    are shared out evenly and shares break inside row tiles.  The step's
+    choice equals the oracle's; with 1, 2, 3 and 5 active orientations every
+    active orientation keeps the lowest feasible corner it has when all run,
+    and the others report none."""
+    rs = np.random.default_rng(43)
+    kernels = kernels_with_long_axes(rs, small=6)[:6]
+    env = (rs.uniform(0, 1, (128, 128, 128)) < 0.01).astype(np.uint8)
+    env[:2] = 1
+    s = make_session(ctx, env, (0, 0, 0), kernels, False)
+    s.set_tile(0, 16, 16)
+    print("[fused] row-major shares at 128^3", flush=True)
+    r = s.step(stamp=False)
+    o, t, _ = oracle.place_step(env, (0, 0, 0), kernels, direct=True)
+    assert r["orientation"] == o and tuple(r["translation"]) == tuple(t)
+    full = s.orientation_minima()
+    assert all(m is not None for m in full)
+    for active in ([4], [0, 5], [1, 2, 3], [5, 4, 2, 1, 0]):
+        print(f"[fused] row-major shares active={active}", flush=True)
+        s.set_active(active)
+        s.step(stamp=False)
+        got = s.orientation_minima()
+        for k in range(len(kernels)):
+            assert got[k] == (full[k] if k in active else None), (active, k)
+    s.close()
