@@ -600,8 +600,10 @@ void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batche
     }
 }
 
-// Windows up to this length per axis apply their counts in the C2R pass.
-constexpr int kApplyMaxN = 256;
+// Update windows whose z axis is this long apply their counts in the C2R
+// pass (the 128-point windows that lengths 81 .. 128 round up to); others
+// write a window field that k_apply_update adds.
+constexpr int kApplyN = 128;
 
 template <typename T>
 void rf_c2r_launch(int n, Tile tile, hipStream_t st, int lx_max, int ly_max, int orients, const cx<T>* in, long long in_o,
@@ -614,7 +616,7 @@ void rf_c2r_launch(int n, Tile tile, hipStream_t st, int lx_max, int ly_max, int
         constexpr int N = TT::n, R = TT::r, W = TT::w;
         const std::size_t lds = std::max(rf_lds<T, N, R>(W / 2), sizeof(cx<T>) * (N + (N / 2 + 1) * (W + 1)));
         if (fu) {
-            if constexpr (N <= kApplyMaxN)
+            if constexpr (N == kApplyN)
                 hipLaunchKernelGGL((rf_c2r_z_argmin<T, N, R, W, false, true>), dim3((lx_max + W - 1) / W, ly_max, orients),
                                    dim3((W / 2) * (N / R)), lds, st, in, in_o, nx, ny, descs, inv_total, res, nullptr,
                                    offsets, excl, nullptr, filt, tw, *fu);
@@ -1170,7 +1172,7 @@ public:
                            ax, ay, az, orient_best_.as<unsigned long long>(), rescan_.as<int>(), halo_, sign);
     }
 
-    bool can_apply() const { return pass_uses_rf(kPassC2r) && n_[2] <= kApplyMaxN; }
+    bool can_apply() const { return pass_uses_rf(kPassC2r) && n_[2] == kApplyN; }
 
     // Window sessions: one step whose C2R pass adds its counts into another
     // session's fields by `fu` (no stamp, no field output).  `variant` keys
