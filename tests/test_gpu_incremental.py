@@ -118,9 +118,10 @@ def test_incremental_refuses_restrictions(ctx):
 
 @pytest.mark.parametrize("sides", [(44, 46, 50), (47, 6, 42)])
 def test_incremental_window_128_equals_full(ctx, sides):
-    """Kernels of 41..64 cells on an axis (2 Kmax - 1 in 81..127) put that axis
-    of the update window on the 128-point register transforms (16 x 8); an
-    axis with short kernels keeps its LDS Stockham length.  On a 200^3 grid,
+    """Kernels of 42..64 cells on an axis (2 Kmax - 1 in 83..127, good lengths
+    90..128) put that axis of the update window on the 128-point register
+    transforms (16 x 8); an axis with short kernels keeps its LDS Stockham
+    length.  On a 200^3 grid,
     with stamped and rejected corners and then two removed copies: every
     orientation's field (rejected corners held blocked), the choices and the
     occupancy equal full steps'."""
