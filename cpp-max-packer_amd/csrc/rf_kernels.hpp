@@ -15,6 +15,42 @@ template <typename T> __device__ __forceinline__ void load_tw_lds(cx<T>* dst, co
     for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = tw[i];
 }
 
+// An incremental update's target.  The C2R pass of a circular update window
+// (period nx, ny, N; the new cells at offsets [0, a) per axis) adds each
+// count of the new cells' correlation support into the main session's fields
+// instead of writing a window field: window index j stands for the linear
+// shift j when j < a and j - n otherwise, the support is a + k - 1 shifts
+// per axis (k the kernel's extent), and F_o(u + shift) += sign * count for
+// main corners inside the legal range, u the placement's corner (`origin`)
+// less `shift`.  An orientation whose lowest feasible corner receives a
+// count is flagged for a rescan.
+struct FieldUpdate {
+    int* F;
+    const OrientDesc* md;
+    const long long* moff;
+    const unsigned long long* origin;
+    const unsigned long long* best;
+    int* rescan;
+    int ax, ay, az;
+    int shift, sign;
+};
+
+// Window index j (period n) as the linear shift it stands for, or -2^30
+// (no corner of a grid of at most 2^16 cells per axis plus it is legal) when
+// j lies outside the support of a new-cell extent a and a kernel extent k.
+__device__ __forceinline__ int window_shift(int j, int n, int a, int k)
+{
+    const int aa = min(a, n), neg = min(a + k - 1, n) - aa;
+    return j < aa ? j : j >= n - neg ? j - n : -(1 << 30);
+}
+
+// An update window's index j that lies in the gap between the support's
+// two runs (the update's counts there are zero); a = 0: no gap.
+__device__ __forceinline__ bool window_gap(int j, int n, int a, int k)
+{
+    return a > 0 && window_shift(j, n, a, k) == -(1 << 30);
+}
+
 // Strided pencils (consecutive pencils contiguous in memory).  Same arguments
 // as k_c2c.  Lane mapping: pencil = lane % W (fastest), t = lane / W.
 template <typename T, int N, int R, int W>
@@ -22,7 +58,8 @@ __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long 
                                                       long long in_sb, cx<T>* out, long long out_o, long long out_sn,
                                                       long long out_sw, long long out_sb, int count_w, int n_in,
                                                       int n_out, const OrientDesc* descs, int mode, const int* act,
-                                                      const cx<T>* __restrict__ tw, int fwd)
+                                                      const cx<T>* __restrict__ tw, int fwd, int gap_ax = 0,
+                                                      int gap_ay = 0)
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
@@ -33,8 +70,11 @@ __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long 
     const int o = descs ? slot_orientation(act) : int(blockIdx.z);
     if (o < 0) return;
     int cw = count_w, ni = n_in, no = n_out;
+    int kx = 0, ky = 0;
     if (descs) {
         const OrientDesc d = descs[o];
+        kx = d.kx;
+        ky = d.ky;
         if (mode == 1) {
             if (d.lx <= 0) return;
             cw = d.kx;
@@ -48,7 +88,11 @@ __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long 
     const int p0 = blockIdx.x * W;
     if (p0 >= cw) return;
     const int w = threadIdx.x % W, t = threadIdx.x / W;
-    const bool live = p0 + w < cw;
+    // mode 2 with gaps (an update window's inverse y pass whose C2R applies
+    // the counts): columns and rows outside the update's support are
+    // neither read nor written, since that pass reads those columns as zero
+    // and skips those rows
+    const bool live = p0 + w < cw && !(mode == 2 && window_gap(p0 + w, cw, gap_ax, kx));
     const cx<T>* src = in + o * in_o + blockIdx.y * in_sb + (p0 + w) * in_sw;
     T2* dst = out + o * out_o + blockIdx.y * out_sb + (p0 + w) * out_sw;
     load_tw_lds<T>(tws, tw, N);
@@ -64,7 +108,7 @@ __global__ void __launch_bounds__(W*(N / R)) rf_cols(const cx<T>* in, long long 
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             const int n = t + P * s;
-            if (n < no) nt_store(&dst[n * out_sn], v[s]);
+            if (n < no && !(mode == 2 && window_gap(n, N, gap_ay, ky))) nt_store(&dst[n * out_sn], v[s]);
         }
     }
 }
@@ -111,7 +155,7 @@ template <typename T, int N, int R, int W>
 __global__ void __launch_bounds__(W*(N / R))
 rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __restrict__ SB, int ny, int nh,
          cx<T>* __restrict__ out, long long out_o, const OrientDesc* __restrict__ descs, const int* __restrict__ act,
-         const cx<T>* __restrict__ tw)
+         const cx<T>* __restrict__ tw, int gap_ax = 0)
 {
     using T2 = cx<T>;
     using S = RegShape<N, R>;
@@ -171,7 +215,9 @@ rf_mul_x(const cx<T>* __restrict__ H, long long h_o, int h_px, const cx<T>* __re
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const int x = t + P * s;
-                    if (x < lx) nt_store(&orow[x], v[s]);
+                    // gap_ax: an update window's columns outside the support
+                    // are not needed (see rf_cols)
+                    if (x < lx && !window_gap(x, lx, gap_ax, kx)) nt_store(&orow[x], v[s]);
                 }
             }
         }
@@ -516,35 +562,6 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_r2c_z(const In* __restri
     }
 }
 
-// An incremental update's target.  The C2R pass of a circular update window
-// (period nx, ny, N; the new cells at offsets [0, a) per axis) adds each
-// count of the new cells' correlation support into the main session's fields
-// instead of writing a window field: window index j stands for the linear
-// shift j when j < a and j - n otherwise, the support is a + k - 1 shifts
-// per axis (k the kernel's extent), and F_o(u + shift) += sign * count for
-// main corners inside the legal range, u the placement's corner (`origin`)
-// less `shift`.  An orientation whose lowest feasible corner receives a
-// count is flagged for a rescan.
-struct FieldUpdate {
-    int* F;
-    const OrientDesc* md;
-    const long long* moff;
-    const unsigned long long* origin;
-    const unsigned long long* best;
-    int* rescan;
-    int ax, ay, az;
-    int shift, sign;
-};
-
-// Window index j (period n) as the linear shift it stands for, or -2^30
-// (no corner of a grid of at most 2^16 cells per axis plus it is legal) when
-// j lies outside the support of a new-cell extent a and a kernel extent k.
-__device__ __forceinline__ int window_shift(int j, int n, int a, int k)
-{
-    const int aa = min(a, n), neg = min(a + k - 1, n) - aa;
-    return j < aa ? j : j >= n - neg ? j - n : -(1 << 30);
-}
-
 // C2R along z of column pairs + rounding, the reference's per-value checks,
 // the feasible mask and the lowest (z, y, x) key (see k_c2r_z_argmin; OUT and
 // the rounding as there).  APPLY: the counts go into another session's
@@ -614,7 +631,8 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
         constexpr int KSTEP = BLOCK / W;
         constexpr int ITER = (NH + KSTEP - 1) / KSTEP;
         const int c = threadIdx.x % W, k0 = threadIdx.x / W;
-        const bool col_in = c < lim;
+        // APPLY: columns in the gap between the support's runs read as zero
+        const bool col_in = c < lim && !(APPLY && window_gap(x0 + c, nx, fu.ax, m.kx));
         const T2* p = src + k0 * plane + c;
         const long long step = KSTEP * plane;
         T2 tmp[ITER];

@@ -463,14 +463,15 @@ template <typename T>
 void rf_cols_launch(int n, Tile tile, hipStream_t st, int count_w_max, int batches, int orients, const cx<T>* in,
                     long long in_o, long long in_sn, long long in_sw, long long in_sb, cx<T>* out, long long out_o,
                     long long out_sn, long long out_sw, long long out_sb, int count_w, int n_in, int n_out,
-                    const OrientDesc* descs, int mode, const int* act, const cx<T>* tw, int fwd)
+                    const OrientDesc* descs, int mode, const int* act, const cx<T>* tw, int fwd, int gap_ax = 0,
+                    int gap_ay = 0)
 {
     const bool ok = dispatch_tile(ColTiles {}, n, tile, [&](auto tt) {
         using TT = decltype(tt);
         constexpr int N = TT::n, R = TT::r, W = TT::w;
         hipLaunchKernelGGL((rf_cols<T, N, R, W>), dim3((count_w_max + W - 1) / W, batches, orients), dim3(W * (N / R)),
                            (rf_lds<T, N, R>(W)), st, in, in_o, in_sn, in_sw, in_sb, out, out_o, out_sn, out_sw, out_sb,
-                           count_w, n_in, n_out, descs, mode, act, tw, fwd);
+                           count_w, n_in, n_out, descs, mode, act, tw, fwd, gap_ax, gap_ay);
     });
     if (!ok) throw DeviceError { "no such FFT tile" };
 }
@@ -522,7 +523,7 @@ inline int mul_grid(const void* fn, int threads, std::size_t lds, int cus, long 
 template <typename T>
 void rf_mul_launch(int n, Tile tile, hipStream_t st, int cus, int slots, int ny, int nh, const cx<T>* H, long long h_o,
                    int h_px, const cx<T>* SB, cx<T>* out, long long out_o, const OrientDesc* descs, const int* act,
-                   const cx<T>* tw)
+                   const cx<T>* tw, int gap_ax = 0)
 {
     const bool ok = dispatch_tile(MulTiles {}, n, tile, [&](auto tt) {
         using TT = decltype(tt);
@@ -536,7 +537,7 @@ void rf_mul_launch(int n, Tile tile, hipStream_t st, int cus, int slots, int ny,
         const long long cap = (long long)std::max(per, 1) * std::max(cus, 1);
         const long long grid = tiles >= cap ? tiles : std::min(cap, tiles * std::max(slots, 1));
         hipLaunchKernelGGL((rf_mul_x<T, N, R, W>), dim3(unsigned(grid)), dim3(threads), lds, st, H, h_o, h_px, SB, ny, nh,
-                           out, out_o, descs, act, tw);
+                           out, out_o, descs, act, tw, gap_ax);
     });
     if (!ok) throw DeviceError { "no such FFT tile" };
 }
@@ -1476,11 +1477,15 @@ public:
                     return;
                 }
             }
-            rf_mul_launch<T>(nx, tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx));
+            // an update window whose C2R pass applies its counts needs no
+            // column or row outside the update's support
+            rf_mul_launch<T>(nx, tile, stream_, cus_, slots, ny, nh, H2, h2o, h2p_, SB, TT, to, dd, act, twiddles<T>(nx),
+                             apply_on_ ? apply_.ax : 0);
         }
         else if (pass == kPassInvY) {
             rf_cols_launch<T>(ny, tile, stream_, lmax_[0], nh, slots, TT, to, (long long)nx, 1LL, (long long)nx * ny, TT, to,
-                              (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, act, twiddles<T>(ny), 0);
+                              (long long)nx, 1LL, (long long)nx * ny, 0, ny, 0, dd, 2, act, twiddles<T>(ny), 0,
+                              apply_on_ ? apply_.ax : 0, apply_on_ ? apply_.ay : 0);
         }
         else {
             const T inv_total = T(1) / (T(nx) * T(ny) * T(nz));
