@@ -663,8 +663,9 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
     int* fo = OUT && fields ? fields + field_offsets[o] : nullptr;
     const bool any_excluded = excl[0] != 0;
     const bool regions_on = filt.act && filt.act[1];
-    const unsigned long long b_apply = APPLY ? fu.best[o] : 0ULL;
-    bool hit = false;
+    // APPLY: each value's count and target offset in the orientation's field
+    // (its legal box holds fewer than 2^31 corners), applied after the loop
+    int cnt[2][R] = {}, toff[2][R] = {};
 #pragma unroll
     for (int s = 0; s < R; ++s) {
         const int z = t + P * s;
@@ -687,12 +688,9 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
                 const int tz = oz + window_shift(z, N, fu.az, m.kz);
                 const int tx = ox + window_shift(x, nx, fu.ax, m.kx);
                 const int c = int(rv);
-                if (c != 0 && tx >= 0 && tx < m.lx && tz >= 0 && tz < m.lz) {
-                    fm[tx + (long long)m.lx * (ty + (long long)m.ly * tz)] += fu.sign * c;
-                    const unsigned long long key = ((unsigned long long)tz << 48) | ((unsigned long long)ty << 32) |
-                                                   ((unsigned long long)tx << 16) | (unsigned long long)o;
-                    hit = hit || key == b_apply;
-                }
+                const bool in = c != 0 && tx >= 0 && tx < m.lx && tz >= 0 && tz < m.lz;
+                cnt[half][s] = in ? c : 0;
+                toff[half][s] = in ? tx + m.lx * (ty + m.ly * tz) : 0;
                 continue;
             }
             if (rv == T(0)) {
@@ -703,7 +701,29 @@ __global__ void __launch_bounds__((W / 2) * (N / R)) rf_c2r_z_argmin(const cx<T>
             }
         }
     }
-    if (APPLY && hit) fu.rescan[o] = 1;
+    if constexpr (APPLY) {
+        // the orientation's current minimum as an offset (-1: none)
+        const unsigned long long b = fu.best[o];
+        const int boff = b == ~0ULL ? -1
+                                    : int((b >> 16) & 0xffffULL) +
+                                          m.lx * (int((b >> 32) & 0xffffULL) + m.ly * int(b >> 48));
+        bool hit = false;
+        // a half's sixteen loads are issued together, before any of its
+        // stores (each target corner receives from exactly one window value)
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            int old[R];
+#pragma unroll
+            for (int s = 0; s < R; ++s) old[s] = cnt[half][s] != 0 ? fm[toff[half][s]] : 0;
+#pragma unroll
+            for (int s = 0; s < R; ++s)
+                if (cnt[half][s] != 0) {
+                    fm[toff[half][s]] = old[s] + fu.sign * cnt[half][s];
+                    hit = hit || toff[half][s] == boff;
+                }
+        }
+        if (hit) fu.rescan[o] = 1;
+    }
     // wave-level reduction, then one LDS atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long om = __shfl_down(mine, off, 64);
