@@ -316,13 +316,16 @@ __global__ void k_block_corner(int* __restrict__ F, const OrientDesc* __restrict
 }
 
 // When flagged, the lowest (z, y, x) corner of an orientation with F == 0 at
-// or after its old minimum (the feasible set only shrinks).  kRescanBlocks
-// blocks share an orientation: block j takes the planes z0 + j, z0 + j + B,
-// ... in order and stops before a plane past one where a zero was already
-// found (found[o] only decreases, and each plane has one owner, so every
-// plane below the final result is scanned in full).  k_rescan_finish then
-// publishes found[o] as the minimum and resets it to ~0 for the next rescan.
+// or after its old minimum (the feasible set only shrinks).  Keys follow the
+// field's linear order (z-major, then y, then x), so the scan walks chunks of
+// kRescanChunk corners from the old minimum's index: kRescanBlocks blocks
+// share an orientation, block j taking chunks j, j + B, ... in order, and
+// stops before a chunk past a zero already found (found[o] only decreases
+// and each chunk has one owner, so every chunk below the final result is
+// scanned in full).  k_rescan_finish then publishes found[o] as the minimum
+// and resets it to ~0 for the next rescan.
 constexpr int kRescanBlocks = 8;
+constexpr int kRescanChunk = 8192;
 __global__ __launch_bounds__(1024) void k_rescan(const int* __restrict__ F, const OrientDesc* __restrict__ md,
                                                  const long long* __restrict__ moff,
                                                  const unsigned long long* __restrict__ best,
@@ -332,19 +335,36 @@ __global__ __launch_bounds__(1024) void k_rescan(const int* __restrict__ F, cons
     const int o = blockIdx.x;
     if (!rescan[o]) return;
     const OrientDesc d = md[o];
+    if (d.lx <= 0) return;
+    const long long plane = (long long)d.lx * d.ly, total = plane * d.lz;
+    auto index_of = [&](unsigned long long k) {
+        return (long long)(k >> 48) * plane + (long long)((k >> 32) & 0xffffULL) * d.lx + (long long)((k >> 16) & 0xffffULL);
+    };
     const unsigned long long b0 = best[o];
+    if (b0 == ~0ULL) return;  // no feasible corner was left
     const int* f = F + moff[o];
-    const long long plane = (long long)d.lx * d.ly;
-    for (int z = int(b0 >> 48) + int(blockIdx.y); z < d.lz; z += int(gridDim.y)) {
-        if (threadIdx.x == 0)
-            stop = int(__hip_atomic_load(&found[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 48) < z;
+    for (long long c0 = index_of(b0) + (long long)blockIdx.y * kRescanChunk; c0 < total;
+         c0 += (long long)gridDim.y * kRescanChunk) {
+        if (threadIdx.x == 0) {
+            const unsigned long long k = __hip_atomic_load(&found[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            stop = k != ~0ULL && index_of(k) < c0;
+        }
         __syncthreads();
         if (stop) break;
+        // a thread's first zero is its smallest
+        const long long c1 = c0 + kRescanChunk < total ? c0 + kRescanChunk : total;
+        long long first = -1;
+        for (long long i = c0 + threadIdx.x; i < c1; i += blockDim.x)
+            if (f[i] == 0) {
+                first = i;
+                break;
+            }
         unsigned long long mine = ~0ULL;
-        for (long long i = threadIdx.x; i < plane; i += blockDim.x) {
-            if (f[z * plane + i] != 0) continue;
-            const unsigned long long k = pack_key(int(i % d.lx), int(i / d.lx), z, o);
-            if (k >= b0 && k < mine) mine = k;
+        if (first >= 0) {
+            const int z = int(first / plane);
+            const long long rem = first - z * plane;
+            const int y = int(rem / d.lx);
+            mine = pack_key(int(rem - (long long)y * d.lx), y, z, o);
         }
         for (int off = 32; off > 0; off >>= 1) {
             const unsigned long long om = __shfl_down(mine, off, 64);
