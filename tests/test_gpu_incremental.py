@@ -116,9 +116,9 @@ def test_incremental_refuses_restrictions(ctx):
         s.set_active([0, 1])
 
 
-@pytest.mark.parametrize("sides", [(44, 46, 50), (47, 6, 42)])
-def test_incremental_window_128_equals_full(ctx, sides):
-    """Kernels of 42..64 cells on an axis (2 Kmax - 1 in 83..127, good lengths
+@pytest.mark.parametrize("sides,precision", [((44, 46, 50), 32), ((47, 6, 42), 32), ((44, 46, 50), 64)])
+def test_incremental_window_128_equals_full(ctx, sides, precision):
+    """FP32 and FP64.  Kernels of 42..64 cells on an axis (2 Kmax - 1 in 83..127, good lengths
     90..128) put that axis of the update window on the 128-point register
     transforms (16 x 8); an axis with short kernels keeps its LDS Stockham
     length.  On a 200^3 grid,
@@ -137,8 +137,8 @@ def test_incremental_window_128_equals_full(ctx, sides):
         c = (rs.uniform(0, 1, s[::-1]) < 0.6).astype(np.uint8)
         c.flat[0] = 1
         kernels.append(dict(first=tuple(int(v) for v in rs.integers(-3, 3, 3)), shape=s, cells=c))
-    inc = make_session(ctx, env, first, kernels, 32, True)
-    full = make_session(ctx, env, first, kernels, 32, False)
+    inc = make_session(ctx, env, first, kernels, precision, True)
+    full = make_session(ctx, env, first, kernels, precision, False)
     full.keep_fields(True)
     excluded = []
 
@@ -158,7 +158,7 @@ def test_incremental_window_128_equals_full(ctx, sides):
             assert np.array_equal(got, want), (label, i)
 
     for step in range(10):
-        print(f"[incremental] window 128 sides={sides} step={step}", flush=True)
+        print(f"[incremental] window 128 sides={sides} fp{precision} step={step}", flush=True)
         a = inc.step(stamp=False)
         b = full.step(stamp=False)
         compare(a, b, step)
@@ -178,7 +178,7 @@ def test_incremental_window_128_equals_full(ctx, sides):
         inc.remove_copy(cid)
         full.remove_copy(cid)
     for step in range(3):
-        print(f"[incremental] window 128 sides={sides} after removals step={step}", flush=True)
+        print(f"[incremental] window 128 sides={sides} fp{precision} after removals step={step}", flush=True)
         a = inc.step(stamp=True)
         b = full.step(stamp=True)
         compare(a, b, ("removed", step))
