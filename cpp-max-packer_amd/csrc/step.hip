@@ -323,27 +323,28 @@ __global__ void k_block_corner(int* __restrict__ F, const OrientDesc* __restrict
 // stops before a chunk past a zero already found (found[o] only decreases
 // and each chunk has one owner, so every chunk below the final result is
 // scanned in full).  k_rescan_finish then publishes found[o] as the minimum
-// and resets it to ~0 for the next rescan.
+// and resets it to ~0 for the next rescan; with one block per orientation
+// (gridDim.y == 1, for small fields) the block does that itself.
 constexpr int kRescanBlocks = 8;
 constexpr int kRescanChunk = 8192;
+constexpr long long kRescanWide = 1LL << 22;  // corners per orientation from which kRescanBlocks share one
 __global__ __launch_bounds__(1024) void k_rescan(const int* __restrict__ F, const OrientDesc* __restrict__ md,
                                                  const long long* __restrict__ moff,
-                                                 const unsigned long long* __restrict__ best,
-                                                 const int* __restrict__ rescan, unsigned long long* __restrict__ found)
+                                                 unsigned long long* __restrict__ best, int* __restrict__ rescan,
+                                                 unsigned long long* __restrict__ found)
 {
     __shared__ int stop;
     const int o = blockIdx.x;
     if (!rescan[o]) return;
     const OrientDesc d = md[o];
-    if (d.lx <= 0) return;
-    const long long plane = (long long)d.lx * d.ly, total = plane * d.lz;
+    const long long plane = d.lx > 0 ? (long long)d.lx * d.ly : 0, total = plane * (d.lx > 0 ? d.lz : 0);
     auto index_of = [&](unsigned long long k) {
         return (long long)(k >> 48) * plane + (long long)((k >> 32) & 0xffffULL) * d.lx + (long long)((k >> 16) & 0xffffULL);
     };
     const unsigned long long b0 = best[o];
-    if (b0 == ~0ULL) return;  // no feasible corner was left
     const int* f = F + moff[o];
-    for (long long c0 = index_of(b0) + (long long)blockIdx.y * kRescanChunk; c0 < total;
+    // (no feasible corner was left when b0 is ~0: nothing to scan)
+    for (long long c0 = b0 == ~0ULL ? total : index_of(b0) + (long long)blockIdx.y * kRescanChunk; c0 < total;
          c0 += (long long)gridDim.y * kRescanChunk) {
         if (threadIdx.x == 0) {
             const unsigned long long k = __hip_atomic_load(&found[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -372,6 +373,14 @@ __global__ __launch_bounds__(1024) void k_rescan(const int* __restrict__ F, cons
         }
         if ((threadIdx.x & 63) == 0 && mine != ~0ULL) atomicMin(&found[o], mine);
         __syncthreads();  // every wave has read `stop` before it is written again
+    }
+    if (gridDim.y == 1) {
+        __syncthreads();  // every wave's atomicMin is done
+        if (threadIdx.x == 0) {
+            best[o] = found[o];
+            found[o] = ~0ULL;
+            rescan[o] = 0;
+        }
     }
 }
 
@@ -1223,14 +1232,20 @@ public:
         apply_on_ = false;
     }
 
-    // Searches again the minima of the orientations flagged in rescan_.
+    // Searches again the minima of the orientations flagged in rescan_: over
+    // kRescanBlocks blocks per orientation and a finishing kernel for fields
+    // of kRescanWide corners or more per orientation, else one block each
+    // (the extra blocks and launch cost more than they save on small fields).
     void launch_rescan()
     {
-        hipLaunchKernelGGL(k_rescan, dim3(O_, kRescanBlocks), dim3(1024), 0, stream_, fields_.as<int>(),
+        const bool wide = fields_total_ >= (long long)std::max(O_, 1) * kRescanWide;
+        hipLaunchKernelGGL(k_rescan, dim3(O_, wide ? kRescanBlocks : 1), dim3(1024), 0, stream_, fields_.as<int>(),
                            d_descs_.as<OrientDesc>(), d_offsets_.as<long long>(), orient_best_.as<unsigned long long>(),
                            rescan_.as<int>(), rfound_.as<unsigned long long>());
-        hipLaunchKernelGGL(k_rescan_finish, dim3((O_ + 255) / 256), dim3(256), 0, stream_,
-                           orient_best_.as<unsigned long long>(), rescan_.as<int>(), rfound_.as<unsigned long long>(), O_);
+        if (wide)
+            hipLaunchKernelGGL(k_rescan_finish, dim3((O_ + 255) / 256), dim3(256), 0, stream_,
+                               orient_best_.as<unsigned long long>(), rescan_.as<int>(), rfound_.as<unsigned long long>(),
+                               O_);
     }
 
     void launch_incremental(bool stamp)
