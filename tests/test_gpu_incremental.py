@@ -116,17 +116,19 @@ def test_incremental_refuses_restrictions(ctx):
         s.set_active([0, 1])
 
 
-@pytest.mark.parametrize("sides,precision", [((44, 46, 50), 32), ((47, 6, 42), 32), ((44, 46, 50), 64)])
-def test_incremental_window_128_equals_full(ctx, sides, precision):
-    """FP32 and FP64.  Kernels of 42..64 cells on an axis (2 Kmax - 1 in 83..127, good lengths
-    90..128) put that axis of the update window on the 128-point register
-    transforms (16 x 8); an axis with short kernels keeps its LDS Stockham
-    length.  On a 200^3 grid,
+@pytest.mark.parametrize("sides,precision,side", [((44, 46, 50), 32, 200), ((47, 6, 42), 32, 200),
+                                                  ((44, 46, 50), 64, 200), ((44, 46, 50), 32, 232)])
+def test_incremental_window_128_equals_full(ctx, sides, precision, side):
+    """FP32 and FP64.  Kernels of 42..64 cells on an axis (2 Kmax - 1 in
+    83..127, good lengths 90..128) put that axis of the update window on the
+    128-point register transforms (16 x 8); an axis with short kernels keeps
+    its LDS Stockham length.  On a side^3 grid (at 232^3 the fields pass 4M
+    corners per orientation, so each rescan spreads over several blocks),
     with stamped and rejected corners and then two removed copies: every
     orientation's field (rejected corners held blocked), the choices and the
     occupancy equal full steps'."""
     rs = np.random.default_rng(sum(sides))
-    shape = (200, 200, 200)
+    shape = (side, side, side)
     first = (0, 0, 0)
     # about 80 scattered cells and a floor: large kernels still find free corners
     env = (rs.uniform(0, 1, shape[::-1]) < 1e-5).astype(np.uint8)
