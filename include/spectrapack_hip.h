@@ -258,7 +258,8 @@ int sp_hip_session_overlap_field(sp_hip_session* s, uint32_t orientation, int32_
 int sp_hip_session_profile_step(sp_hip_session* s, int stamp, float* kernel_ms, char* names, uint32_t capacity,
                                 uint32_t* count);
 /* 1 (default): register-resident FFT passes for 128-, 256- and 512-point axes;
- * 0: the generic LDS Stockham passes for every axis (same results). */
+ * 0: the generic LDS Stockham passes for every axis (same results).  A change
+ * after an incremental session's first step fails with INVALID_ARGUMENT. */
 int sp_hip_session_set_fast_path(sp_hip_session* s, int on);
 /* 1 (default): on FP32 sessions over a 256^3 FFT grid whose steps keep no
  * per-value output (no overlap fields, no raw values, the inverse y and z
